@@ -1,0 +1,295 @@
+"""Python front end of the MI355X CABARET/ARK step (``lib/libark_amd.so``).
+
+A thin ctypes layer over the C-ABI in ``include/ark/ark.h``; it mirrors the
+reference's phase-by-phase time-step API (source/main.cpp:12-17, 80-104):
+``time_step_size``, ``phase1``, ``stress_tensor``, ``use_forces``,
+``phase2``, plus the fused ``step``.  Fields are numpy arrays in the
+reference's Arr3d layout and extents (source/Arr3d.cpp:19-21,
+source/main.cpp:198-277): shape (N3+2, N2+2, N1+2) for cell fields and
+(N3+3, N2+3, N1+3) for face and stress fields, x fastest.
+
+There is no CPU fallback: every compute call runs the HIP kernels, and
+loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Dict, Iterable, Optional
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libark_amd.so")
+
+FIELDS = [
+    "roCon", "u1Con", "u2Con", "u3Con", "tCon",
+    "ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon",
+    "f1", "f2", "f3",
+    "condition",
+    "ro1", "t1", "u11", "u21", "u31", "p1",
+    "ro2", "t2", "u12", "u22", "u32", "p2",
+    "ro3", "t3", "u13", "u23", "u33", "p3",
+    "sigm11", "sigm21", "sigm31",
+    "sigm12", "sigm22", "sigm32",
+    "sigm13", "sigm23", "sigm33",
+]
+FIELD_ID = {n: q for q, n in enumerate(FIELDS)}
+FACE_FIRST = FIELD_ID["ro1"]
+
+ARK_OK, ARK_ERR_INVALID, ARK_ERR_HIP, ARK_ERR_UNSUPPORTED, ARK_ERR_NONFINITE, ARK_ERR_STATE = range(6)
+PREC_FP64, PREC_MIXED = 0, 1
+
+EXPORTED_SYMBOLS = [
+    "ark_create", "ark_destroy", "ark_last_error", "ark_upload", "ark_download", "ark_set_dt",
+    "ark_get_dt", "ark_time_step_size", "ark_phase1", "ark_stress_tensor", "ark_use_forces",
+    "ark_phase2", "ark_step", "ark_step_phases", "ark_diagnostics", "ark_init_tgv",
+    "ark_synchronize", "ark_get_stream", "ark_profile_enable", "ark_profile_read",
+    "ark_profile_reset", "ark_device_bytes",
+]
+
+
+class ArkError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"ark status {status}: {message}")
+        self.status = status
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("N1", ctypes.c_int), ("N2", ctypes.c_int), ("N3", ctypes.c_int),
+        ("dx1", ctypes.c_double), ("dx2", ctypes.c_double), ("dx3", ctypes.c_double),
+        ("CFL", ctypes.c_double), ("VIS", ctypes.c_double), ("sound", ctypes.c_double),
+        ("ro0", ctypes.c_double), ("t0", ctypes.c_double),
+        ("periodic", ctypes.c_int * 3),
+        ("precision", ctypes.c_int), ("device", ctypes.c_int),
+    ]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [
+        ("ke_reference", ctypes.c_double), ("ke_physical", ctypes.c_double),
+        ("mass", ctypes.c_double), ("max_abs_u", ctypes.c_double),
+        ("nonfinite", ctypes.c_int64), ("dt", ctypes.c_double), ("time", ctypes.c_double),
+        ("nstep", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+
+def build(force: bool = True) -> None:
+    """Compile the library for gfx950 (hipcc cross-compiles without a GPU)."""
+    args = ["make", "-s", "-C", PKG_DIR, "-j8"]
+    if force:
+        args.append("-B")
+    subprocess.run(args, check=True)
+
+
+def lib() -> ctypes.CDLL:
+    """Load libark_amd.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run __graft_entry__.build() "
+                           "(the HIP extension is required; there is no CPU path)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    L.ark_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.ark_destroy.argtypes = [vp]
+    L.ark_destroy.restype = None
+    L.ark_last_error.argtypes = [vp]
+    L.ark_last_error.restype = ctypes.c_char_p
+    L.ark_upload.argtypes = [vp, ip, dp, ip, ip, ip]
+    L.ark_download.argtypes = [vp, ip, dp, ip, ip, ip]
+    L.ark_set_dt.argtypes = [vp, ctypes.c_double]
+    L.ark_get_dt.argtypes = [vp, dp]
+    L.ark_time_step_size.argtypes = [vp, dp]
+    for fn in ("ark_phase1", "ark_stress_tensor", "ark_use_forces", "ark_phase2", "ark_synchronize",
+               "ark_profile_reset"):
+        getattr(L, fn).argtypes = [vp]
+    L.ark_step.argtypes = [vp, ip]
+    L.ark_step_phases.argtypes = [vp, ip]
+    L.ark_diagnostics.argtypes = [vp, ctypes.POINTER(Diag)]
+    L.ark_init_tgv.argtypes = [vp, ctypes.c_double]
+    L.ark_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    L.ark_profile_enable.argtypes = [vp, ip]
+    L.ark_profile_read.argtypes = [vp, ip, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), dp,
+                                   ctypes.POINTER(ctypes.c_int)]
+    L.ark_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    for fn in EXPORTED_SYMBOLS:
+        if fn not in ("ark_destroy", "ark_last_error"):
+            getattr(L, fn).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def field_shape(name: str, N) -> tuple:
+    N1, N2, N3 = N
+    e = 3 if FIELD_ID[name] >= FACE_FIRST else 2
+    return (N3 + e, N2 + e, N1 + e)
+
+
+class Solver:
+    """One rank's subdomain resident on one GPU (an ``ark_ctx``)."""
+
+    def __init__(self, N1: int, N2: int, N3: int, dx=None, CFL: float = 0.3,
+                 VIS: float = 1.0 / 1600.0, sound: float = 10.0, ro0: float = 1.0, t0: float = 1.0,
+                 precision: int = PREC_FP64, device: int = 0):
+        self.N = (int(N1), int(N2), int(N3))
+        if dx is None:
+            two_pi = 2.0 * np.pi
+            dx = ((two_pi - 0.0) / N1, (two_pi - 0.0) / N2, (two_pi - 0.0) / N3)
+        self.dx = tuple(float(v) for v in dx)
+        p = Params()
+        p.N1, p.N2, p.N3 = self.N
+        p.dx1, p.dx2, p.dx3 = self.dx
+        p.CFL, p.VIS, p.sound, p.ro0, p.t0 = CFL, VIS, sound, ro0, t0
+        p.periodic[0] = p.periodic[1] = p.periodic[2] = 1
+        p.precision, p.device = precision, device
+        self.params = p
+        self._L = lib()
+        h = ctypes.c_void_p()
+        st = self._L.ark_create(ctypes.byref(p), ctypes.byref(h))
+        self._h = h
+        if st != ARK_OK:
+            msg = self._L.ark_last_error(h).decode() if h.value else "ark_create failed"
+            self.close()
+            raise ArkError(st, msg)
+
+    # ---- plumbing -----------------------------------------------------------
+    def _check(self, st: int) -> None:
+        if st != ARK_OK:
+            raise ArkError(st, self._L.ark_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.ark_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- fields ------------------------------------------------------------
+    def shape(self, name: str) -> tuple:
+        return field_shape(name, self.N)
+
+    def upload(self, name: str, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        shp = self.shape(name)
+        if a.shape != shp:
+            raise ValueError(f"{name}: shape {a.shape} != {shp}")
+        self._check(self._L.ark_upload(self._h, FIELD_ID[name],
+                                       a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       shp[2], shp[1], shp[0]))
+
+    def download(self, name: str) -> np.ndarray:
+        shp = self.shape(name)
+        out = np.empty(shp, dtype=np.float64)
+        self._check(self._L.ark_download(self._h, FIELD_ID[name],
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         shp[2], shp[1], shp[0]))
+        return out
+
+    def upload_all(self, arrays: Dict[str, np.ndarray], names: Optional[Iterable[str]] = None) -> None:
+        for n in (names or FIELDS):
+            if n in arrays:
+                self.upload(n, arrays[n])
+
+    def download_all(self, names: Optional[Iterable[str]] = None) -> Dict[str, np.ndarray]:
+        return {n: self.download(n) for n in (names or FIELDS)}
+
+    @property
+    def dt(self) -> float:
+        v = ctypes.c_double()
+        self._check(self._L.ark_get_dt(self._h, ctypes.byref(v)))
+        return v.value
+
+    @dt.setter
+    def dt(self, value: float) -> None:
+        self._check(self._L.ark_set_dt(self._h, float(value)))
+
+    # ---- the reference's phases (source/main.cpp:374-2495) --------------------
+    def time_step_size(self) -> float:
+        v = ctypes.c_double()
+        self._check(self._L.ark_time_step_size(self._h, ctypes.byref(v)))
+        return v.value
+
+    def phase1(self) -> None:
+        self._check(self._L.ark_phase1(self._h))
+
+    def stress_tensor(self) -> None:
+        self._check(self._L.ark_stress_tensor(self._h))
+
+    def use_forces(self) -> None:
+        self._check(self._L.ark_use_forces(self._h))
+
+    def phase2(self) -> None:
+        self._check(self._L.ark_phase2(self._h))
+
+    def phase(self, name: str) -> None:
+        {"TimeStepSize": self.time_step_size, "Phase1": self.phase1,
+         "StressTensor": self.stress_tensor, "UseForces": self.use_forces,
+         "Phase2": self.phase2, "Step": lambda: self.step(1)}[name]()
+
+    def step(self, nsteps: int = 1) -> None:
+        """nsteps of the driver loop (source/main.cpp:82-91), fused kernels."""
+        self._check(self._L.ark_step(self._h, int(nsteps)))
+
+    def step_phases(self, nsteps: int = 1) -> None:
+        """nsteps through one launch per reference phase."""
+        self._check(self._L.ark_step_phases(self._h, int(nsteps)))
+
+    def init_tgv(self, U0: float = 1.0) -> None:
+        self._check(self._L.ark_init_tgv(self._h, float(U0)))
+
+    def diagnostics(self, allow_nonfinite: bool = False) -> Dict[str, float]:
+        d = Diag()
+        st = self._L.ark_diagnostics(self._h, ctypes.byref(d))
+        if st != ARK_OK and not (allow_nonfinite and st == ARK_ERR_NONFINITE):
+            self._check(st)
+        return {k: getattr(d, k) for k, _ in Diag._fields_}
+
+    def synchronize(self) -> None:
+        self._check(self._L.ark_synchronize(self._h))
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._check(self._L.ark_get_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    def device_bytes(self) -> int:
+        v = ctypes.c_int64()
+        self._check(self._L.ark_device_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    # ---- per-kernel timing (HIP events on the context's stream) ---------------
+    def profile(self, on: bool = True) -> None:
+        self._check(self._L.ark_profile_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self) -> None:
+        self._check(self._L.ark_profile_reset(self._h))
+
+    def profile_read(self) -> Dict[str, tuple]:
+        n = ctypes.c_int()
+        cap = 64
+        names = (ctypes.c_char * 64 * cap)()
+        launches = (ctypes.c_int64 * cap)()
+        total = (ctypes.c_double * cap)()
+        self._check(self._L.ark_profile_read(self._h, cap, ctypes.cast(names, ctypes.c_void_p), launches,
+                                             total, ctypes.byref(n)))
+        out = {}
+        for q in range(min(n.value, cap)):
+            out[bytes(names[q]).split(b"\0", 1)[0].decode()] = (int(launches[q]), float(total[q]))
+        return out

@@ -1,0 +1,559 @@
+// C-ABI of the MI355X CABARET/ARK step (include/ark/ark.h): context, device
+// memory in the padded SoA layout, per-phase entry points and the step
+// drivers.  Everything runs on the context's one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ark/ark.h"
+#include "ark_internal.hpp"
+#include "ark_kernels.hpp"
+
+using namespace arkd;
+
+namespace {
+
+constexpr int kOriginPad = 15;  // element (0,0,0) sits 15 elements into the allocation
+
+bool is_face_field(int f) { return f >= F_ro1; }
+// The eighteen persistent face fields are double-buffered (Phase2 reads the
+// old faces and writes the new ones).
+bool is_face_state(int f) { return f >= F_ro1 && f <= F_p3; }
+
+const char* kFieldNames[F_NUM] = {
+    "roCon", "u1Con", "u2Con", "u3Con", "tCon", "ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon",
+    "f1", "f2", "f3", "condition",
+    "ro1", "t1", "u11", "u21", "u31", "p1", "ro2", "t2", "u12", "u22", "u32", "p2",
+    "ro3", "t3", "u13", "u23", "u33", "p3",
+    "sigm11", "sigm21", "sigm31", "sigm12", "sigm22", "sigm32", "sigm13", "sigm23", "sigm33"};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+ark_status ark_ctx::fail(ark_status s, const std::string& msg) {
+    err = msg;
+    return s;
+}
+
+ark_status ark_ctx::hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return ARK_OK;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return ARK_ERR_HIP;
+}
+
+ark_status ark_ctx::launch(const char* name, hipError_t (*fn)(void*), void* arg) {
+    if (!profiling) return hip(fn(arg), name);
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return fail(ARK_ERR_HIP, "hipEventCreate");
+    hipEventRecord(a, stream);
+    hipError_t e = fn(arg);
+    hipEventRecord(b, stream);
+    prof_events.push_back({name, a, b});
+    return hip(e, name);
+}
+
+Fields ark_ctx::fields() const {
+    Fields F;
+    for (int f = 0; f < F_NUM; ++f) F.a[f] = cur[f];
+    return F;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+ark_status ark_create(const ark_params* p, ark_ctx** out) {
+    if (!p || !out) return ARK_ERR_INVALID;
+    *out = nullptr;
+    ark_ctx* c = new ark_ctx();
+    c->prm = *p;
+    if (p->N1 < 1 || p->N2 < 1 || p->N3 < 1) {
+        c->err = "N1, N2, N3 must be >= 1";
+        *out = c;
+        return ARK_ERR_INVALID;
+    }
+    if (!(p->periodic[0] && p->periodic[1] && p->periodic[2])) {
+        c->err = "only the triply periodic box is implemented (physical boundaries are not)";
+        *out = c;
+        return ARK_ERR_UNSUPPORTED;
+    }
+    if (p->precision != ARK_PREC_FP64 && p->precision != ARK_PREC_MIXED) {
+        c->err = "precision must be ARK_PREC_FP64 or ARK_PREC_MIXED";
+        *out = c;
+        return ARK_ERR_INVALID;
+    }
+    if (!(p->dx1 > 0 && p->dx2 > 0 && p->dx3 > 0 && p->sound > 0 && p->CFL > 0 && p->VIS >= 0)) {
+        c->err = "dx, sound and CFL must be > 0, VIS >= 0";
+        *out = c;
+        return ARK_ERR_INVALID;
+    }
+    Geo& g = c->geo;
+    g.N1 = p->N1; g.N2 = p->N2; g.N3 = p->N3;
+    long long px = ((long long)p->N1 + 3 + 15) / 16 * 16;
+    long long elems = px * (p->N2 + 3) * (long long)(p->N3 + 3);
+    if (elems + kOriginPad >= (1ll << 31)) {
+        c->err = "grid too large for 32-bit indexing";
+        *out = c;
+        return ARK_ERR_UNSUPPORTED;
+    }
+    g.px = (int)px;
+    g.sxy = (int)(px * (p->N2 + 3));
+    c->elems = elems + kOriginPad + 16;
+
+    Consts& K = c->K;
+    K.dx[0] = p->dx1; K.dx[1] = p->dx2; K.dx[2] = p->dx3;
+    K.ds1 = p->dx2 * p->dx3;            // source/main.cpp:455-460
+    K.ds2 = p->dx1 * p->dx3;
+    K.ds3 = p->dx1 * p->dx2;
+    K.dvc = p->dx1 * p->dx2 * p->dx3;
+    K.VIS = p->VIS; K.sound = p->sound; K.ro0 = p->ro0; K.t0 = p->t0;
+
+    ark_status s = c->hip(hipSetDevice(p->device), "hipSetDevice");
+    if (s == ARK_OK) s = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (int f = 0; f < F_NUM && s == ARK_OK; ++f) {
+        int nbuf = is_face_state(f) ? 2 : 1;
+        for (int b = 0; b < nbuf && s == ARK_OK; ++b) {
+            double* mem = nullptr;
+            s = c->hip(hipMalloc(&mem, c->elems * sizeof(double)), "hipMalloc field");
+            if (s != ARK_OK) break;
+            c->allocs.push_back(mem);
+            c->bytes += c->elems * sizeof(double);
+            s = c->hip(hipMemsetAsync(mem, 0, c->elems * sizeof(double), c->stream), "hipMemset field");
+            if (b == 0) c->cur[f] = mem + kOriginPad;
+            else c->alt[f] = mem + kOriginPad;
+        }
+    }
+    if (s == ARK_OK) {
+        s = c->hip(hipMalloc(&c->S, sizeof(Scalars)), "hipMalloc scalars");
+        if (s == ARK_OK) c->allocs.push_back(c->S);
+    }
+    if (s == ARK_OK) {
+        c->diag_blocks = 1024;
+        s = c->hip(hipMalloc(&c->diag_part, (5 * c->diag_blocks + 8) * sizeof(double)), "hipMalloc diag");
+        if (s == ARK_OK) c->allocs.push_back(c->diag_part);
+    }
+    if (s == ARK_OK) {
+        // dt starts from min(1e8, dtv), the constant part of the reference's
+        // running min3d(dt, dtu, dtv) (source/main.cpp:377, 393-405).
+        double dtv;
+        if (p->VIS > pow(10, -16)) {
+            double dtv1 = p->CFL * p->dx1 * p->dx1 / (2. * p->VIS);
+            double dtv2 = p->CFL * p->dx2 * p->dx2 / (2. * p->VIS);
+            double dtv3 = p->CFL * p->dx3 * p->dx3 / (2. * p->VIS);
+            dtv = dtv1;
+            dtv = dtv > dtv2 ? dtv2 : dtv;
+            dtv = dtv > dtv3 ? dtv3 : dtv;
+        } else {
+            dtv = pow(10, 16);
+        }
+        double d0 = pow(10, 8);
+        d0 = d0 > dtv ? dtv : d0;
+        Scalars h{};
+        h.dt = 0.0;
+        h.time = 0.0;
+        h.nstep = 0;
+        memcpy(&h.dt_init, &d0, 8);
+        h.dt_bits = h.dt_init;
+        h.nonfinite = 0;
+        s = c->hip(hipMemcpyAsync(c->S, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "upload scalars");
+        if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    }
+    *out = c;
+    return s;
+}
+
+void ark_destroy(ark_ctx* c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& e : c->prof_events) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    for (void* m : c->allocs) hipFree(m);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* ark_last_error(const ark_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+static ark_status check_extents(ark_ctx* c, int field, int e1, int e2, int e3) {
+    if (field < 0 || field >= F_NUM) return c->fail(ARK_ERR_INVALID, "field id out of range");
+    int add = is_face_field(field) ? 3 : 2;
+    if (e1 != c->geo.N1 + add || e2 != c->geo.N2 + add || e3 != c->geo.N3 + add) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "%s: extents %dx%dx%d, expected %dx%dx%d", kFieldNames[field], e1, e2, e3,
+                 c->geo.N1 + add, c->geo.N2 + add, c->geo.N3 + add);
+        return c->fail(ARK_ERR_INVALID, buf);
+    }
+    return ARK_OK;
+}
+
+static ark_status copy_planes(ark_ctx* c, double* dev, const double* host_src, double* host_dst, int e1, int e2,
+                              int e3) {
+    for (int k = 0; k < e3; ++k) {
+        hipError_t e;
+        if (host_src)
+            e = hipMemcpy2DAsync(dev + (size_t)k * c->geo.sxy, c->geo.px * sizeof(double),
+                                 host_src + (size_t)k * e1 * e2, e1 * sizeof(double), e1 * sizeof(double), e2,
+                                 hipMemcpyHostToDevice, c->stream);
+        else
+            e = hipMemcpy2DAsync(host_dst + (size_t)k * e1 * e2, e1 * sizeof(double), dev + (size_t)k * c->geo.sxy,
+                                 c->geo.px * sizeof(double), e1 * sizeof(double), e2, hipMemcpyDeviceToHost,
+                                 c->stream);
+        if (e != hipSuccess) return c->hip(e, "hipMemcpy2DAsync");
+    }
+    return c->hip(hipStreamSynchronize(c->stream), "sync after copy");
+}
+
+ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2, int e3) {
+    if (!c || !host) return ARK_ERR_INVALID;
+    ark_status s = check_extents(c, field, e1, e2, e3);
+    if (s != ARK_OK) return s;
+    if (field == F_condition) {
+        long long n = (long long)e1 * e2 * e3;
+        c->has_solid = false;
+        for (long long q = 0; q < n; ++q)
+            if (host[q] != 0.0) { c->has_solid = true; break; }
+    }
+    s = copy_planes(c, c->cur[field], host, nullptr, e1, e2, e3);
+    if (s == ARK_OK && c->alt[field]) s = copy_planes(c, c->alt[field], host, nullptr, e1, e2, e3);
+    return s;
+}
+
+ark_status ark_download(ark_ctx* c, int field, double* host, int e1, int e2, int e3) {
+    if (!c || !host) return ARK_ERR_INVALID;
+    ark_status s = check_extents(c, field, e1, e2, e3);
+    if (s != ARK_OK) return s;
+    s = c->hip(hipStreamSynchronize(c->stream), "sync before download");
+    if (s != ARK_OK) return s;
+    return copy_planes(c, c->cur[field], nullptr, host, e1, e2, e3);
+}
+
+ark_status ark_set_dt(ark_ctx* c, double dt) {
+    if (!c) return ARK_ERR_INVALID;
+    c->host_dt = dt;
+    ark_status s = c->hip(hipMemcpyAsync(&c->S->dt, &c->host_dt, sizeof(double), hipMemcpyHostToDevice, c->stream),
+                          "set dt");
+    return s == ARK_OK ? c->hip(hipStreamSynchronize(c->stream), "sync") : s;
+}
+
+ark_status ark_get_dt(ark_ctx* c, double* dt) {
+    if (!c || !dt) return ARK_ERR_INVALID;
+    ark_status s = c->hip(hipMemcpyAsync(&c->host_dt, &c->S->dt, sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                          "get dt");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s == ARK_OK) *dt = c->host_dt;
+    return s;
+}
+
+// ---- phases ----------------------------------------------------------------
+namespace {
+
+struct CflArgs { ark_ctx* c; };
+hipError_t run_cfl(void* p) {
+    ark_ctx* c = ((CflArgs*)p)->c;
+    const ark_params& P = c->prm;
+    return launch_cfl(c->stream, c->geo, c->fields(), P.CFL * P.dx1, P.CFL * P.dx2, P.CFL * P.dx3, P.sound, c->S);
+}
+struct FinArgs { ark_ctx* c; int advance; };
+hipError_t run_cfl_fin(void* p) {
+    FinArgs* a = (FinArgs*)p;
+    return launch_cfl_finalize(a->c->stream, a->c->S, a->advance);
+}
+hipError_t run_phase1(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_phase1(c->stream, c->geo, c->fields(), c->K, c->S);
+}
+hipError_t run_stress_ghost(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    GhostPlan plan{};
+    const int cells[3] = {F_u1Con, F_u2Con, F_u3Con};
+    for (int q = 0; q < 3; ++q) {
+        plan.p[plan.n] = c->cur[cells[q]];
+        plan.is_face[plan.n] = 0;
+        plan.axis_mask[plan.n] = 7;
+        plan.n++;
+    }
+    return launch_ghost(c->stream, c->geo, plan);
+}
+hipError_t run_stress_faces(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_stress_faces(c->stream, c->geo, c->fields(), c->K);
+}
+hipError_t run_forces(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_forces(c->stream, c->geo, c->fields(), c->K);
+}
+hipError_t run_use_forces(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_use_forces(c->stream, c->geo, c->fields(), c->K, c->S);
+}
+hipError_t run_phase2_ghost(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    GhostPlan plan{};
+    const int cells[7] = {F_u1nCon, F_u2nCon, F_u3nCon, F_roCon, F_ronCon, F_tCon, F_tnCon};
+    for (int q = 0; q < 7; ++q) {
+        plan.p[plan.n] = c->cur[cells[q]];
+        plan.is_face[plan.n] = 0;
+        plan.axis_mask[plan.n] = 7;
+        plan.n++;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const int faces[5] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_ro(d), face_p(d)};
+        for (int q = 0; q < 5; ++q) {
+            plan.p[plan.n] = c->cur[faces[q]];
+            plan.is_face[plan.n] = 1;
+            plan.axis_mask[plan.n] = 1 << d;
+            plan.n++;
+        }
+    }
+    return launch_ghost(c->stream, c->geo, plan);
+}
+struct P2Args { ark_ctx* c; int axis; };
+hipError_t run_phase2_axis(void* p) {
+    P2Args* a = (P2Args*)p;
+    ark_ctx* c = a->c;
+    int d = a->axis;
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    NextFaces nx;
+    nx.p[0] = c->alt[face_u(d, d)];
+    nx.p[1] = c->alt[face_u(d, ta)];
+    nx.p[2] = c->alt[face_u(d, tb)];
+    nx.p[3] = c->alt[face_p(d)];
+    nx.p[4] = c->alt[face_ro(d)];
+    nx.p[5] = c->alt[face_t(d)];
+    return launch_phase2(c->stream, d, c->geo, c->fields(), nx, c->K, c->S);
+}
+
+}  // namespace
+
+static ark_status require_fluid(ark_ctx* c) {
+    if (c->has_solid)
+        return c->fail(ARK_ERR_UNSUPPORTED,
+                       "solid cells (condition != 0) are not implemented on the GPU path; the periodic box must be "
+                       "all fluid");
+    return ARK_OK;
+}
+
+static ark_status time_step_size(ark_ctx* c, int advance) {
+    CflArgs a{c};
+    ark_status s = c->launch("cfl", run_cfl, &a);
+    FinArgs f{c, advance};
+    if (s == ARK_OK) s = c->launch("cfl_finalize", run_cfl_fin, &f);
+    return s;
+}
+
+ark_status ark_time_step_size(ark_ctx* c, double* dt_out) {
+    if (!c) return ARK_ERR_INVALID;
+    ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = time_step_size(c, 0);
+    if (s == ARK_OK && dt_out) s = ark_get_dt(c, dt_out);
+    return s;
+}
+
+ark_status ark_phase1(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    ark_status s = require_fluid(c);
+    return s == ARK_OK ? c->launch("phase1", run_phase1, c) : s;
+}
+
+ark_status ark_stress_tensor(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = c->launch("ghost_stress", run_stress_ghost, c);
+    if (s == ARK_OK) s = c->launch("stress_faces", run_stress_faces, c);
+    if (s == ARK_OK) s = c->launch("forces", run_forces, c);
+    return s;
+}
+
+ark_status ark_use_forces(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    ark_status s = require_fluid(c);
+    return s == ARK_OK ? c->launch("use_forces", run_use_forces, c) : s;
+}
+
+ark_status ark_phase2(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = c->launch("ghost_phase2", run_phase2_ghost, c);
+    static const char* names[3] = {"phase2_x", "phase2_y", "phase2_z"};
+    for (int d = 0; d < 3 && s == ARK_OK; ++d) {
+        P2Args a{c, d};
+        s = c->launch(names[d], run_phase2_axis, &a);
+    }
+    if (s == ARK_OK)
+        for (int f = F_ro1; f <= F_p3; ++f) std::swap(c->cur[f], c->alt[f]);
+    return s;
+}
+
+ark_status ark_step_phases(ark_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return ARK_ERR_INVALID;
+    ark_status s = require_fluid(c);
+    for (int n = 0; n < nsteps && s == ARK_OK; ++n) {
+        s = time_step_size(c, 1);
+        if (s == ARK_OK) s = ark_phase1(c);
+        if (s == ARK_OK) s = ark_stress_tensor(c);
+        if (s == ARK_OK) s = ark_use_forces(c);
+        if (s == ARK_OK) s = ark_phase2(c);
+        if (s == ARK_OK) s = ark_phase1(c);
+        if (s == ARK_OK) s = ark_use_forces(c);
+    }
+    return s;
+}
+
+ark_status ark_step(ark_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return ARK_ERR_INVALID;
+    ark_status s = require_fluid(c);
+    if (s != ARK_OK) return s;
+    return ark_step_fused(c, nsteps);
+}
+
+ark_status ark_diagnostics(ark_ctx* c, ark_diag* out) {
+    if (!c || !out) return ARK_ERR_INVALID;
+    double* part = c->diag_part;
+    double* res = c->diag_part + 5 * c->diag_blocks;
+    ark_status s = c->hip(launch_diag(c->stream, c->geo, c->fields(), c->K, part, c->diag_blocks, res), "diag");
+    double h[5];
+    Scalars sc;
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(h, res, sizeof h, hipMemcpyDeviceToHost, c->stream), "diag copy");
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(&sc, c->S, sizeof sc, hipMemcpyDeviceToHost, c->stream), "scalars");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s != ARK_OK) return s;
+    out->ke_reference = h[0];
+    out->ke_physical = h[1];
+    out->mass = h[2];
+    out->max_abs_u = h[3];
+    out->nonfinite = (int64_t)h[4];
+    out->dt = sc.dt;
+    out->time = sc.time;
+    out->nstep = sc.nstep;
+    if (out->nonfinite > 0) return c->fail(ARK_ERR_NONFINITE, "non-finite values in the cell fields");
+    return ARK_OK;
+}
+
+ark_status ark_init_tgv(ark_ctx* c, double U0) {
+    if (!c) return ARK_ERR_INVALID;
+    const int N[3] = {c->geo.N1, c->geo.N2, c->geo.N3};
+    const double dx[3] = {c->prm.dx1, c->prm.dx2, c->prm.dx3};
+    // 18 tables of up to max(N)+3 entries
+    int len = std::max(N[0], std::max(N[1], N[2])) + 3;
+    std::vector<double> h(18 * (size_t)len, 0.0);
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < N[d] + 3; ++i) {
+            int w = ((i - 1) % N[d] + N[d]) % N[d] + 1;   // periodic image, 1..N
+            double xc = (w - 0.5) * dx[d];
+            double xf = (w - 1.0) * dx[d];
+            h[(0 * 3 + d) * (size_t)len + i] = sin(xc);
+            h[(1 * 3 + d) * (size_t)len + i] = cos(xc);
+            h[(2 * 3 + d) * (size_t)len + i] = cos(2.0 * xc);
+            h[(3 * 3 + d) * (size_t)len + i] = sin(xf);
+            h[(4 * 3 + d) * (size_t)len + i] = cos(xf);
+            h[(5 * 3 + d) * (size_t)len + i] = cos(2.0 * xf);
+        }
+    double* dtab = nullptr;
+    ark_status s = c->hip(hipMalloc(&dtab, h.size() * sizeof(double)), "hipMalloc tables");
+    if (s != ARK_OK) return s;
+    s = c->hip(hipMemcpyAsync(dtab, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream),
+               "upload tables");
+    TgvTables T;
+    for (int d = 0; d < 3; ++d) {
+        T.sc[d] = dtab + (0 * 3 + d) * (size_t)len;
+        T.cc[d] = dtab + (1 * 3 + d) * (size_t)len;
+        T.c2c[d] = dtab + (2 * 3 + d) * (size_t)len;
+        T.sf[d] = dtab + (3 * 3 + d) * (size_t)len;
+        T.cf[d] = dtab + (4 * 3 + d) * (size_t)len;
+        T.c2f[d] = dtab + (5 * 3 + d) * (size_t)len;
+    }
+    Fields F2;
+    for (int f = 0; f < F_NUM; ++f) F2.a[f] = c->alt[f];
+    double amp = c->prm.ro0 * U0 * U0 / 16.0;
+    if (s == ARK_OK) s = c->hip(launch_init_tgv(c->stream, c->geo, c->fields(), F2, T, U0, amp, c->K), "init_tgv");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    hipFree(dtab);
+    c->has_solid = false;
+    if (s == ARK_OK) {
+        // fresh run: TIME = 0, nStep = 0, running MIN reset
+        Scalars hs{};
+        s = c->hip(hipMemcpy(&hs, c->S, sizeof hs, hipMemcpyDeviceToHost), "scalars");
+        hs.time = 0.0;
+        hs.nstep = 0;
+        hs.dt = 0.0;
+        hs.dt_bits = hs.dt_init;
+        hs.nonfinite = 0;
+        if (s == ARK_OK) s = c->hip(hipMemcpy(c->S, &hs, sizeof hs, hipMemcpyHostToDevice), "scalars");
+    }
+    return s;
+}
+
+ark_status ark_synchronize(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+ark_status ark_get_stream(ark_ctx* c, void** stream) {
+    if (!c || !stream) return ARK_ERR_INVALID;
+    *stream = (void*)c->stream;
+    return ARK_OK;
+}
+
+ark_status ark_profile_enable(ark_ctx* c, int on) {
+    if (!c) return ARK_ERR_INVALID;
+    c->profiling = on != 0;
+    return ARK_OK;
+}
+
+ark_status ark_profile_reset(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    ark_status s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    for (auto& e : c->prof_events) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    c->prof_events.clear();
+    return s;
+}
+
+ark_status ark_profile_read(ark_ctx* c, int max_entries, char (*names)[64], int64_t* launches, double* total_ms,
+                            int* n_entries) {
+    if (!c || !n_entries) return ARK_ERR_INVALID;
+    ark_status s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s != ARK_OK) return s;
+    std::vector<std::string> keys;
+    std::vector<int64_t> cnt;
+    std::vector<double> tot;
+    for (auto& e : c->prof_events) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e.a, e.b);
+        size_t q = std::find(keys.begin(), keys.end(), e.name) - keys.begin();
+        if (q == keys.size()) {
+            keys.push_back(e.name);
+            cnt.push_back(0);
+            tot.push_back(0.0);
+        }
+        cnt[q] += 1;
+        tot[q] += ms;
+    }
+    int n = (int)std::min<size_t>(keys.size(), (size_t)std::max(max_entries, 0));
+    for (int q = 0; q < n; ++q) {
+        if (names) {
+            strncpy(names[q], keys[q].c_str(), 63);
+            names[q][63] = 0;
+        }
+        if (launches) launches[q] = cnt[q];
+        if (total_ms) total_ms[q] = tot[q];
+    }
+    *n_entries = (int)keys.size();
+    return ARK_OK;
+}
+
+ark_status ark_device_bytes(const ark_ctx* c, int64_t* bytes) {
+    if (!c || !bytes) return ARK_ERR_INVALID;
+    *bytes = (int64_t)c->bytes;
+    return ARK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Internal definition of the context behind the C-ABI handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "ark/ark.h"
+#include "ark_device.hpp"
+
+struct ark_prof_event {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+struct ark_ctx {
+    ark_params prm{};
+    arkd::Geo geo{};
+    arkd::Consts K{};
+    hipStream_t stream = nullptr;
+    size_t elems = 0;                  // elements per field allocation
+    double* cur[arkd::F_NUM] = {};     // element (0,0,0) of each field
+    double* alt[arkd::F_NUM] = {};     // second buffer of the 18 face-state fields
+    arkd::Scalars* S = nullptr;        // device scalars
+    double* diag_part = nullptr;
+    int diag_blocks = 0;
+    double host_dt = 0.0;
+    bool has_solid = false;
+    bool profiling = false;
+    std::vector<ark_prof_event> prof_events;
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+    std::string err;
+
+    ark_status fail(ark_status s, const std::string& msg);
+    ark_status hip(hipError_t e, const char* what);
+    // Runs fn(arg) (which enqueues kernels on `stream`), bracketed by events when profiling.
+    ark_status launch(const char* name, hipError_t (*fn)(void*), void* arg);
+    arkd::Fields fields() const;
+};
+
+// The fused step (ark_fused.hip).
+ark_status ark_step_fused(ark_ctx* c, int nsteps);

@@ -1,0 +1,522 @@
+// Phase-by-phase gfx950 kernels of the CABARET/ARK step: one kernel (or a
+// short sequence) per reference phase function.  They define the semantics
+// the fused step kernels are checked against, and they back the per-phase
+// entry points of the C-ABI.
+//
+// Thread mapping: x-fastest everywhere.  A block is 64 x 4 threads over
+// (i, j) of one k-plane, so a wavefront reads 64 consecutive doubles of each
+// field it touches (coalesced 512-byte rows).
+#include "ark_kernels.hpp"
+
+namespace arkd {
+
+// ---------------------------------------------------------------------------
+// TimeStepSize (source/main.cpp:374-415).  Per cell: min3d of the three
+// convective bounds; across cells a MIN that ignores NaN, which is what the
+// reference's sequential min3d(dt, dtu, dtv) does.  Positive doubles order
+// like their bit patterns as unsigned integers (NaN patterns sort above +Inf),
+// so the MIN is an integer atomicMin on the bits.
+__global__ void __launch_bounds__(256) k_cfl(Geo g, const double* __restrict__ u1, const double* __restrict__ u2,
+                                              const double* __restrict__ u3, double c1, double c2, double c3,
+                                              double sound, Scalars* S) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    unsigned long long bits = ~0ull;
+    int bad = 0;
+    if (i <= g.N1 && j <= g.N2 && k <= g.N3) {
+        int c = at(g, i, j, k);
+        double a1 = u1[c], a2 = u2[c], a3 = u3[c];
+        double dtu1 = c1 / (sound + fabs(a1));
+        double dtu2 = c2 / (sound + fabs(a2));
+        double dtu3 = c3 / (sound + fabs(a3));
+        double dtu = min3d(dtu1, dtu2, dtu3);
+        bits = (unsigned long long)__double_as_longlong(dtu);
+        bad = !(isfinite(a1) && isfinite(a2) && isfinite(a3));
+    }
+    // wavefront MIN, then one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o < bits ? o : bits;
+    }
+    unsigned long long badmask = __ballot(bad);
+    if ((threadIdx.x & 63) == 0) {
+        if (bits != ~0ull) atomicMin(&S->dt_bits, bits);
+        if (badmask) atomicAdd((unsigned long long*)&S->nonfinite, (unsigned long long)__popcll(badmask));
+    }
+}
+
+// dt := MIN, reset the running MIN; optionally the driver's bookkeeping
+// TIME += dt/2 twice and nStep++ (source/main.cpp:83-84, 91).
+__global__ void k_cfl_finalize(Scalars* S, int advance) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double dt = __longlong_as_double((long long)S->dt_bits);
+        S->dt = dt;
+        S->dt_bits = S->dt_init;
+        if (advance) {
+            S->nstep += 1;
+            double t = S->time + 0.5 * dt;
+            S->time = t + 0.5 * dt;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Phase1 (source/main.cpp:417-575): Con <- nCon, then the flux-form half-step.
+__device__ __forceinline__ void phase1_cell(const Fields& F, int c, int px, int sxy, double dt, const Consts& K) {
+    double** a = (double**)F.a;
+    const double ds1 = K.ds1, ds2 = K.ds2, ds3 = K.ds3, dvc = K.dvc;
+    double u1_c = a[F_u1nCon][c], u2_c = a[F_u2nCon][c], u3_c = a[F_u3nCon][c];
+    double ro_c = a[F_ronCon][c], t_c = a[F_tnCon][c];
+    a[F_u1Con][c] = u1_c;
+    a[F_u2Con][c] = u2_c;
+    a[F_u3Con][c] = u3_c;
+    a[F_roCon][c] = ro_c;
+    a[F_tCon][c] = t_c;
+    double u1_cn, u2_cn, u3_cn, ro_cn, t_cn;
+    if (a[F_condition][c] < 0.5) {
+        const int fe = c + 1, fw = c, fn = c + px, fs = c, ft = c + sxy, fb = c;
+        double u1_e = a[F_u11][fe], u2_e = a[F_u21][fe], u3_e = a[F_u31][fe];
+        double ro_e = a[F_ro1][fe], t_e = a[F_t1][fe], p_e = a[F_p1][fe];
+        double u1_w = a[F_u11][fw], u2_w = a[F_u21][fw], u3_w = a[F_u31][fw];
+        double ro_w = a[F_ro1][fw], t_w = a[F_t1][fw], p_w = a[F_p1][fw];
+        double u1_n = a[F_u12][fn], u2_n = a[F_u22][fn], u3_n = a[F_u32][fn];
+        double ro_n = a[F_ro2][fn], t_n = a[F_t2][fn], p_n = a[F_p2][fn];
+        double u1_s = a[F_u12][fs], u2_s = a[F_u22][fs], u3_s = a[F_u32][fs];
+        double ro_s = a[F_ro2][fs], t_s = a[F_t2][fs], p_s = a[F_p2][fs];
+        double u1_t = a[F_u13][ft], u2_t = a[F_u23][ft], u3_t = a[F_u33][ft];
+        double ro_t = a[F_ro3][ft], t_t = a[F_t3][ft], p_t = a[F_p3][ft];
+        double u1_b = a[F_u13][fb], u2_b = a[F_u23][fb], u3_b = a[F_u33][fb];
+        double ro_b = a[F_ro3][fb], t_b = a[F_t3][fb], p_b = a[F_p3][fb];
+
+        ro_cn = (ro_c * dvc - 0.5 * dt * (
+                     (ro_e * u1_e - ro_w * u1_w) * ds1 +
+                     (ro_n * u2_n - ro_s * u2_s) * ds2 +
+                     (ro_t * u3_t - ro_b * u3_b) * ds3)) / dvc;
+        u1_cn = (ro_c * u1_c * dvc - 0.5 * dt * (
+                     ((ro_e * u1_e * u1_e + p_e) - (ro_w * u1_w * u1_w + p_w)) * ds1 +
+                     (ro_n * u1_n * u2_n - ro_s * u1_s * u2_s) * ds2 +
+                     (ro_t * u1_t * u3_t - ro_b * u1_b * u3_b) * ds3)) / (ro_cn * dvc);
+        u2_cn = (ro_c * u2_c * dvc - 0.5 * dt * (
+                     (ro_e * u2_e * u1_e - ro_w * u2_w * u1_w) * ds1 +
+                     ((ro_n * u2_n * u2_n + p_n) - (ro_s * u2_s * u2_s + p_s)) * ds2 +
+                     (ro_t * u2_t * u3_t - ro_b * u2_b * u3_b) * ds3)) / (ro_cn * dvc);
+        u3_cn = (ro_c * u3_c * dvc - 0.5 * dt * (
+                     (ro_e * u3_e * u1_e - ro_w * u3_w * u1_w) * ds1 +
+                     (ro_n * u3_n * u2_n - ro_s * u3_s * u2_s) * ds2 +
+                     ((ro_t * u3_t * u3_t + p_t) - (ro_b * u3_b * u3_b + p_b)) * ds3)) / (ro_cn * dvc);
+        t_cn = (ro_c * t_c * dvc - 0.5 * dt * (
+                    (ro_e * t_e * u1_e - ro_w * t_w * u1_w) * ds1 +
+                    (ro_n * t_n * u2_n - ro_s * t_s * u2_s) * ds2 +
+                    (ro_t * t_t * u3_t - ro_b * t_b * u3_b) * ds3)) / (dvc * ro_cn);
+    } else {
+        u1_cn = 0.; u2_cn = 0.; u3_cn = 0.; ro_cn = K.ro0; t_cn = K.t0;
+    }
+    a[F_u1nCon][c] = u1_cn;
+    a[F_u2nCon][c] = u2_cn;
+    a[F_u3nCon][c] = u3_cn;
+    a[F_ronCon][c] = ro_cn;
+    a[F_tnCon][c] = t_cn;
+}
+
+__global__ void __launch_bounds__(256) k_phase1(Geo g, Fields F, Consts K, const Scalars* S) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 || j > g.N2 || k > g.N3) return;
+    phase1_cell(F, at(g, i, j, k), g.px, g.sxy, S->dt, K);
+}
+
+// ---------------------------------------------------------------------------
+// Periodic ghost fill: the single-GPU image of the reference's MPI halo
+// exchange with periods={1,1,1} (source/main.cpp:619-653, 688-719, 754-785,
+// 1101-1189, 1296-1384, 1491-1579).  For a cell array along axis d:
+// plane 0 <- plane N_d, plane N_d+1 <- plane 1; for a face array along its own
+// axis: face 0 <- face N_d, face N_d+2 <- face 2.  Only the interior of the
+// two other axes is written (no edges or corners), and no written element is
+// a source of another: the whole plan runs as one launch.
+__global__ void __launch_bounds__(256) k_ghost(Geo g, GhostPlan plan) {
+    int a = blockIdx.y, d = blockIdx.z;
+    if (a >= plan.n || !((plan.axis_mask[a] >> d) & 1)) return;
+    const int N[3] = {g.N1, g.N2, g.N3};
+    const int st[3] = {1, g.px, g.sxy};
+    int o1 = d == 0 ? 1 : 0;           // x-fastest enumeration of the plane
+    int o2 = d == 2 ? 1 : 2;
+    int plane = N[o1] * N[o2];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * plane) return;
+    int side = t >= plane;
+    int r = t - side * plane;
+    int q = r % N[o1] + 1, p = r / N[o1] + 1;
+    double* A = plan.p[a];
+    int base = q * st[o1] + p * st[o2];
+    if (!plan.is_face[a]) {
+        if (side == 0) A[base] = A[base + N[d] * st[d]];
+        else A[base + (N[d] + 1) * st[d]] = A[base + st[d]];
+    } else {
+        if (side == 0) A[base] = A[base + N[d] * st[d]];
+        else A[base + (N[d] + 2) * st[d]] = A[base + 2 * st[d]];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// StressTensor (source/main.cpp:856-982): sigma on the faces of every
+// orientation; one thread per (i,j,k) in [1,N1+1]x[1,N2+1]x[1,N3+1] writes
+// the faces of the orientations for which (i,j,k) is a computed face.
+__device__ __forceinline__ void stress_one(double* const* a, int c, int cw, double dx, double VIS, double& v1,
+                                           double& v2, double& v3) {
+    double cond_c = a[F_condition][c], cond_cw = a[F_condition][cw];
+    double u1_c = a[F_u1Con][c], u1_cw = a[F_u1Con][cw];
+    double u2_c = a[F_u2Con][c], u2_cw = a[F_u2Con][cw];
+    double u3_c = a[F_u3Con][c], u3_cw = a[F_u3Con][cw];
+    if (cond_cw < 0.5 && cond_c < 0.5) {
+        v1 = VIS * (u1_c - u1_cw) / dx;
+        v2 = VIS * (u2_c - u2_cw) / dx;
+        v3 = VIS * (u3_c - u3_cw) / dx;
+    } else if (cond_cw < 0.5 && cond_c > 0.5) {
+        v1 = 2 * VIS * (-u1_cw) / dx;
+        v2 = 2 * VIS * (-u2_cw) / dx;
+        v3 = 2 * VIS * (-u3_cw) / dx;
+    } else if (cond_cw > 0.5 && cond_c < 0.5) {
+        v1 = 2 * VIS * u1_c / dx;
+        v2 = 2 * VIS * u2_c / dx;
+        v3 = 2 * VIS * u3_c / dx;
+    } else {
+        v1 = v2 = v3 = 0.;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_stress_faces(Geo g, Fields F, Consts K) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 + 1 || j > g.N2 + 1 || k > g.N3 + 1) return;
+    double* const* a = F.a;
+    int c = at(g, i, j, k);
+    double v1, v2, v3;
+    if (j <= g.N2 && k <= g.N3) {
+        stress_one(a, c, c - 1, K.dx[0], K.VIS, v1, v2, v3);
+        a[F_sigm11][c] = v1; a[F_sigm21][c] = v2; a[F_sigm31][c] = v3;
+    }
+    if (i <= g.N1 && k <= g.N3) {
+        stress_one(a, c, c - g.px, K.dx[1], K.VIS, v1, v2, v3);
+        a[F_sigm12][c] = v1; a[F_sigm22][c] = v2; a[F_sigm32][c] = v3;
+    }
+    if (i <= g.N1 && j <= g.N2) {
+        stress_one(a, c, c - g.sxy, K.dx[2], K.VIS, v1, v2, v3);
+        a[F_sigm13][c] = v1; a[F_sigm23][c] = v2; a[F_sigm33][c] = v3;
+    }
+}
+
+// Friction force (source/main.cpp:995-1023).
+__global__ void __launch_bounds__(256) k_forces(Geo g, Fields F, Consts K) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 || j > g.N2 || k > g.N3) return;
+    double* const* a = F.a;
+    int c = at(g, i, j, k);
+    const int sx = 1, sy = g.px, sz = g.sxy;
+    if (a[F_condition][c] == 0) {
+        double ro_c = a[F_roCon][c];
+        a[F_f1][c] = ((a[F_sigm11][c + sx] - a[F_sigm11][c]) * K.ds1 +
+                      (a[F_sigm12][c + sy] - a[F_sigm12][c]) * K.ds2 +
+                      (a[F_sigm13][c + sz] - a[F_sigm13][c]) * K.ds3) * ro_c;
+        a[F_f2][c] = ((a[F_sigm21][c + sx] - a[F_sigm21][c]) * K.ds1 +
+                      (a[F_sigm22][c + sy] - a[F_sigm22][c]) * K.ds2 +
+                      (a[F_sigm23][c + sz] - a[F_sigm23][c]) * K.ds3) * ro_c;
+        a[F_f3][c] = ((a[F_sigm31][c + sx] - a[F_sigm31][c]) * K.ds1 +
+                      (a[F_sigm32][c + sy] - a[F_sigm32][c]) * K.ds2 +
+                      (a[F_sigm33][c + sz] - a[F_sigm33][c]) * K.ds3) * ro_c;
+    } else {
+        a[F_f1][c] = 0.;
+        a[F_f2][c] = 0.;
+        a[F_f3][c] = 0.;
+    }
+}
+
+// UseForces (source/main.cpp:1026-1045).
+__global__ void __launch_bounds__(256) k_use_forces(Geo g, Fields F, Consts K, const Scalars* S) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 || j > g.N2 || k > g.N3) return;
+    double* const* a = F.a;
+    int c = at(g, i, j, k);
+    if (a[F_condition][c] == 0) {
+        const double dt = S->dt, dvc = K.dvc;
+        double ro_cn = a[F_ronCon][c];
+        a[F_u1nCon][c] = (ro_cn * dvc * a[F_u1nCon][c] + 0.5 * dt * a[F_f1][c]) / (dvc * ro_cn);
+        a[F_u2nCon][c] = (ro_cn * dvc * a[F_u2nCon][c] + 0.5 * dt * a[F_f2][c]) / (dvc * ro_cn);
+        a[F_u3nCon][c] = (ro_cn * dvc * a[F_u3nCon][c] + 0.5 * dt * a[F_f3][c]) / (dvc * ro_cn);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Phase2 sweep along axis D, one thread per face (source/main.cpp:1726-2011,
+// 2023-2255, 2261-2494).  The reference fills 1-D line buffers cell by cell
+// and then reads them face by face; the buffers are not a recurrence (face q
+// reads the forward extrapolation of cell q-1 and the backward one of cell q),
+// so a face thread evaluates the two cell halves it needs directly.  Faces are
+// read from the current buffer and written to the next one (the reference's
+// in-place update reads only old values).  Threads of the first and last face
+// also carry the ghost faces 0 and N_D+2, set by the ghost fill, across.
+template <int D>
+__global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, Consts K, const Scalars* S) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    const int E1 = g.N1 + (D == 0), E2 = g.N2 + (D == 1), E3 = g.N3 + (D == 2);
+    if (i > E1 || j > E2 || k > E3) return;
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    const int sd = D == 0 ? 1 : D == 1 ? g.px : g.sxy;
+    const int q = D == 0 ? i : D == 1 ? j : k;
+    const int Nd = D == 0 ? g.N1 : D == 1 ? g.N2 : g.N3;
+    double* const* a = F.a;
+    const double* Un = a[face_u(D, D)];
+    const double* Ua = a[face_u(D, TA)];
+    const double* Ub = a[face_u(D, TB)];
+    const double* P = a[face_p(D)];
+    const double* T = a[face_t(D)];
+    const double* unN = a[F_u1nCon + D];
+    const double* unC = a[F_u1Con + D];
+    const double* uaN = a[F_u1nCon + TA];
+    const double* uaC = a[F_u1Con + TA];
+    const double* ubN = a[F_u1nCon + TB];
+    const double* ubC = a[F_u1Con + TB];
+    const double* roN = a[F_ronCon];
+    const double* roC = a[F_roCon];
+    const double* tN = a[F_tnCon];
+    const double* tC = a[F_tCon];
+    const double dt = S->dt, dx = K.dx[D];
+
+    const int c = at(g, i, j, k);   // cell q and face q
+    const int cb = c - sd;          // cell q-1 and face q-1
+    const int cf = c + sd;          // face q+1
+
+    CellSide sb;   // cell q-1: forward face q, backward face q-1
+    sb.un_f = Un[c]; sb.un_b = Un[cb]; sb.un_cn = unN[cb]; sb.un_c = unC[cb];
+    sb.ua_f = Ua[c]; sb.ua_b = Ua[cb]; sb.ua_cn = uaN[cb]; sb.ua_c = uaC[cb];
+    sb.ub_f = Ub[c]; sb.ub_b = Ub[cb]; sb.ub_cn = ubN[cb]; sb.ub_c = ubC[cb];
+    sb.ro_cn = roN[cb]; sb.ro_c = roC[cb];
+    sb.t_f = T[c]; sb.t_b = T[cb]; sb.t_cn = tN[cb]; sb.t_c = tC[cb];
+    sb.p_f = P[c]; sb.p_b = P[cb];
+    CellSide sf;   // cell q: forward face q+1, backward face q
+    sf.un_f = Un[cf]; sf.un_b = Un[c]; sf.un_cn = unN[c]; sf.un_c = unC[c];
+    sf.ua_f = Ua[cf]; sf.ua_b = Ua[c]; sf.ua_cn = uaN[c]; sf.ua_c = uaC[c];
+    sf.ub_f = Ub[cf]; sf.ub_b = Ub[c]; sf.ub_cn = ubN[c]; sf.ub_c = ubC[c];
+    sf.ro_cn = roN[c]; sf.ro_c = roC[c];
+    sf.t_f = T[cf]; sf.t_b = T[c]; sf.t_cn = tN[c]; sf.t_c = tC[c];
+    sf.p_f = P[cf]; sf.p_b = P[c];
+
+    Fwd fw = cell_forward(sb, dt, dx, K);
+    Bwd bw = cell_backward<D != 1>(sf, dt, dx, K);
+    FaceOut o = face_combine(fw, bw, sb.un_cn, sf.un_cn, sb.t_cn, sf.t_cn, sb.ua_cn, sf.ua_cn, sb.ub_cn,
+                             sf.ub_cn, T[c], Ua[c], Ub[c], K);
+    nx.p[0][c] = o.un;
+    nx.p[1][c] = o.a;
+    nx.p[2][c] = o.b;
+    nx.p[3][c] = o.p;
+    nx.p[4][c] = o.ro;
+    nx.p[5][c] = o.t;
+    if (q == 1) {
+        nx.p[0][cb] = Un[cb]; nx.p[1][cb] = Ua[cb]; nx.p[2][cb] = Ub[cb];
+        nx.p[3][cb] = P[cb]; nx.p[4][cb] = a[face_ro(D)][cb];
+    }
+    if (q == Nd + 1) {
+        nx.p[0][cf] = Un[cf]; nx.p[1][cf] = Ua[cf]; nx.p[2][cf] = Ub[cf];
+        nx.p[3][cf] = P[cf]; nx.p[4][cf] = a[face_ro(D)][cf];
+    }
+}
+
+template __global__ void k_phase2<0>(Geo, Fields, NextFaces, Consts, const Scalars*);
+template __global__ void k_phase2<1>(Geo, Fields, NextFaces, Consts, const Scalars*);
+template __global__ void k_phase2<2>(Geo, Fields, NextFaces, Consts, const Scalars*);
+
+// ---------------------------------------------------------------------------
+// Diagnostics (WriteEnergy, source/main.cpp:2851-2876, plus mass and a
+// non-finite scan): fixed block->cell assignment and a fixed-order final
+// reduction, so repeated calls return identical sums.
+__global__ void __launch_bounds__(256) k_diag_partial(Geo g, Fields F, Consts K, double* part) {
+    const int ncell = g.N1 * g.N2 * g.N3;
+    double ke = 0, kp = 0, m = 0, umax = 0, bad = 0;
+    for (int lin = blockIdx.x * blockDim.x + threadIdx.x; lin < ncell; lin += gridDim.x * blockDim.x) {
+        int i = lin % g.N1 + 1, r = lin / g.N1;
+        int j = r % g.N2 + 1, k = r / g.N2 + 1;
+        int c = at(g, i, j, k);
+        double u1 = F.a[F_u1nCon][c], u2 = F.a[F_u2nCon][c], u3 = F.a[F_u3nCon][c];
+        double ro = F.a[F_ronCon][c], t = F.a[F_tnCon][c];
+        double q = u1 * u1 + u2 * u2 + u3 * u3;
+        ke += q / 2;
+        kp += 0.5 * ro * q * K.dvc;
+        m += ro * K.dvc;
+        double um = fmax(fabs(u1), fmax(fabs(u2), fabs(u3)));
+        umax = fmax(umax, um);
+        if (!(isfinite(u1) && isfinite(u2) && isfinite(u3) && isfinite(ro) && isfinite(t))) bad += 1;
+    }
+    __shared__ double sh[5][256];
+    sh[0][threadIdx.x] = ke; sh[1][threadIdx.x] = kp; sh[2][threadIdx.x] = m;
+    sh[3][threadIdx.x] = umax; sh[4][threadIdx.x] = bad;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int v = 0; v < 5; ++v) {
+                if (v == 3) sh[v][threadIdx.x] = fmax(sh[v][threadIdx.x], sh[v][threadIdx.x + s]);
+                else sh[v][threadIdx.x] += sh[v][threadIdx.x + s];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < 5; ++v) part[v * gridDim.x + blockIdx.x] = sh[v][0];
+}
+
+__global__ void __launch_bounds__(256) k_diag_final(const double* part, int nblocks, double* out) {
+    __shared__ double sh[5][256];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int v = 0; v < 5; ++v) {
+            double x = part[v * nblocks + b];
+            acc[v] = v == 3 ? fmax(acc[v], x) : acc[v] + x;
+        }
+    for (int v = 0; v < 5; ++v) sh[v][threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int v = 0; v < 5; ++v)
+                sh[v][threadIdx.x] = v == 3 ? fmax(sh[v][threadIdx.x], sh[v][threadIdx.x + s])
+                                            : sh[v][threadIdx.x] + sh[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < 5; ++v) out[v] = sh[v][0];
+}
+
+// ---------------------------------------------------------------------------
+// Taylor-Green initial state from 1-D tables (sin, cos, cos 2x of every cell
+// centre and face coordinate, evaluated on the host): the products below
+// are the host formula's, term for term.
+__global__ void __launch_bounds__(256) k_init_tgv(Geo g, Fields F, Fields F2, TgvTables T, double U0, double amp,
+                                                   Consts K) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int k = blockIdx.z;
+    if (i > g.N1 + 2 || j > g.N2 + 2 || k > g.N3 + 2) return;
+    int c = at(g, i, j, k);
+    double* const* a = F.a;
+    const double s2 = K.sound * K.sound;
+    if (i <= g.N1 + 1 && j <= g.N2 + 1 && k <= g.N3 + 1) {
+        double sx = T.sc[0][i], cx = T.cc[0][i], c2x = T.c2c[0][i];
+        double sy = T.sc[1][j], cy = T.cc[1][j], c2y = T.c2c[1][j];
+        double cz = T.cc[2][k], c2z = T.c2c[2][k];
+        double u = U0 * sx * cy * cz;
+        double v = -U0 * cx * sy * cz;
+        double p = amp * (c2x + c2y) * (c2z + 2.0);
+        double ro = K.ro0 + p / s2;
+        a[F_u1nCon][c] = u; a[F_u2nCon][c] = v; a[F_u3nCon][c] = 0.0;
+        a[F_ronCon][c] = ro; a[F_tnCon][c] = K.t0;
+        a[F_u1Con][c] = u; a[F_u2Con][c] = v; a[F_u3Con][c] = 0.0;
+        a[F_roCon][c] = ro; a[F_tCon][c] = K.t0;
+        a[F_condition][c] = 0.0;
+        a[F_f1][c] = 0.0; a[F_f2][c] = 0.0; a[F_f3][c] = 0.0;
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double* tx_s = d == 0 ? T.sf[0] : T.sc[0];
+        const double* tx_c = d == 0 ? T.cf[0] : T.cc[0];
+        const double* tx_2 = d == 0 ? T.c2f[0] : T.c2c[0];
+        const double* ty_s = d == 1 ? T.sf[1] : T.sc[1];
+        const double* ty_c = d == 1 ? T.cf[1] : T.cc[1];
+        const double* ty_2 = d == 1 ? T.c2f[1] : T.c2c[1];
+        const double* tz_c = d == 2 ? T.cf[2] : T.cc[2];
+        const double* tz_2 = d == 2 ? T.c2f[2] : T.c2c[2];
+        double u = U0 * tx_s[i] * ty_c[j] * tz_c[k];
+        double v = -U0 * tx_c[i] * ty_s[j] * tz_c[k];
+        double p = amp * (tx_2[i] + ty_2[j]) * (tz_2[k] + 2.0);
+        double ro = K.ro0 + p / s2;
+        double vals[6] = {u, v, 0.0, p, ro, K.t0};
+        const int slots[6] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_p(d), face_ro(d), face_t(d)};
+        for (int s = 0; s < 6; ++s) {
+            a[slots[s]][c] = vals[s];
+            if (F2.a[slots[s]]) F2.a[slots[s]][c] = vals[s];
+        }
+    }
+    for (int s = F_sigm11; s <= F_sigm33; ++s) a[s][c] = 0.0;
+}
+
+}  // namespace arkd
+
+// ---------------------------------------------------------------------------
+// Launchers.
+namespace arkd {
+
+static dim3 cell_grid(int e1, int e2, int e3) { return dim3((e1 + 63) / 64, (e2 + 3) / 4, e3); }
+static const dim3 kBlock(64, 4, 1);
+
+hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, double c2, double c3, double sound,
+                      Scalars* S) {
+    hipLaunchKernelGGL(k_cfl, cell_grid(g.N1, g.N2, g.N3), kBlock, 0, st, g, F.a[F_u1Con], F.a[F_u2Con],
+                       F.a[F_u3Con], c1, c2, c3, sound, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance) {
+    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(64), 0, st, S, advance);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S) {
+    hipLaunchKernelGGL(k_phase1, cell_grid(g.N1, g.N2, g.N3), kBlock, 0, st, g, F, K, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan) {
+    int m = g.N1 * g.N2;
+    if (g.N1 * g.N3 > m) m = g.N1 * g.N3;
+    if (g.N2 * g.N3 > m) m = g.N2 * g.N3;
+    dim3 grid((2 * m + 255) / 256, plan.n, 3);
+    hipLaunchKernelGGL(k_ghost, grid, dim3(256), 0, st, g, plan);
+    return hipGetLastError();
+}
+
+hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K) {
+    hipLaunchKernelGGL(k_stress_faces, cell_grid(g.N1 + 1, g.N2 + 1, g.N3 + 1), kBlock, 0, st, g, F, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K) {
+    hipLaunchKernelGGL(k_forces, cell_grid(g.N1, g.N2, g.N3), kBlock, 0, st, g, F, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S) {
+    hipLaunchKernelGGL(k_use_forces, cell_grid(g.N1, g.N2, g.N3), kBlock, 0, st, g, F, K, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase2(hipStream_t st, int axis, const Geo& g, const Fields& F, const NextFaces& nx,
+                         const Consts& K, const Scalars* S) {
+    if (axis == 0)
+        hipLaunchKernelGGL(k_phase2<0>, cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S);
+    else if (axis == 1)
+        hipLaunchKernelGGL(k_phase2<1>, cell_grid(g.N1, g.N2 + 1, g.N3), kBlock, 0, st, g, F, nx, K, S);
+    else
+        hipLaunchKernelGGL(k_phase2<2>, cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, double* part, int nblocks,
+                       double* out) {
+    hipLaunchKernelGGL(k_diag_partial, dim3(nblocks), dim3(256), 0, st, g, F, K, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, part, nblocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_tgv(hipStream_t st, const Geo& g, const Fields& F, const Fields& F2, const TgvTables& T,
+                           double U0, double amp, const Consts& K) {
+    hipLaunchKernelGGL(k_init_tgv, cell_grid(g.N1 + 3, g.N2 + 3, g.N3 + 3), kBlock, 0, st, g, F, F2, T, U0, amp, K);
+    return hipGetLastError();
+}
+
+}  // namespace arkd

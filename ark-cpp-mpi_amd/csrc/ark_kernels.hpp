@@ -1,0 +1,46 @@
+// Kernel argument blocks and host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include "ark_device.hpp"
+
+namespace arkd {
+
+// A list of arrays to ghost-fill: axis_mask bit d set = fill along axis d.
+struct GhostPlan {
+    double* p[24];
+    int is_face[24];
+    int axis_mask[24];
+    int n;
+};
+
+// The next-buffer pointers of the six face fields along one axis, in the
+// order normal velocity, tangential a, tangential b, p, rho, t.
+struct NextFaces {
+    double* p[6];
+};
+
+// 1-D Taylor-Green tables per axis, indices 0..N_d+2: sin/cos/cos2 of the
+// cell-centre coordinate (c) and of the face coordinate (f).
+struct TgvTables {
+    const double *sc[3], *cc[3], *c2c[3];
+    const double *sf[3], *cf[3], *c2f[3];
+};
+
+// Launchers (ark_kernels.hip).  Every one enqueues on `st` and returns the
+// launch status.
+hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, double c2, double c3, double sound,
+                      Scalars* S);
+hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
+hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
+hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan);
+hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
+hipError_t launch_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
+hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
+hipError_t launch_phase2(hipStream_t st, int axis, const Geo& g, const Fields& F, const NextFaces& nx,
+                         const Consts& K, const Scalars* S);
+hipError_t launch_diag(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, double* part, int nblocks,
+                       double* out);
+hipError_t launch_init_tgv(hipStream_t st, const Geo& g, const Fields& F, const Fields& F2, const TgvTables& T,
+                           double U0, double amp, const Consts& K);
+
+}  // namespace arkd

@@ -1,0 +1,160 @@
+/* ark.h -- C-ABI of the MI355X-native CABARET/ARK step.
+ *
+ * One context = one rank's subdomain of the reference solver
+ * (albanul/ark-cpp-mpi), resident in HBM on one GPU, advanced by
+ * hand-written gfx950 kernels on one HIP stream.  The reference's MPI halo
+ * exchange becomes an on-device periodic ghost fill.
+ *
+ * Every entry point replaces one reference function or global; the citation
+ * on each names it (paths relative to the reference checkout).  Fields are
+ * exchanged with the caller in the reference's own layout: a full x-fastest
+ * array with the reference's extents (source/Arr3d.cpp:19-21,
+ * source/main.cpp:198-277), i.e. exactly what Arr3d::ref() points to.
+ *
+ * Errors: every call returns an ark_status; no exception crosses the ABI.
+ * The context keeps a sticky message for the last failure (ark_last_error).
+ * Threading: one host thread per context.  Calls are asynchronous on the
+ * context's stream except where marked "synchronises".
+ */
+#ifndef ARK_ARK_H
+#define ARK_ARK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ARK_OK = 0,
+    ARK_ERR_INVALID = 1,      /* bad argument, extent or field id */
+    ARK_ERR_HIP = 2,          /* a HIP runtime call failed */
+    ARK_ERR_UNSUPPORTED = 3,  /* valid input this build does not implement */
+    ARK_ERR_NONFINITE = 4,    /* a NaN/Inf was found by a diagnostic */
+    ARK_ERR_STATE = 5         /* call out of order for the context's state */
+} ark_status;
+
+typedef enum {
+    ARK_PREC_FP64 = 0,   /* every operation in IEEE binary64, bitwise equal to the reference */
+    ARK_PREC_MIXED = 1   /* FP32 fluxes/invariants/stress, FP64 closing update and sums */
+} ark_precision;
+
+/* The reference's arrays, in the reference's naming (headers/bulk.h,
+ * headers/forces.h, headers/stress.h).  Cell fields have extents
+ * (N1+2)(N2+2)(N3+2), face and stress fields (N1+3)(N2+3)(N3+3). */
+typedef enum {
+    ARK_roCon, ARK_u1Con, ARK_u2Con, ARK_u3Con, ARK_tCon,
+    ARK_ronCon, ARK_u1nCon, ARK_u2nCon, ARK_u3nCon, ARK_tnCon,
+    ARK_f1, ARK_f2, ARK_f3,
+    ARK_condition,
+    ARK_ro1, ARK_t1, ARK_u11, ARK_u21, ARK_u31, ARK_p1,
+    ARK_ro2, ARK_t2, ARK_u12, ARK_u22, ARK_u32, ARK_p2,
+    ARK_ro3, ARK_t3, ARK_u13, ARK_u23, ARK_u33, ARK_p3,
+    ARK_sigm11, ARK_sigm21, ARK_sigm31,
+    ARK_sigm12, ARK_sigm22, ARK_sigm32,
+    ARK_sigm13, ARK_sigm23, ARK_sigm33,
+    ARK_NUM_FIELDS
+} ark_field;
+
+/* Replaces the reference's scalar globals (headers/constants.h:20,
+ * headers/bulk.h:17-34) for one rank: N_d = n_d - 1 interior cells per axis,
+ * grid steps, CFL, VIS, sound, ro0_g, t0.  periodic must be 1 on every axis
+ * (the reference's MPI exchange with periods={1,1,1}, source/main.cpp:56-61). */
+typedef struct {
+    int N1, N2, N3;
+    double dx1, dx2, dx3;
+    double CFL, VIS, sound, ro0, t0;
+    int periodic[3];
+    int precision;   /* ark_precision */
+    int device;      /* HIP device ordinal */
+} ark_params;
+
+typedef struct ark_ctx ark_ctx;
+
+/* Diagnostics over interior cells of the level-n fields (nCon), FP64 sums
+ * reduced in a fixed order (deterministic). */
+typedef struct {
+    double ke_reference;   /* sum 1/2|u|^2 (not rho-weighted, no volume): source/main.cpp:2853-2861 */
+    double ke_physical;    /* sum 1/2 rho|u|^2 dV */
+    double mass;           /* sum rho dV */
+    double max_abs_u;      /* max over cells of max_d |u_d| */
+    int64_t nonfinite;     /* cells with a NaN/Inf in rho, u1, u2, u3 or t */
+    double dt;             /* dt of the last step taken */
+    double time;           /* TIME: += dt/2 twice per step (source/main.cpp:84,91) */
+    int64_t nstep;         /* nStep (source/main.cpp:83) */
+} ark_diag;
+
+/* Allocation done by the reference's Input() (source/main.cpp:198-277):
+ * every array on the device, zero-filled. */
+ark_status ark_create(const ark_params* params, ark_ctx** out);
+/* FreeMemory() (source/main.cpp:2893-2961). */
+void ark_destroy(ark_ctx* ctx);
+const char* ark_last_error(const ark_ctx* ctx);
+
+/* Copy one Arr3d (host pointer = Arr3d::ref(), extents as constructed) into
+ * or out of the context.  Extents must be the reference's for that field.
+ * ark_download synchronises. */
+ark_status ark_upload(ark_ctx* ctx, int field, const double* host, int e1, int e2, int e3);
+ark_status ark_download(ark_ctx* ctx, int field, double* host, int e1, int e2, int e3);
+
+/* The global dt (headers/constants.h:20) and TIME/nStep bookkeeping. */
+ark_status ark_set_dt(ark_ctx* ctx, double dt);
+ark_status ark_get_dt(ark_ctx* ctx, double* dt);   /* synchronises */
+
+/* TimeStepSize(), source/main.cpp:374-415: dt = min over interior cells of
+ * the CFL bounds, on the device; the MPI MIN allreduce is the identity on one
+ * GPU.  dt_out may be NULL (then nothing synchronises). */
+ark_status ark_time_step_size(ark_ctx* ctx, double* dt_out);
+/* Phase1(), source/main.cpp:417-575. */
+ark_status ark_phase1(ark_ctx* ctx);
+/* StressTensor(), source/main.cpp:577-1024 (the u*Con halo exchange at
+ * 603-801 becomes the on-device periodic ghost fill). */
+ark_status ark_stress_tensor(ark_ctx* ctx);
+/* UseForces(), source/main.cpp:1026-1045. */
+ark_status ark_use_forces(ark_ctx* ctx);
+/* Phase2(), source/main.cpp:1047-2495 (the 12-field halo exchange at
+ * 1049-1631 becomes the on-device periodic ghost fill). */
+ark_status ark_phase2(ark_ctx* ctx);
+
+/* nsteps iterations of the reference's driver loop body (source/main.cpp:
+ * 82-91: TimeStepSize, Phase1, StressTensor, UseForces, Phase2, Phase1,
+ * UseForces) through the fused kernels.  Results are bitwise equal to the
+ * phase-by-phase calls in FP64 mode.  Nothing synchronises: dt stays on the
+ * device between steps. */
+ark_status ark_step(ark_ctx* ctx, int nsteps);
+
+/* The same nsteps through the phase-by-phase kernels (one launch per
+ * reference phase): the unfused baseline that ark_step is checked against. */
+ark_status ark_step_phases(ark_ctx* ctx, int nsteps);
+
+/* WriteEnergy(), source/main.cpp:2851-2876 (without the file write), plus
+ * mass and a non-finite scan.  Synchronises. */
+ark_status ark_diagnostics(ark_ctx* ctx, ark_diag* out);
+
+/* Taylor-Green vortex initial state built on the device from 1-D trig tables
+ * that are evaluated on the host with the C library's sin/cos, so the fields
+ * are bitwise equal to a host evaluation of the same formula.  Replaces the
+ * reference's InitializeData() (source/main.cpp:280-372) for the periodic
+ * box; Con := nCon, ghosts are periodic copies, f and sigma are zero. */
+ark_status ark_init_tgv(ark_ctx* ctx, double U0);
+
+/* Block until all work on the context's stream is done. */
+ark_status ark_synchronize(ark_ctx* ctx);
+/* The context's hipStream_t, for callers that order their own work on it. */
+ark_status ark_get_stream(ark_ctx* ctx, void** stream);
+
+/* Per-kernel timing with HIP events on the context's stream.  When enabled,
+ * every launch is bracketed by events; ark_profile_read synchronises and
+ * returns, per kernel name, the number of launches and the summed duration. */
+ark_status ark_profile_enable(ark_ctx* ctx, int on);
+ark_status ark_profile_read(ark_ctx* ctx, int max_entries, char (*names)[64],
+                            int64_t* launches, double* total_ms, int* n_entries);
+ark_status ark_profile_reset(ark_ctx* ctx);
+
+/* Bytes of device memory held by the context. */
+ark_status ark_device_bytes(const ark_ctx* ctx, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARK_ARK_H */

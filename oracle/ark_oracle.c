@@ -1,0 +1,584 @@
+/* Test infrastructure only: the oracle (checker) for the HIP step, never shipped.
+ *
+ * A CPU restatement of the reference solver's time step on a triply periodic
+ * box.  Every arithmetic expression keeps the reference's operand order and
+ * association (built with -ffp-contract=off, no fast-math), so its results are
+ * bitwise equal to the reference's on the same inputs.  Pinned by the golden
+ * vectors that the unmodified reference writes (tests/golden, via
+ * oracle/ref_harness.cpp) and by closed-form checks in tests/.
+ *
+ * Where the reference repeats one algorithm three times (the directional
+ * sweeps of Phase2, the stress faces), this file has one axis-generic routine
+ * with the per-axis differences made explicit:
+ *   - gq of the X1 and X3 sweeps uses 2(r_cn-r_c)/dt, X2 uses 2(q_cn-q_c)/dt
+ *     (source/main.cpp:1804, 2083, 2321);
+ *   - in the X2 sweep's solid-wall branches the tangential u1 is taken from
+ *     u2nCon (source/main.cpp:2219, 2234) and t from tbBuf[i] rather than
+ *     tbBuf[j] (source/main.cpp:2215); the line buffers are therefore shared
+ *     across sweeps exactly as in the reference;
+ *   - t faces are never part of the ghost fill (source/main.cpp:1101-1127).
+ */
+#include "ark_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+int ark_oracle_is_face(int slot) { return slot >= ORC_ro1; }
+
+/* min3d / max3d, source/main.cpp:2963-2973 (ternary order kept: NaN and
+ * signed-zero behaviour follow the reference, not fmin/fmax). */
+static double min3d(double x1, double x2, double x3) {
+    x1 = x1 > x2 ? x2 : x1;
+    x1 = x1 > x3 ? x3 : x1;
+    return x1;
+}
+static double max3d(double x1, double x2, double x3) {
+    x1 = x1 < x2 ? x2 : x1;
+    x1 = x1 < x3 ? x3 : x1;
+    return x1;
+}
+
+typedef struct {
+    long cs[3]; /* cell strides along x, y, z (extent Nd+2) */
+    long fs[3]; /* face strides (extent Nd+3) */
+    int N[3];
+} geom;
+
+static geom make_geom(const ark_oracle_state* s) {
+    geom g;
+    g.N[0] = s->N1; g.N[1] = s->N2; g.N[2] = s->N3;
+    g.cs[0] = 1; g.cs[1] = s->N1 + 2; g.cs[2] = (long)(s->N1 + 2) * (s->N2 + 2);
+    g.fs[0] = 1; g.fs[1] = s->N1 + 3; g.fs[2] = (long)(s->N1 + 3) * (s->N2 + 3);
+    return g;
+}
+
+static long C(const geom* g, int i, int j, int k) { return i * g->cs[0] + j * g->cs[1] + k * g->cs[2]; }
+static long F(const geom* g, int i, int j, int k) { return i * g->fs[0] + j * g->fs[1] + k * g->fs[2]; }
+
+/* ---- periodic ghost fill: what MPI_Sendrecv to oneself delivers with
+ * periods={1,1,1} (source/main.cpp:619-653, 1101-1189 and the y/z analogues).
+ * Only the interior of the two other axes is filled: no edges, no corners. */
+static void fill_cell_axis(const geom* g, double* a, int d) {
+    int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+    for (int p = 1; p <= g->N[o2]; ++p)
+        for (int q = 1; q <= g->N[o1]; ++q) {
+            long base = q * g->cs[o1] + p * g->cs[o2];
+            a[base + 0 * g->cs[d]] = a[base + (long)g->N[d] * g->cs[d]];
+            a[base + (long)(g->N[d] + 1) * g->cs[d]] = a[base + 1 * g->cs[d]];
+        }
+}
+static void fill_face_axis(const geom* g, double* a, int d) {
+    int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+    for (int p = 1; p <= g->N[o2]; ++p)
+        for (int q = 1; q <= g->N[o1]; ++q) {
+            long base = q * g->fs[o1] + p * g->fs[o2];
+            a[base + 0 * g->fs[d]] = a[base + (long)g->N[d] * g->fs[d]];
+            a[base + (long)(g->N[d] + 2) * g->fs[d]] = a[base + 2 * g->fs[d]];
+        }
+}
+
+/* ---- TimeStepSize, source/main.cpp:374-415 (one rank: the MIN allreduce is
+ * the identity). */
+void ark_oracle_time_step_size(ark_oracle_state* s) {
+    geom g = make_geom(s);
+    const double* u1 = s->f[ORC_u1Con];
+    const double* u2 = s->f[ORC_u2Con];
+    const double* u3 = s->f[ORC_u3Con];
+    double dt = pow(10, 8);
+    for (int i = 1; i <= s->N1; ++i)
+        for (int j = 1; j <= s->N2; ++j)
+            for (int k = 1; k <= s->N3; ++k) {
+                long c = C(&g, i, j, k);
+                double dtu1 = s->CFL * s->dx1 / (s->sound + fabs(u1[c]));
+                double dtu2 = s->CFL * s->dx2 / (s->sound + fabs(u2[c]));
+                double dtu3 = s->CFL * s->dx3 / (s->sound + fabs(u3[c]));
+                double dtu = min3d(dtu1, dtu2, dtu3);
+                double dtv;
+                if (s->VIS > pow(10, -16)) {
+                    double dtv1 = s->CFL * s->dx1 * s->dx1 / (2. * s->VIS);
+                    double dtv2 = s->CFL * s->dx2 * s->dx2 / (2. * s->VIS);
+                    double dtv3 = s->CFL * s->dx3 * s->dx3 / (2. * s->VIS);
+                    dtv = min3d(dtv1, dtv2, dtv3);
+                } else {
+                    dtv = pow(10, 16);
+                }
+                dt = min3d(dt, dtu, dtv);
+            }
+    s->dt = dt;
+}
+
+/* ---- Phase1, source/main.cpp:417-575. */
+void ark_oracle_phase1(ark_oracle_state* s) {
+    geom g = make_geom(s);
+    double** f = s->f;
+    for (int i = 1; i <= s->N1; i++)
+        for (int j = 1; j <= s->N2; j++)
+            for (int k = 1; k <= s->N3; k++) {
+                long c = C(&g, i, j, k);
+                f[ORC_u1Con][c] = f[ORC_u1nCon][c];
+                f[ORC_u2Con][c] = f[ORC_u2nCon][c];
+                f[ORC_u3Con][c] = f[ORC_u3nCon][c];
+                f[ORC_roCon][c] = f[ORC_ronCon][c];
+                f[ORC_tCon][c] = f[ORC_tnCon][c];
+            }
+    const double ds1 = s->dx2 * s->dx3, ds2 = s->dx1 * s->dx3, ds3 = s->dx1 * s->dx2;
+    const double dvc = s->dx1 * s->dx2 * s->dx3;
+    const double dt = s->dt;
+    for (int i = 1; i <= s->N1; i++)
+        for (int j = 1; j <= s->N2; j++)
+            for (int k = 1; k <= s->N3; k++) {
+                long c = C(&g, i, j, k);
+                double u1_cn, u2_cn, u3_cn, ro_cn, t_cn;
+                if (f[ORC_condition][c] < 0.5) {
+                    long fe = F(&g, i + 1, j, k), fw = F(&g, i, j, k);
+                    long fn = F(&g, i, j + 1, k), fsth = F(&g, i, j, k);
+                    long ft = F(&g, i, j, k + 1), fb = F(&g, i, j, k);
+                    double u1_e = f[ORC_u11][fe], u2_e = f[ORC_u21][fe], u3_e = f[ORC_u31][fe];
+                    double ro_e = f[ORC_ro1][fe], t_e = f[ORC_t1][fe], p_e = f[ORC_p1][fe];
+                    double u1_w = f[ORC_u11][fw], u2_w = f[ORC_u21][fw], u3_w = f[ORC_u31][fw];
+                    double ro_w = f[ORC_ro1][fw], t_w = f[ORC_t1][fw], p_w = f[ORC_p1][fw];
+                    double u1_n = f[ORC_u12][fn], u2_n = f[ORC_u22][fn], u3_n = f[ORC_u32][fn];
+                    double ro_n = f[ORC_ro2][fn], t_n = f[ORC_t2][fn], p_n = f[ORC_p2][fn];
+                    double u1_s = f[ORC_u12][fsth], u2_s = f[ORC_u22][fsth], u3_s = f[ORC_u32][fsth];
+                    double ro_s = f[ORC_ro2][fsth], t_s = f[ORC_t2][fsth], p_s = f[ORC_p2][fsth];
+                    double u1_t = f[ORC_u13][ft], u2_t = f[ORC_u23][ft], u3_t = f[ORC_u33][ft];
+                    double ro_t = f[ORC_ro3][ft], t_t = f[ORC_t3][ft], p_t = f[ORC_p3][ft];
+                    double u1_b = f[ORC_u13][fb], u2_b = f[ORC_u23][fb], u3_b = f[ORC_u33][fb];
+                    double ro_b = f[ORC_ro3][fb], t_b = f[ORC_t3][fb], p_b = f[ORC_p3][fb];
+                    double u1_c = f[ORC_u1Con][c], u2_c = f[ORC_u2Con][c], u3_c = f[ORC_u3Con][c];
+                    double ro_c = f[ORC_roCon][c], t_c = f[ORC_tCon][c];
+
+                    ro_cn = (ro_c * dvc - 0.5 * dt * (
+                                 (ro_e * u1_e - ro_w * u1_w) * ds1 +
+                                 (ro_n * u2_n - ro_s * u2_s) * ds2 +
+                                 (ro_t * u3_t - ro_b * u3_b) * ds3)) / dvc;
+                    u1_cn = (ro_c * u1_c * dvc - 0.5 * dt * (
+                                 ((ro_e * u1_e * u1_e + p_e) - (ro_w * u1_w * u1_w + p_w)) * ds1 +
+                                 (ro_n * u1_n * u2_n - ro_s * u1_s * u2_s) * ds2 +
+                                 (ro_t * u1_t * u3_t - ro_b * u1_b * u3_b) * ds3)) / (ro_cn * dvc);
+                    u2_cn = (ro_c * u2_c * dvc - 0.5 * dt * (
+                                 (ro_e * u2_e * u1_e - ro_w * u2_w * u1_w) * ds1 +
+                                 ((ro_n * u2_n * u2_n + p_n) - (ro_s * u2_s * u2_s + p_s)) * ds2 +
+                                 (ro_t * u2_t * u3_t - ro_b * u2_b * u3_b) * ds3)) / (ro_cn * dvc);
+                    u3_cn = (ro_c * u3_c * dvc - 0.5 * dt * (
+                                 (ro_e * u3_e * u1_e - ro_w * u3_w * u1_w) * ds1 +
+                                 (ro_n * u3_n * u2_n - ro_s * u3_s * u2_s) * ds2 +
+                                 ((ro_t * u3_t * u3_t + p_t) - (ro_b * u3_b * u3_b + p_b)) * ds3)) / (ro_cn * dvc);
+                    t_cn = (ro_c * t_c * dvc - 0.5 * dt * (
+                                (ro_e * t_e * u1_e - ro_w * t_w * u1_w) * ds1 +
+                                (ro_n * t_n * u2_n - ro_s * t_s * u2_s) * ds2 +
+                                (ro_t * t_t * u3_t - ro_b * t_b * u3_b) * ds3)) / (dvc * ro_cn);
+                } else {
+                    u1_cn = 0.; u2_cn = 0.; u3_cn = 0.; ro_cn = s->ro0; t_cn = s->t0;
+                }
+                f[ORC_u1nCon][c] = u1_cn;
+                f[ORC_u2nCon][c] = u2_cn;
+                f[ORC_u3nCon][c] = u3_cn;
+                f[ORC_ronCon][c] = ro_cn;
+                f[ORC_tnCon][c] = t_cn;
+            }
+}
+
+/* ---- StressTensor, source/main.cpp:577-1024.  One routine for the three
+ * face orientations (the reference's loops at 856-894, 900-938, 944-982). */
+static void stress_faces(ark_oracle_state* s, const geom* g, int d) {
+    double** f = s->f;
+    const int sig[3][3] = {{ORC_sigm11, ORC_sigm21, ORC_sigm31},
+                           {ORC_sigm12, ORC_sigm22, ORC_sigm32},
+                           {ORC_sigm13, ORC_sigm23, ORC_sigm33}};
+    const double dx = d == 0 ? s->dx1 : (d == 1 ? s->dx2 : s->dx3);
+    const double VIS = s->VIS;
+    int lo[3] = {1, 1, 1}, hi[3] = {s->N1, s->N2, s->N3};
+    hi[d] += 1;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+                long c = C(g, i, j, k), cw = c - g->cs[d];
+                double uc[3] = {f[ORC_u1Con][c], f[ORC_u2Con][c], f[ORC_u3Con][c]};
+                double uw[3] = {f[ORC_u1Con][cw], f[ORC_u2Con][cw], f[ORC_u3Con][cw]};
+                double cond_c = f[ORC_condition][c], cond_cw = f[ORC_condition][cw];
+                double v[3];
+                for (int m = 0; m < 3; ++m) {
+                    if (cond_cw < 0.5 && cond_c < 0.5) v[m] = VIS * (uc[m] - uw[m]) / dx;
+                    else if (cond_cw < 0.5 && cond_c > 0.5) v[m] = 2 * VIS * (-uw[m]) / dx;
+                    else if (cond_cw > 0.5 && cond_c < 0.5) v[m] = 2 * VIS * uc[m] / dx;
+                    else v[m] = 0.;
+                }
+                long fi = F(g, i, j, k);
+                for (int m = 0; m < 3; ++m) f[sig[d][m]][fi] = v[m];
+            }
+}
+
+void ark_oracle_stress_tensor(ark_oracle_state* s) {
+    geom g = make_geom(s);
+    double** f = s->f;
+    long nf = (long)(s->N1 + 3) * (s->N2 + 3) * (s->N3 + 3);
+    for (int a = ORC_sigm11; a <= ORC_sigm33; ++a) memset(f[a], 0, sizeof(double) * nf);
+    for (int d = 0; d < 3; ++d) {
+        fill_cell_axis(&g, f[ORC_u1Con], d);
+        fill_cell_axis(&g, f[ORC_u2Con], d);
+        fill_cell_axis(&g, f[ORC_u3Con], d);
+    }
+    for (int d = 0; d < 3; ++d) stress_faces(s, &g, d);
+    const double ds1 = s->dx2 * s->dx3, ds2 = s->dx1 * s->dx3, ds3 = s->dx1 * s->dx2;
+    const int sig[3][3] = {{ORC_sigm11, ORC_sigm12, ORC_sigm13},
+                           {ORC_sigm21, ORC_sigm22, ORC_sigm23},
+                           {ORC_sigm31, ORC_sigm32, ORC_sigm33}};
+    const int fo[3] = {ORC_f1, ORC_f2, ORC_f3};
+    for (int i = 1; i <= s->N1; ++i)
+        for (int j = 1; j <= s->N2; ++j)
+            for (int k = 1; k <= s->N3; ++k) {
+                long c = C(&g, i, j, k);
+                long fi = F(&g, i, j, k);
+                for (int m = 0; m < 3; ++m) {
+                    if (f[ORC_condition][c] == 0) {
+                        double ro_c = f[ORC_roCon][c];
+                        const double* s1 = f[sig[m][0]];
+                        const double* s2 = f[sig[m][1]];
+                        const double* s3 = f[sig[m][2]];
+                        f[fo[m]][c] = ((s1[fi + g.fs[0]] - s1[fi]) * ds1 +
+                                       (s2[fi + g.fs[1]] - s2[fi]) * ds2 +
+                                       (s3[fi + g.fs[2]] - s3[fi]) * ds3) * ro_c;
+                    } else {
+                        f[fo[m]][c] = 0.;
+                    }
+                }
+            }
+}
+
+/* ---- UseForces, source/main.cpp:1026-1045. */
+void ark_oracle_use_forces(ark_oracle_state* s) {
+    geom g = make_geom(s);
+    double** f = s->f;
+    const double dvc = s->dx1 * s->dx2 * s->dx3;
+    const double dt = s->dt;
+    for (int i = 1; i <= s->N1; ++i)
+        for (int j = 1; j <= s->N2; ++j)
+            for (int k = 1; k <= s->N3; ++k) {
+                long c = C(&g, i, j, k);
+                if (f[ORC_condition][c] == 0) {
+                    double ro_cn = f[ORC_ronCon][c];
+                    f[ORC_u1nCon][c] = (ro_cn * dvc * f[ORC_u1nCon][c] + 0.5 * dt * f[ORC_f1][c]) / (dvc * ro_cn);
+                    f[ORC_u2nCon][c] = (ro_cn * dvc * f[ORC_u2nCon][c] + 0.5 * dt * f[ORC_f2][c]) / (dvc * ro_cn);
+                    f[ORC_u3nCon][c] = (ro_cn * dvc * f[ORC_u3nCon][c] + 0.5 * dt * f[ORC_f3][c]) / (dvc * ro_cn);
+                }
+            }
+}
+
+/* ---- Phase2, source/main.cpp:1047-2495. */
+typedef struct { double *r, *q, *tf, *tb, *af, *ab, *bf, *bb; } line_buffers;
+
+static void phase2_sweep(ark_oracle_state* s, const geom* g, int d, line_buffers* B) {
+    double** f = s->f;
+    const int ucon[3] = {ORC_u1Con, ORC_u2Con, ORC_u3Con};
+    const int uncon[3] = {ORC_u1nCon, ORC_u2nCon, ORC_u3nCon};
+    /* face arrays along d: component m on faces normal to d */
+    const int uface[3][3] = {{ORC_u11, ORC_u21, ORC_u31},
+                             {ORC_u12, ORC_u22, ORC_u32},
+                             {ORC_u13, ORC_u23, ORC_u33}};
+    const int pface[3] = {ORC_p1, ORC_p2, ORC_p3};
+    const int rface[3] = {ORC_ro1, ORC_ro2, ORC_ro3};
+    const int tface[3] = {ORC_t1, ORC_t2, ORC_t3};
+    /* tangential components in the order the reference's buffers hold them */
+    const int ta = d == 0 ? 1 : 0;
+    const int tb = d == 2 ? 1 : 2;
+    const double dx = d == 0 ? s->dx1 : (d == 1 ? s->dx2 : s->dx3);
+    const double sound = s->sound, ro0_g = s->ro0, dt = s->dt, t0 = s->t0;
+    const int gq_uses_r = (d != 1);
+    const int n = g->N[d] + 1; /* the reference's node count along d */
+
+    const double* Un = f[uface[d][d]];
+    const double* Ua = f[uface[d][ta]];
+    const double* Ub = f[uface[d][tb]];
+    const double* P = f[pface[d]];
+    const double* T = f[tface[d]];
+    const double* un_c = f[ucon[d]];
+    const double* un_n = f[uncon[d]];
+    const double* ua_c = f[ucon[ta]];
+    const double* ua_n = f[uncon[ta]];
+    const double* ub_c = f[ucon[tb]];
+    const double* ub_n = f[uncon[tb]];
+    const double* ro_c_a = f[ORC_roCon];
+    const double* ro_n_a = f[ORC_ronCon];
+    const double* t_c_a = f[ORC_tCon];
+    const double* t_n_a = f[ORC_tnCon];
+    const double* cond = f[ORC_condition];
+
+    /* line order of the reference: X1 k-outer j-inner, X2 k-outer i-inner,
+     * X3 i-outer j-inner */
+    int outer = d == 2 ? 0 : 2;
+    int inner = d == 0 ? 1 : (d == 1 ? 0 : 1);
+    for (int o = 1; o <= g->N[outer]; ++o)
+        for (int in = 1; in <= g->N[inner]; ++in) {
+            long cbase = o * g->cs[outer] + in * g->cs[inner];
+            long fbase = o * g->fs[outer] + in * g->fs[inner];
+            for (int q = 0; q <= n; q++) {
+                long c = cbase + q * g->cs[d];
+                long ff = fbase + (q + 1) * g->fs[d], fb = fbase + q * g->fs[d];
+                double u1_f = Un[ff], u1_b = Un[fb], u1_cn = un_n[c], u1_c = un_c[c];
+                double u2_f = Ua[ff], u2_b = Ua[fb], u2_cn = ua_n[c], u2_c = ua_c[c];
+                double u3_f = Ub[ff], u3_b = Ub[fb], u3_cn = ub_n[c], u3_c = ub_c[c];
+                double ro_cn = ro_n_a[c], ro_c = ro_c_a[c];
+                double t_f = T[ff], t_b = T[fb], t_cn = t_n_a[c], t_c = t_c_a[c];
+                double p_f = P[ff], p_b = P[fb];
+                double p_cn = sound * sound * (ro_cn - ro0_g);
+                double p_c = sound * sound * (ro_c - ro0_g);
+
+                double r_f = u1_f + p_f / (ro0_g * sound);
+                double r_b = u1_b + p_b / (ro0_g * sound);
+                double r_cn = u1_cn + p_cn / (ro0_g * sound);
+                double r_c = u1_c + p_c / (ro0_g * sound);
+                double r_fn = 2 * r_cn - r_b;
+                double q_f = u1_f - p_f / (ro0_g * sound);
+                double q_b = u1_b - p_b / (ro0_g * sound);
+                double q_cn = u1_cn - p_cn / (ro0_g * sound);
+                double q_c = u1_c - p_c / (ro0_g * sound);
+                double q_bn = 2 * q_cn - q_f;
+                double t_fn = 2 * t_cn - t_b;
+                double t_bn = 2 * t_cn - t_f;
+                double u2_fn = 2 * u2_cn - u2_b;
+                double u2_bn = 2 * u2_cn - u2_f;
+                double u3_fn = 2 * u3_cn - u3_b;
+                double u3_bn = 2 * u3_cn - u3_f;
+
+                double gr = 2 * (r_cn - r_c) / dt + (u1_cn + sound) * (r_f - r_b) / dx;
+                double gq = (gq_uses_r ? 2 * (r_cn - r_c) / dt : 2 * (q_cn - q_c) / dt) +
+                            (u1_cn - sound) * (q_f - q_b) / dx;
+                double gt = 2 * (t_cn - t_c) / dt + u1_cn * (t_f - t_b) / dx;
+                double gu2 = 2 * (u2_cn - u2_c) / dt + u1_cn * (u2_f - u2_b) / dx;
+                double gu3 = 2 * (u3_cn - u3_c) / dt + u1_cn * (u3_f - u3_b) / dx;
+
+                double rmax = max3d(r_f, r_c, r_b) + dt * gr;
+                double rmin = min3d(r_f, r_c, r_b) + dt * gr;
+                double qmax = max3d(q_f, q_c, q_b) + dt * gq;
+                double qmin = min3d(q_f, q_c, q_b) + dt * gq;
+                double tmax = max3d(t_f, t_c, t_b) + dt * gt;
+                double tmin = min3d(t_f, t_c, t_b) + dt * gt;
+                double u2_max = max3d(u2_f, u2_c, u2_b) + dt * gu2;
+                double u2_min = min3d(u2_f, u2_c, u2_b) + dt * gu2;
+                double u3_max = max3d(u3_f, u3_c, u3_b) + dt * gu3;
+                double u3_min = min3d(u3_f, u3_c, u3_b) + dt * gu3;
+
+                if (r_fn > rmax) r_fn = rmax;
+                if (r_fn < rmin) r_fn = rmin;
+                if (q_bn > qmax) q_bn = qmax;
+                if (q_bn < qmin) q_bn = qmin;
+                if (t_fn > tmax) t_fn = tmax;
+                if (t_fn < tmin) t_fn = tmin;
+                if (t_bn > tmax) t_bn = tmax;
+                if (t_bn < tmin) t_bn = tmin;
+                if (u2_fn > u2_max) u2_fn = u2_max;
+                if (u2_fn < u2_min) u2_fn = u2_min;
+                if (u2_bn > u2_max) u2_bn = u2_max;
+                if (u2_bn < u2_min) u2_bn = u2_min;
+                if (u3_fn > u3_max) u3_fn = u3_max;
+                if (u3_fn < u3_min) u3_fn = u3_min;
+                if (u3_bn > u3_max) u3_bn = u3_max;
+                if (u3_bn < u3_min) u3_bn = u3_min;
+
+                B->r[q + 1] = r_fn;
+                B->q[q] = q_bn;
+                B->tf[q + 1] = t_fn;
+                B->tb[q] = t_bn;
+                B->af[q + 1] = u2_fn;
+                B->ab[q] = u2_bn;
+                B->bf[q + 1] = u3_fn;
+                B->bb[q] = u3_bn;
+            }
+
+            double* Un_w = f[uface[d][d]];
+            double* Ua_w = f[uface[d][ta]];
+            double* Ub_w = f[uface[d][tb]];
+            double* P_w = f[pface[d]];
+            double* R_w = f[rface[d]];
+            double* T_w = f[tface[d]];
+            for (int q = 1; q <= n; q++) {
+                long c = cbase + q * g->cs[d], cb = c - g->cs[d];
+                long fq = fbase + q * g->fs[d];
+                double cond_b = cond[cb], cond_f = cond[c];
+                double rn, qn, un, pn, ro_n, tn, ua, ubv;
+                if (cond_b < 0.5 && cond_f < 0.5) {
+                    rn = B->r[q];
+                    qn = B->q[q];
+                    un = (rn + qn) / 2;
+                    pn = (rn - qn) * sound * ro0_g / 2;
+                    ro_n = ro0_g + pn / (sound * sound);
+                    double ucf = un_n[c], ucb = un_n[cb];
+                    if (ucf >= 0 && ucb >= 0) {
+                        tn = B->tf[q]; ua = B->af[q]; ubv = B->bf[q];
+                    } else if (ucf <= 0 && ucb <= 0) {
+                        tn = B->tb[q]; ua = B->ab[q]; ubv = B->bb[q];
+                    } else if (ucb >= 0 && ucf <= 0) {
+                        if (ucb > -ucf) { tn = B->tf[q]; ua = B->af[q]; ubv = B->bf[q]; }
+                        else { tn = B->tb[q]; ua = B->ab[q]; ubv = B->bb[q]; }
+                    } else {
+                        tn = t_n_a[c] + t_n_a[cb] - T_w[fq];
+                        ua = ua_n[c] + ua_n[cb] - Ua_w[fq];
+                        ubv = ub_n[c] + ub_n[cb] - Ub_w[fq];
+                    }
+                } else {
+                    un = 0.;
+                    /* X2 reads the normal velocity where the tangential u1 belongs */
+                    const double* ua_n_wall = (d == 1) ? un_n : ua_n;
+                    if (cond_b > 0.5 && cond_f < 0.5) {
+                        if (un_n[c] <= 0) {
+                            qn = B->q[q];
+                            ua = B->ab[q];
+                            ubv = B->bb[q];
+                            pn = -qn * sound * ro0_g;
+                            ro_n = ro0_g + pn / (sound * sound);
+                            tn = (d == 1) ? B->tb[in] : B->tb[q];
+                        } else {
+                            ua = ua_n_wall[c];
+                            ubv = ub_n[c];
+                            ro_n = ro_n_a[c];
+                            pn = (ro_n - ro0_g) * (sound * sound);
+                            tn = t_n_a[c];
+                        }
+                    } else if (cond_b < 0.5 && cond_f > 0.5) {
+                        if (un_n[cb] >= 0) {
+                            rn = B->r[q];
+                            ua = B->af[q];
+                            ubv = B->bf[q];
+                            pn = rn * sound * ro0_g;
+                            ro_n = ro0_g + pn / (sound * sound);
+                            tn = B->tf[q];
+                        } else {
+                            ua = ua_n_wall[cb];
+                            ubv = ub_n[cb];
+                            ro_n = ro_n_a[cb];
+                            pn = (ro_n - ro0_g) * (sound * sound);
+                            tn = t_n_a[cb];
+                        }
+                    } else {
+                        ua = 0.; ubv = 0.; pn = 0.; ro_n = ro0_g; tn = t0;
+                    }
+                }
+                P_w[fq] = pn;
+                Un_w[fq] = un;
+                R_w[fq] = ro_n;
+                T_w[fq] = tn;
+                Ua_w[fq] = ua;
+                Ub_w[fq] = ubv;
+            }
+        }
+}
+
+void ark_oracle_phase2(ark_oracle_state* s) {
+    geom g = make_geom(s);
+    double** f = s->f;
+    const int cell_fields[] = {ORC_u1nCon, ORC_u2nCon, ORC_u3nCon, ORC_roCon, ORC_ronCon, ORC_tCon, ORC_tnCon};
+    const int face_fields[3][5] = {{ORC_u11, ORC_u21, ORC_u31, ORC_ro1, ORC_p1},
+                                   {ORC_u12, ORC_u22, ORC_u32, ORC_ro2, ORC_p2},
+                                   {ORC_u13, ORC_u23, ORC_u33, ORC_ro3, ORC_p3}};
+    for (int d = 0; d < 3; ++d) {
+        for (int a = 0; a < 7; ++a) fill_cell_axis(&g, f[cell_fields[a]], d);
+        for (int a = 0; a < 5; ++a) fill_face_axis(&g, f[face_fields[d][a]], d);
+    }
+    int nmax = s->N1 + 1;
+    if (s->N2 + 1 > nmax) nmax = s->N2 + 1;
+    if (s->N3 + 1 > nmax) nmax = s->N3 + 1;
+    double* mem = (double*)calloc((size_t)8 * (nmax + 2), sizeof(double));
+    line_buffers B;
+    B.r = mem; B.q = mem + (nmax + 2); B.tf = mem + 2 * (nmax + 2); B.tb = mem + 3 * (nmax + 2);
+    B.af = mem + 4 * (nmax + 2); B.ab = mem + 5 * (nmax + 2); B.bf = mem + 6 * (nmax + 2);
+    B.bb = mem + 7 * (nmax + 2);
+    for (int d = 0; d < 3; ++d) phase2_sweep(s, &g, d, &B);
+    free(mem);
+}
+
+void ark_oracle_step(ark_oracle_state* s, int nsteps, double* dt_out) {
+    for (int n = 0; n < nsteps; ++n) {
+        ark_oracle_time_step_size(s);
+        if (dt_out) dt_out[n] = s->dt;
+        ark_oracle_phase1(s);
+        ark_oracle_stress_tensor(s);
+        ark_oracle_use_forces(s);
+        ark_oracle_phase2(s);
+        ark_oracle_phase1(s);
+        ark_oracle_use_forces(s);
+    }
+}
+
+/* WriteEnergy's sum (source/main.cpp:2853-2861), loop order kept, without the
+ * volume factor (the caller applies it). */
+double ark_oracle_energy(const ark_oracle_state* s) {
+    geom g = make_geom(s);
+    const double* u1 = s->f[ORC_u1nCon];
+    const double* u2 = s->f[ORC_u2nCon];
+    const double* u3 = s->f[ORC_u3nCon];
+    double energy = 0;
+    for (int i = 1; i <= s->N1; i++)
+        for (int j = 1; j <= s->N2; j++)
+            for (int k = 1; k <= s->N3; k++) {
+                long c = C(&g, i, j, k);
+                energy += (u1[c] * u1[c] + u2[c] * u2[c] + u3[c] * u3[c]) / 2;
+            }
+    return energy;
+}
+
+double ark_oracle_mass(const ark_oracle_state* s) {
+    geom g = make_geom(s);
+    const double* ro = s->f[ORC_ronCon];
+    double dvc = s->dx1 * s->dx2 * s->dx3;
+    double m = 0;
+    for (int k = 1; k <= s->N3; k++)
+        for (int j = 1; j <= s->N2; j++)
+            for (int i = 1; i <= s->N1; i++) m += ro[C(&g, i, j, k)] * dvc;
+    return m;
+}
+
+/* ---- Taylor-Green initial state, same formula and evaluation order as the
+ * reference harness's init_tgv (oracle/ref_harness.cpp). */
+static int wrap_index(int g, int N) { return ((g - 1) % N + N) % N + 1; }
+
+void ark_oracle_init_tgv(ark_oracle_state* s, double U0) {
+    geom g = make_geom(s);
+    double** f = s->f;
+    const double amp = s->ro0 * U0 * U0 / 16.0;
+    const double dxs[3] = {s->dx1, s->dx2, s->dx3};
+    for (int a = 0; a < ORC_NUM_FIELDS; ++a) {
+        long n = ark_oracle_is_face(a) ? (long)(s->N1 + 3) * (s->N2 + 3) * (s->N3 + 3)
+                                       : (long)(s->N1 + 2) * (s->N2 + 2) * (s->N3 + 2);
+        memset(f[a], 0, sizeof(double) * n);
+    }
+    for (int k = 0; k <= s->N3 + 1; ++k)
+        for (int j = 0; j <= s->N2 + 1; ++j)
+            for (int i = 0; i <= s->N1 + 1; ++i) {
+                double x = (wrap_index(i, s->N1) - 0.5) * dxs[0];
+                double y = (wrap_index(j, s->N2) - 0.5) * dxs[1];
+                double z = (wrap_index(k, s->N3) - 0.5) * dxs[2];
+                double u = U0 * sin(x) * cos(y) * cos(z);
+                double v = -U0 * cos(x) * sin(y) * cos(z);
+                double w = 0.0;
+                double p = amp * (cos(2.0 * x) + cos(2.0 * y)) * (cos(2.0 * z) + 2.0);
+                long c = C(&g, i, j, k);
+                f[ORC_u1nCon][c] = u; f[ORC_u2nCon][c] = v; f[ORC_u3nCon][c] = w;
+                f[ORC_ronCon][c] = s->ro0 + p / (s->sound * s->sound);
+                f[ORC_tnCon][c] = s->t0;
+                f[ORC_u1Con][c] = u; f[ORC_u2Con][c] = v; f[ORC_u3Con][c] = w;
+                f[ORC_roCon][c] = f[ORC_ronCon][c];
+                f[ORC_tCon][c] = s->t0;
+            }
+    const int U[3][3] = {{ORC_u11, ORC_u21, ORC_u31}, {ORC_u12, ORC_u22, ORC_u32}, {ORC_u13, ORC_u23, ORC_u33}};
+    const int P[3] = {ORC_p1, ORC_p2, ORC_p3};
+    const int R[3] = {ORC_ro1, ORC_ro2, ORC_ro3};
+    const int T[3] = {ORC_t1, ORC_t2, ORC_t3};
+    for (int d = 0; d < 3; ++d)
+        for (int k = 0; k <= s->N3 + 2; ++k)
+            for (int j = 0; j <= s->N2 + 2; ++j)
+                for (int i = 0; i <= s->N1 + 2; ++i) {
+                    double x = (wrap_index(i, s->N1) - (d == 0 ? 1.0 : 0.5)) * dxs[0];
+                    double y = (wrap_index(j, s->N2) - (d == 1 ? 1.0 : 0.5)) * dxs[1];
+                    double z = (wrap_index(k, s->N3) - (d == 2 ? 1.0 : 0.5)) * dxs[2];
+                    double u = U0 * sin(x) * cos(y) * cos(z);
+                    double v = -U0 * cos(x) * sin(y) * cos(z);
+                    double p = amp * (cos(2.0 * x) + cos(2.0 * y)) * (cos(2.0 * z) + 2.0);
+                    long q = F(&g, i, j, k);
+                    f[U[d][0]][q] = u; f[U[d][1]][q] = v; f[U[d][2]][q] = 0.0;
+                    f[P[d]][q] = p;
+                    f[R[d]][q] = s->ro0 + p / (s->sound * s->sound);
+                    f[T[d]][q] = s->t0;
+                }
+}

@@ -1,0 +1,210 @@
+"""Writes the golden vectors under tests/golden/ from the UNMODIFIED reference.
+
+Runs only where the reference checkout and MPICH exist (the build container):
+it drives ``oracle/_ref/ark_ref_harness`` (built by ``make -C oracle ref`` from
+the reference's own sources) and stores inputs and outputs as small fixtures.
+The reference itself never travels; only these data files do.
+
+Fixtures written:
+  phase_fluid.npz, phase_solid.npz
+      a seeded random state on a non-cubic 6x5x4 box with distinct dx per
+      axis (so a transposed index or axis mix-up fails), plus the reference's
+      output of each phase function (TimeStepSize, Phase1, StressTensor,
+      UseForces, Phase2) and of two full steps.  "solid" adds a random solid
+      mask (condition=1 cells) and exercises the wall branches.
+  tgv8.npz
+      the full reference state (all 41 arrays with ghosts) of the 8^3
+      Taylor-Green run after 1, 2 and 10 steps, with its dt/KE/mass series.
+  tgv_digests.json
+      per-array SHA-256 and norms of the 16^3 x 10-step and 64^3 x 200-step
+      Taylor-Green runs (1 rank, full arrays), their per-step dt, KE and mass
+      series, and the result of the decomposition-invariance check (the same
+      32^3 run on 1, 2, 4 and 8 ranks must give bitwise-equal fields).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+from ark_oracle import FIELDS, OracleState, read_meta  # noqa: E402
+
+HARNESS = os.path.join(REPO, "oracle", "_ref", "ark_ref_harness")
+MPIRUN = "/opt/conda/bin/mpirun"
+PHASES = ["TimeStepSize", "Phase1", "StressTensor", "UseForces", "Phase2", "Step"]
+
+
+def run(args, ranks=1):
+    cmd = [MPIRUN, "-n", str(ranks), HARNESS] + [str(a) for a in args]
+    res = subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return res.stdout
+
+
+def random_state(seed: int, solid: bool) -> OracleState:
+    rng = np.random.default_rng(seed)
+    N1, N2, N3 = 6, 5, 4
+    st = OracleState(N1, N2, N3, dx=(0.3, 0.25, 0.2), dt=0.0125, CFL=0.3, VIS=0.05,
+                     sound=10.0, ro0=1.0, t0=1.0)
+
+    def vel(shape):
+        v = rng.uniform(-1.0, 1.0, size=shape)
+        v[rng.random(shape) < 0.12] = 0.0          # exact zeros: ucf/ucb == 0 branches
+        v[rng.random(shape) < 0.05] *= -1e-3       # tiny values of both signs
+        return v
+
+    for name in FIELDS:
+        shp = st.shape(name)
+        if name.startswith("sigm") or name in ("f1", "f2", "f3"):
+            st.arrays[name] = rng.uniform(-1, 1, size=shp)   # overwritten by the phases
+        elif name == "condition":
+            st.arrays[name] = (rng.random(shp) < 0.3).astype(np.float64) if solid else np.zeros(shp)
+        elif name.startswith("ro"):
+            st.arrays[name] = 1.0 + 0.02 * rng.uniform(-1, 1, size=shp)
+        elif name.startswith("p"):
+            st.arrays[name] = 2.0 * rng.uniform(-1, 1, size=shp)
+        elif name.startswith("t"):
+            st.arrays[name] = 1.0 + 0.1 * rng.uniform(-1, 1, size=shp)
+        else:
+            st.arrays[name] = vel(shp)
+    # ties between a face value and its cell value hit the equal-operand paths of min3d/max3d
+    st.arrays["u11"][2, 2, 3] = st.arrays["u1Con"][2, 2, 3]
+    st.arrays["u22"][1, 3, 2] = st.arrays["u2Con"][1, 3, 2]
+    return st
+
+
+def reference_phase(state: OracleState, phase: str, work: str) -> OracleState:
+    ind = os.path.join(work, "in")
+    outd = os.path.join(work, "out_" + phase)
+    shutil.rmtree(ind, ignore_errors=True)
+    state.to_dir(ind)
+    os.makedirs(outd, exist_ok=True)
+    if phase == "Step":
+        run(["run", ind, 2, outd])
+    else:
+        run(["phase", phase, ind, outd])
+    return OracleState.from_dir(outd)
+
+
+def pack_state(prefix: str, st: OracleState, names=None) -> dict:
+    out = {f"{prefix}:{k}": np.asarray(v) for k, v in st.scalars().items()}
+    for n in (names or FIELDS):
+        out[f"{prefix}:{n}"] = st[n]
+    return out
+
+
+def phase_fixture(path: str, seed: int, solid: bool, work: str) -> None:
+    st = random_state(seed, solid)
+    data = pack_state("in", st)
+    for ph in PHASES:
+        res = reference_phase(st, ph, work)
+        changed = [n for n in FIELDS if not np.array_equal(res[n].view(np.int64), st[n].view(np.int64))]
+        data.update(pack_state(ph, res, changed))
+        data[f"{ph}:changed"] = np.array(changed)
+    np.savez_compressed(path, **data)
+
+
+def digest(a: np.ndarray) -> dict:
+    a = np.ascontiguousarray(a, dtype="<f8")
+    flat = a.ravel()
+    idx = np.linspace(0, flat.size - 1, 16).astype(np.int64)
+    return {
+        "sha256": hashlib.sha256(a.tobytes()).hexdigest(),
+        "sum": float(flat.sum()).hex(), "l2": float(np.sqrt((flat * flat).sum())).hex(),
+        "linf": float(np.abs(flat).max()).hex(), "min": float(flat.min()).hex(),
+        "max": float(flat.max()).hex(),
+        "sample_index": idx.tolist(), "sample": [float(flat[q]).hex() for q in idx],
+    }
+
+
+def read_series(path: str) -> dict:
+    rows = [l.split() for l in open(os.path.join(path, "series.txt")) if not l.startswith("#")]
+    return {"step": [int(r[0]) for r in rows], "dt": [r[1] for r in rows],
+            "TIME": [r[2] for r in rows], "ke_reference": [r[3] for r in rows],
+            "mass": [r[4] for r in rows]}
+
+
+def tgv_run(N: int, steps: int, work: str, dumps, ranks=1) -> str:
+    d = os.path.join(work, f"tgv{N}_{steps}_{ranks}")
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    run(["-quiet", "tgv", N, N, N, steps, d] + list(dumps), ranks=ranks)
+    return d
+
+
+def main() -> None:
+    if not os.path.exists(HARNESS):
+        sys.exit("reference harness not built: make -C oracle ref")
+    work = tempfile.mkdtemp(prefix="ark_golden_")
+    try:
+        phase_fixture(os.path.join(HERE, "phase_fluid.npz"), 1234, False, work)
+        phase_fixture(os.path.join(HERE, "phase_solid.npz"), 4321, True, work)
+
+        d8 = tgv_run(8, 10, work, [1, 2, 10])
+        data = {}
+        for s in (1, 2, 10):
+            st = OracleState.from_dir(os.path.join(d8, f"s{s}"))
+            data.update(pack_state(f"s{s}", st))
+        ser = read_series(d8)
+        data["series_dt"] = np.array([float.fromhex(v) for v in ser["dt"]])
+        data["series_ke"] = np.array([float.fromhex(v) for v in ser["ke_reference"]])
+        data["series_mass"] = np.array([float.fromhex(v) for v in ser["mass"]])
+        np.savez_compressed(os.path.join(HERE, "tgv8.npz"), **data)
+
+        dig = {"runs": {}}
+        for N, steps in ((16, 10), (64, 200)):
+            d = tgv_run(N, steps, work, [steps])
+            st = OracleState.from_dir(os.path.join(d, f"s{steps}"))
+            meta = read_meta(os.path.join(d, f"s{steps}"))
+            dig["runs"][f"{N}^3x{steps}"] = {
+                "N": N, "steps": steps, "VIS": float(st.VIS).hex(), "CFL": float(st.CFL).hex(),
+                "sound": float(st.sound).hex(), "final_dt": float(meta["dt"]).hex(),
+                "series": read_series(d),
+                "arrays": {n: digest(st[n]) for n in FIELDS},
+            }
+        # decomposition invariance: 32^3 x 20 steps on 1, 2, 4, 8 ranks
+        inv = {}
+        base = None
+        for ranks in (1, 2, 4, 8):
+            d = tgv_run(32, 20, work, [20], ranks=ranks)
+            sd = os.path.join(d, "s20")
+            if ranks == 1:
+                st = OracleState.from_dir(sd)
+                fields = {n: np.ascontiguousarray(st[n][1:-1, 1:-1, 1:-1]) for n in
+                          ("ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon")}
+            else:
+                fields = {n: np.fromfile(os.path.join(sd, n + ".interior.f64"), dtype="<f8").reshape(32, 32, 32)
+                          for n in ("ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon")}
+            h = {n: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+                 for n, a in fields.items()}
+            inv[str(ranks)] = h
+            # rho and u must agree bit for bit.  t may not: the reference never
+            # exchanges t ghost faces (source/main.cpp:1101-1127), so at an
+            # internal rank boundary its ghost t face keeps its initial value.
+            phys = {n: v for n, v in h.items() if n != "tnCon"}
+            if base is None:
+                base = phys
+            elif phys != base:
+                raise SystemExit(f"decomposition invariance FAILED at {ranks} ranks")
+        dig["decomposition_invariance_32^3x20"] = inv
+        dig["decomposition_invariance_note"] = (
+            "ronCon/u1nCon/u2nCon/u3nCon are bitwise equal on 1, 2, 4 and 8 ranks; tnCon may "
+            "differ at round-off because t ghost faces are never exchanged by the reference")
+        dig["generator"] = "tests/golden/make_fixtures.py via oracle/_ref/ark_ref_harness"
+        with open(os.path.join(HERE, "tgv_digests.json"), "w") as fh:
+            json.dump(dig, fh, indent=1)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,225 @@
+"""HIP step against the oracle and the reference's golden vectors.
+
+FP64 mode is bit-for-bit: every comparison below is on the raw float64 bits of
+every one of the reference's 41 arrays, ghosts included, unless it says
+otherwise.  All calls go through the C-ABI (lib/libark_amd.so) via ctypes.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, bits_equal, describe_mismatch
+from ark_oracle import FIELDS, OracleState, state_from_npz
+import arkamd
+
+pytestmark = pytest.mark.gpu
+
+PHASES = ["TimeStepSize", "Phase1", "StressTensor", "UseForces", "Phase2"]
+
+
+def solver_for(st: OracleState, precision=arkamd.PREC_FP64) -> arkamd.Solver:
+    s = arkamd.Solver(*st.N, dx=st.dx, CFL=st.CFL, VIS=st.VIS, sound=st.sound, ro0=st.ro0, t0=st.t0,
+                      precision=precision)
+    s.upload_all(st.arrays)
+    s.dt = st.dt
+    return s
+
+
+def assert_state_equal(solver: arkamd.Solver, want: OracleState, names=FIELDS, what=""):
+    got = solver.download_all(names)
+    bad = [describe_mismatch(n, got[n], want[n]) for n in names if not bits_equal(got[n], want[n])]
+    assert not bad, what + "\n" + "\n".join(bad)
+
+
+def random_fluid_state(N, seed, dt=0.01) -> OracleState:
+    rng = np.random.default_rng(seed)
+    st = OracleState(*N, dx=(0.31, 0.27, 0.22), dt=dt, CFL=0.3, VIS=0.02, sound=10.0, ro0=1.0, t0=1.0)
+    for name in FIELDS:
+        shp = st.shape(name)
+        if name == "condition":
+            continue
+        if name.startswith("ro"):
+            st.arrays[name] = 1.0 + 0.02 * rng.uniform(-1, 1, shp)
+        elif name.startswith("p"):
+            st.arrays[name] = 2.0 * rng.uniform(-1, 1, shp)
+        elif name.startswith("t"):
+            st.arrays[name] = 1.0 + 0.1 * rng.uniform(-1, 1, shp)
+        else:
+            v = rng.uniform(-1, 1, shp)
+            v[rng.random(shp) < 0.1] = 0.0
+            st.arrays[name] = v
+    return st
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("N", [(8, 8, 8), (6, 5, 4), (17, 9, 12)])
+def test_init_tgv_bitwise(N):
+    ref = OracleState.tgv(*N)
+    with arkamd.Solver(*N) as s:
+        s.init_tgv(1.0)
+        assert_state_equal(s, ref, what=f"TGV init {N}")
+
+
+@pytest.mark.parametrize("phase", PHASES)
+def test_phase_vs_reference_fixture(phase):
+    """Each phase on the reference's own random non-cubic input (tests/golden)."""
+    z = np.load(os.path.join(GOLDEN, "phase_fluid.npz"))
+    st = state_from_npz(z, "in")
+    changed = set(str(n) for n in z[f"{phase}:changed"])
+    want = st.copy()
+    for n in changed:
+        want.arrays[n] = z[f"{phase}:{n}"]
+    with solver_for(st) as s:
+        if phase == "TimeStepSize":
+            dt = s.time_step_size()
+            assert dt == float(z[f"{phase}:dt"])
+        else:
+            s.phase(phase)
+        assert_state_equal(s, want, what=phase)
+
+
+@pytest.mark.parametrize("stepper", ["step", "step_phases"])
+def test_two_steps_vs_reference_fixture(stepper):
+    z = np.load(os.path.join(GOLDEN, "phase_fluid.npz"))
+    st = state_from_npz(z, "in")
+    changed = set(str(n) for n in z["Step:changed"])
+    want = st.copy()
+    for n in changed:
+        want.arrays[n] = z[f"Step:{n}"]
+    with solver_for(st) as s:
+        getattr(s, stepper)(2)
+        assert s.dt == float(z["Step:dt"])
+        assert_state_equal(s, want, what=stepper)
+
+
+@pytest.mark.parametrize("N,seed", [((7, 3, 5), 11), ((16, 16, 16), 12), ((33, 17, 9), 13)])
+@pytest.mark.parametrize("phase", PHASES)
+def test_phase_vs_oracle_random(N, seed, phase):
+    st = random_fluid_state(N, seed)
+    want = st.copy()
+    want.phase(phase)
+    with solver_for(st) as s:
+        if phase == "TimeStepSize":
+            assert s.time_step_size() == want.dt
+        else:
+            s.phase(phase)
+        assert_state_equal(s, want, what=f"{phase} {N}")
+
+
+def test_tgv8_vs_reference_states():
+    z = np.load(os.path.join(GOLDEN, "tgv8.npz"))
+    with arkamd.Solver(8, 8, 8) as s:
+        s.init_tgv()
+        done = 0
+        for target in (1, 2, 10):
+            for _ in range(target - done):
+                s.step(1)
+            done = target
+            assert_state_equal(s, state_from_npz(z, f"s{target}"), what=f"8^3 step {target}")
+
+
+@pytest.mark.parametrize("N,steps", [(16, 10), (64, 200)])
+def test_tgv_digest(N, steps):
+    """The reference's 16^3 x 10 and 64^3 x 200 Taylor-Green runs: SHA-256 of
+    all 41 final arrays and the whole dt series, bit for bit."""
+    run = json.load(open(os.path.join(GOLDEN, "tgv_digests.json")))["runs"][f"{N}^3x{steps}"]
+    want_dt = [float.fromhex(v) for v in run["series"]["dt"][1:]]
+    with arkamd.Solver(N, N, N) as s:
+        s.init_tgv()
+        dts = []
+        for _ in range(steps):
+            s.step(1)
+            dts.append(s.dt)
+        assert dts == want_dt
+        got = s.download_all()
+        bad = [n for n in FIELDS
+               if hashlib.sha256(np.ascontiguousarray(got[n], dtype="<f8").tobytes()).hexdigest()
+               != run["arrays"][n]["sha256"]]
+        assert not bad, f"arrays differing from the reference digest: {bad}"
+        d = s.diagnostics()
+        assert d["nstep"] == steps
+
+
+@pytest.mark.parametrize("N", [(32, 32, 32), (20, 13, 7)])
+def test_fused_equals_phases(N):
+    with arkamd.Solver(*N) as a, arkamd.Solver(*N) as b:
+        a.init_tgv()
+        b.init_tgv()
+        a.step(7)
+        b.step_phases(7)
+        ga, gb = a.download_all(), b.download_all()
+        bad = [describe_mismatch(n, ga[n], gb[n]) for n in FIELDS if not bits_equal(ga[n], gb[n])]
+        assert not bad, "\n".join(bad)
+        da, db = a.diagnostics(), b.diagnostics()
+        assert da["dt"] == db["dt"] and da["time"] == db["time"] and da["nstep"] == db["nstep"] == 7
+
+
+def test_fused_equals_phases_random_state():
+    st = random_fluid_state((19, 11, 6), 99, dt=0.004)
+    with solver_for(st) as a, solver_for(st) as b:
+        a.step(3)
+        b.step_phases(3)
+        ga, gb = a.download_all(), b.download_all()
+        bad = [describe_mismatch(n, ga[n], gb[n]) for n in FIELDS if not bits_equal(ga[n], gb[n])]
+        assert not bad, "\n".join(bad)
+
+
+def test_determinism_two_runs():
+    outs = []
+    for _ in range(2):
+        with arkamd.Solver(24, 24, 24) as s:
+            s.init_tgv()
+            s.step(5)
+            d = s.diagnostics()
+            outs.append((s.download_all(["u1nCon", "ronCon", "p2"]), d))
+    for n in outs[0][0]:
+        assert bits_equal(outs[0][0][n], outs[1][0][n])
+    assert outs[0][1] == outs[1][1]
+
+
+def test_diagnostics_vs_oracle():
+    st = OracleState.tgv(16)
+    st.step(3)
+    with arkamd.Solver(16, 16, 16) as s:
+        s.init_tgv()
+        s.step(3)
+        d = s.diagnostics()
+    assert d["ke_reference"] == pytest.approx(st.energy_sum(), rel=1e-13)
+    assert d["mass"] == pytest.approx(st.mass(), rel=1e-13)
+    u = [st[n][1:-1, 1:-1, 1:-1] for n in ("u1nCon", "u2nCon", "u3nCon")]
+    ro = st["ronCon"][1:-1, 1:-1, 1:-1]
+    dvc = st.dx[0] * st.dx[1] * st.dx[2]
+    kp = float((0.5 * ro * (u[0] ** 2 + u[1] ** 2 + u[2] ** 2) * dvc).sum())
+    assert d["ke_physical"] == pytest.approx(kp, rel=1e-12)
+    assert d["max_abs_u"] == max(float(np.abs(x).max()) for x in u)
+    assert d["nonfinite"] == 0
+
+
+def test_nonfinite_is_reported():
+    with arkamd.Solver(8, 8, 8) as s:
+        s.init_tgv()
+        u = s.download("u1nCon")
+        u[3, 4, 5] = np.nan
+        s.upload("u1nCon", u)
+        with pytest.raises(arkamd.ArkError) as e:
+            s.diagnostics()
+        assert e.value.status == arkamd.ARK_ERR_NONFINITE
+        assert s.diagnostics(allow_nonfinite=True)["nonfinite"] == 1
+
+
+def test_errors_are_reported():
+    with arkamd.Solver(6, 5, 4) as s:
+        with pytest.raises(ValueError):
+            s.upload("u11", np.zeros((6, 7, 8)))
+        cond = np.zeros(s.shape("condition"))
+        cond[2, 2, 2] = 1.0
+        s.upload("condition", cond)
+        with pytest.raises(arkamd.ArkError) as e:
+            s.phase2()
+        assert e.value.status == arkamd.ARK_ERR_UNSUPPORTED
+    with pytest.raises(arkamd.ArkError) as e:
+        arkamd.Solver(0, 4, 4)
+    assert e.value.status == arkamd.ARK_ERR_INVALID

@@ -1,0 +1,40 @@
+"""CPU checks of the C-ABI library: it loads and exports every symbol that
+include/ark/ark.h declares (no compute call is made without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import REPO
+import arkamd
+
+
+def declared_symbols():
+    src = open(os.path.join(REPO, "include", "ark", "ark.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ark_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(arkamd.EXPORTED_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(arkamd.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_params_struct_layout():
+    # ark_params: 3 ints, 8 doubles, int[3], 2 ints -> offsets as the C compiler lays them out
+    P = arkamd.Params
+    assert P.dx1.offset == 16 and P.t0.offset == 72 and P.periodic.offset == 80
+    assert P.precision.offset == 92 and ctypes.sizeof(P) == 104
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    monkeypatch.setattr(arkamd, "_lib", None)
+    monkeypatch.setattr(arkamd, "LIB_PATH", "/nonexistent/libark_amd.so")
+    with pytest.raises(RuntimeError, match="not built"):
+        arkamd.lib()

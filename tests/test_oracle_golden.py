@@ -1,0 +1,115 @@
+"""The oracle (oracle/ark_oracle.c) against the golden vectors that the
+UNMODIFIED reference wrote (tests/golden/make_fixtures.py).  Bitwise: the
+restatement keeps every expression's operand order, so any dropped term,
+wrong sign, wrong index or transposed axis shows up as a bit difference.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, bits_equal, describe_mismatch
+from ark_oracle import FIELDS, OracleState, state_from_npz
+
+PHASES = ["TimeStepSize", "Phase1", "StressTensor", "UseForces", "Phase2", "Step"]
+
+
+@pytest.mark.parametrize("fixture", ["phase_fluid.npz", "phase_solid.npz"])
+@pytest.mark.parametrize("phase", PHASES)
+def test_phase_matches_reference(fixture, phase):
+    z = np.load(os.path.join(GOLDEN, fixture))
+    st = state_from_npz(z, "in")
+    if phase == "Step":
+        st.step(2)
+    else:
+        st.phase(phase)
+    changed = set(str(n) for n in z[f"{phase}:changed"])
+    for name in FIELDS:
+        want = z[f"{phase}:{name}"] if name in changed else z[f"in:{name}"]
+        assert bits_equal(st[name], want), describe_mismatch(name, st[name], want)
+    if phase in ("TimeStepSize", "Step"):
+        assert st.dt == float(z[f"{phase}:dt"])
+
+
+def test_phase_fixture_covers_branches():
+    """The random Phase2 input reaches all four upwind cases on every axis."""
+    z = np.load(os.path.join(GOLDEN, "phase_fluid.npz"))
+    st = state_from_npz(z, "in")
+    N1, N2, N3 = st.N
+    for d, name in enumerate(("u1nCon", "u2nCon", "u3nCon")):
+        u = st[name]
+        sl_f = [slice(1, N3 + 1), slice(1, N2 + 1), slice(1, N1 + 1)]
+        sl_b = list(sl_f)
+        ax = 2 - d
+        sl_f[ax] = slice(1, [N1, N2, N3][d] + 2)
+        sl_b[ax] = slice(0, [N1, N2, N3][d] + 1)
+        # ghosts are filled by Phase2 before use; emulate the periodic fill
+        full = u.copy()
+        n = [N1, N2, N3][d]
+        idx = [slice(None)] * 3
+        src = list(idx); dst = list(idx)
+        dst[ax] = 0; src[ax] = n
+        full[tuple(dst)] = u[tuple(src)]
+        dst[ax] = n + 1; src[ax] = 1
+        full[tuple(dst)] = u[tuple(src)]
+        uf, ub = full[tuple(sl_f)], full[tuple(sl_b)]
+        assert np.any((uf >= 0) & (ub >= 0))
+        assert np.any((uf <= 0) & (ub <= 0))
+        assert np.any((ub >= 0) & (uf <= 0) & ~((uf >= 0) & (ub >= 0)) & ~((uf <= 0) & (ub <= 0)))
+        assert np.any((ub < 0) & (uf > 0))
+
+
+def test_tgv_init_matches_reference():
+    z = np.load(os.path.join(GOLDEN, "tgv8.npz"))
+    st = OracleState.tgv(8)
+    ref = state_from_npz(z, "s1")
+    assert st.dx == ref.dx and st.VIS == ref.VIS and st.CFL == ref.CFL
+    # step the oracle from its own TGV init and compare with the reference's states
+    dts = []
+    for s in (1, 2, 10):
+        dts.extend(st.step(s - (0 if not dts else len(dts))))
+        ref = state_from_npz(z, f"s{s}")
+        for name in FIELDS:
+            assert bits_equal(st[name], ref[name]), f"step {s}: " + describe_mismatch(name, st[name], ref[name])
+    assert np.array_equal(np.array(dts), z["series_dt"][1:11])
+
+
+def test_tgv16_digest():
+    dig = json.load(open(os.path.join(GOLDEN, "tgv_digests.json")))["runs"]["16^3x10"]
+    st = OracleState.tgv(16)
+    dts = st.step(10)
+    for name in FIELDS:
+        h = hashlib.sha256(np.ascontiguousarray(st[name], dtype="<f8").tobytes()).hexdigest()
+        assert h == dig["arrays"][name]["sha256"], name
+    assert list(dts) == [float.fromhex(v) for v in dig["series"]["dt"][1:]]
+
+
+def test_energy_and_mass_series():
+    """KE (reference definition, volume (2 pi)^3) and mass of the oracle follow
+    the reference's per-step series (sums in the same loop order: bitwise)."""
+    z = np.load(os.path.join(GOLDEN, "tgv8.npz"))
+    st = OracleState.tgv(8)
+    L = 2.0 * np.pi - 0.0
+
+    def ke(s):  # harness: allEnergy * (x3_t-x3_b) * (x2_n-x2_s) * (x1_e-x1_w), left to right
+        return s.energy_sum() * L * L * L
+
+    assert ke(st) == z["series_ke"][0]
+    assert st.mass() == z["series_mass"][0]
+    for s in range(1, 11):
+        st.step(1)
+        assert ke(st) == z["series_ke"][s]
+        assert st.mass() == z["series_mass"][s]
+
+
+@pytest.mark.slow
+def test_tgv64_digest():
+    dig = json.load(open(os.path.join(GOLDEN, "tgv_digests.json")))["runs"]["64^3x200"]
+    st = OracleState.tgv(64)
+    dts = st.step(200)
+    assert list(dts) == [float.fromhex(v) for v in dig["series"]["dt"][1:]]
+    for name in ("ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon", "u11", "p1", "u22", "p3"):
+        h = hashlib.sha256(np.ascontiguousarray(st[name], dtype="<f8").tobytes()).hexdigest()
+        assert h == dig["arrays"][name]["sha256"], name
