@@ -132,6 +132,15 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
         if (s == ARK_OK) c->allocs.push_back(c->S);
     }
     if (s == ARK_OK) {
+        double* bases[9];
+        for (int q = 0; q < 9; ++q) bases[q] = c->cur[F_sigm11 + q] - kOriginPad;
+        s = c->hip(hipMalloc(&c->sigma_ptrs, sizeof bases), "hipMalloc sigma table");
+        if (s == ARK_OK) {
+            c->allocs.push_back(c->sigma_ptrs);
+            s = c->hip(hipMemcpy(c->sigma_ptrs, bases, sizeof bases, hipMemcpyHostToDevice), "sigma table");
+        }
+    }
+    if (s == ARK_OK) {
         c->diag_blocks = 1024;
         s = c->hip(hipMalloc(&c->diag_part, (5 * c->diag_blocks + 8) * sizeof(double)), "hipMalloc diag");
         if (s == ARK_OK) c->allocs.push_back(c->diag_part);
@@ -219,6 +228,7 @@ ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2,
         for (long long q = 0; q < n; ++q)
             if (host[q] != 0.0) { c->has_solid = true; break; }
     }
+    if (field >= F_sigm11) c->sigma_dirty = true;
     s = copy_planes(c, c->cur[field], host, nullptr, e1, e2, e3);
     if (s == ARK_OK && c->alt[field]) s = copy_planes(c, c->alt[field], host, nullptr, e1, e2, e3);
     return s;
@@ -279,6 +289,10 @@ hipError_t run_stress_ghost(void* p) {
         plan.n++;
     }
     return launch_ghost(c->stream, c->geo, plan);
+}
+hipError_t run_sigma_zero(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_zero(c->stream, c->sigma_ptrs, 9, (long long)c->elems);
 }
 hipError_t run_stress_faces(void* p) {
     ark_ctx* c = (ark_ctx*)p;
@@ -364,6 +378,10 @@ ark_status ark_phase1(ark_ctx* c) {
 ark_status ark_stress_tensor(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
+    if (s == ARK_OK && c->sigma_dirty) {
+        s = c->launch("sigma_zero", run_sigma_zero, c);
+        if (s == ARK_OK) c->sigma_dirty = false;
+    }
     if (s == ARK_OK) s = c->launch("ghost_stress", run_stress_ghost, c);
     if (s == ARK_OK) s = c->launch("stress_faces", run_stress_faces, c);
     if (s == ARK_OK) s = c->launch("forces", run_forces, c);

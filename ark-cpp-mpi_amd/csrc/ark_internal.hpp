@@ -27,6 +27,8 @@ struct ark_ctx {
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;
+    bool sigma_dirty = false;          // an upload left non-zero values in a stress array
+    double** sigma_ptrs = nullptr;     // device array: allocation bases of the nine stress arrays
     bool profiling = false;
     std::vector<ark_prof_event> prof_events;
     std::vector<void*> allocs;

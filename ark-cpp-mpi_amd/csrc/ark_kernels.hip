@@ -208,6 +208,17 @@ __global__ void __launch_bounds__(256) k_stress_faces(Geo g, Fields F, Consts K)
     }
 }
 
+// The reference zeroes the nine stress arrays over their full extent before
+// every evaluation (source/main.cpp:579-595).  The face kernel above writes
+// only computed faces and nothing else ever writes a stress array, so a
+// context zeroes them once after any upload made them non-zero.
+__global__ void __launch_bounds__(256) k_zero(double* const* arrs, int narr, long long n) {
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (int a = 0; a < narr; ++a)
+        for (long long q = t; q < n; q += stride) arrs[a][q] = 0.0;
+}
+
 // Friction force (source/main.cpp:995-1023).
 __global__ void __launch_bounds__(256) k_forces(Geo g, Fields F, Consts K) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -480,6 +491,11 @@ hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan) {
 
 hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K) {
     hipLaunchKernelGGL(k_stress_faces, cell_grid(g.N1 + 1, g.N2 + 1, g.N3 + 1), kBlock, 0, st, g, F, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(hipStream_t st, double* const* dev_ptrs, int narr, long long n) {
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, st, dev_ptrs, narr, n);
     return hipGetLastError();
 }
 

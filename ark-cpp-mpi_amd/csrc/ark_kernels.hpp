@@ -34,6 +34,8 @@ hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
 hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan);
 hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
+// Zero narr arrays of n elements each; dev_ptrs is a device array of pointers.
+hipError_t launch_zero(hipStream_t st, double* const* dev_ptrs, int narr, long long n);
 hipError_t launch_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_phase2(hipStream_t st, int axis, const Geo& g, const Fields& F, const NextFaces& nx,

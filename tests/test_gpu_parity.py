@@ -55,7 +55,7 @@ def random_fluid_state(N, seed, dt=0.01) -> OracleState:
 
 
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("N", [(8, 8, 8), (6, 5, 4), (17, 9, 12)])
+@pytest.mark.parametrize("N", [(8, 8, 8), (6, 5, 4), (17, 9, 12), (12, 10, 8), (15, 20, 30)])
 def test_init_tgv_bitwise(N):
     ref = OracleState.tgv(*N)
     with arkamd.Solver(*N) as s:
@@ -221,5 +221,5 @@ def test_errors_are_reported():
             s.phase2()
         assert e.value.status == arkamd.ARK_ERR_UNSUPPORTED
     with pytest.raises(arkamd.ArkError) as e:
-        arkamd.Solver(0, 4, 4)
+        arkamd.Solver(0, 4, 4, dx=(1.0, 1.0, 1.0))
     assert e.value.status == arkamd.ARK_ERR_INVALID

@@ -113,3 +113,25 @@ def test_tgv64_digest():
     for name in ("ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon", "u11", "p1", "u22", "p3"):
         h = hashlib.sha256(np.ascontiguousarray(st[name], dtype="<f8").tobytes()).hexdigest()
         assert h == dig["arrays"][name]["sha256"], name
+
+
+HARNESS = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "ark_ref_harness")
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+@pytest.mark.skipif(not (os.path.exists(HARNESS) and os.path.exists(MPIRUN)),
+                    reason="reference build (oracle/_ref) or mpirun not present")
+@pytest.mark.parametrize("N", [(12, 10, 8), (9, 7, 5)])
+def test_live_reference_nonpow2(tmp_path, N):
+    """Live run of the reference build on a non-cubic, non-power-of-two box:
+    initial state and 3 steps, all 41 arrays bitwise."""
+    import subprocess
+    subprocess.run([MPIRUN, "-n", "1", HARNESS, "tgv", *map(str, N), "3", str(tmp_path), "0", "3"],
+                   check=True, capture_output=True)
+    st = OracleState.tgv(*N)
+    for s in (0, 3):
+        if s:
+            st.step(3)
+        ref = OracleState.from_dir(os.path.join(tmp_path, f"s{s}"))
+        for name in FIELDS:
+            assert bits_equal(st[name], ref[name]), f"step {s}: " + describe_mismatch(name, st[name], ref[name])
