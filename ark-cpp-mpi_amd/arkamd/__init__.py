@@ -44,7 +44,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTED_SYMBOLS = [
     "ark_create", "ark_destroy", "ark_last_error", "ark_upload", "ark_download", "ark_set_dt",
     "ark_get_dt", "ark_time_step_size", "ark_phase1", "ark_stress_tensor", "ark_use_forces",
-    "ark_phase2", "ark_step", "ark_step_phases", "ark_diagnostics", "ark_init_tgv",
+    "ark_phase2", "ark_step", "ark_advance", "ark_step_phases", "ark_diagnostics", "ark_init_tgv",
     "ark_synchronize", "ark_get_stream", "ark_profile_enable", "ark_profile_read",
     "ark_profile_reset", "ark_device_bytes",
 ]
@@ -112,6 +112,7 @@ def lib() -> ctypes.CDLL:
         getattr(L, fn).argtypes = [vp]
     L.ark_step.argtypes = [vp, ip]
     L.ark_step_phases.argtypes = [vp, ip]
+    L.ark_advance.argtypes = [vp, ip]
     L.ark_diagnostics.argtypes = [vp, ctypes.POINTER(Diag)]
     L.ark_init_tgv.argtypes = [vp, ctypes.c_double]
     L.ark_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
@@ -246,6 +247,10 @@ class Solver:
     def step(self, nsteps: int = 1) -> None:
         """nsteps of the driver loop (source/main.cpp:82-91), fused kernels."""
         self._check(self._L.ark_step(self._h, int(nsteps)))
+
+    def advance(self, nsteps: int = 1) -> None:
+        """nsteps fused steps maintaining only the persistent state (ark_advance)."""
+        self._check(self._L.ark_advance(self._h, int(nsteps)))
 
     def step_phases(self, nsteps: int = 1) -> None:
         """nsteps through one launch per reference phase."""

@@ -24,6 +24,9 @@ bool is_face_field(int f) { return f >= F_ro1; }
 // The eighteen persistent face fields are double-buffered (Phase2 reads the
 // old faces and writes the new ones).
 bool is_face_state(int f) { return f >= F_ro1 && f <= F_p3; }
+// The level-n cell fields are double-buffered too: the fused step reads
+// them around each cell while it writes level n+1.
+bool is_double_buffered(int f) { return is_face_state(f) || (f >= F_ronCon && f <= F_tnCon); }
 
 const char* kFieldNames[F_NUM] = {
     "roCon", "u1Con", "u2Con", "u3Con", "tCon", "ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon",
@@ -115,7 +118,7 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
     ark_status s = c->hip(hipSetDevice(p->device), "hipSetDevice");
     if (s == ARK_OK) s = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (int f = 0; f < F_NUM && s == ARK_OK; ++f) {
-        int nbuf = is_face_state(f) ? 2 : 1;
+        int nbuf = is_double_buffered(f) ? 2 : 1;
         for (int b = 0; b < nbuf && s == ARK_OK; ++b) {
             double* mem = nullptr;
             s = c->hip(hipMalloc(&mem, c->elems * sizeof(double)), "hipMalloc field");
@@ -229,6 +232,10 @@ ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2,
             if (host[q] != 0.0) { c->has_solid = true; break; }
     }
     if (field >= F_sigm11) c->sigma_dirty = true;
+    if (field == F_ro1 || field == F_ro2 || field == F_ro3 || field == F_p1 || field == F_p2 || field == F_p3) {
+        c->rho_consistent = false;
+        c->rho_stored = true;
+    }
     s = copy_planes(c, c->cur[field], host, nullptr, e1, e2, e3);
     if (s == ARK_OK && c->alt[field]) s = copy_planes(c, c->alt[field], host, nullptr, e1, e2, e3);
     return s;
@@ -238,7 +245,8 @@ ark_status ark_download(ark_ctx* c, int field, double* host, int e1, int e2, int
     if (!c || !host) return ARK_ERR_INVALID;
     ark_status s = check_extents(c, field, e1, e2, e3);
     if (s != ARK_OK) return s;
-    s = c->hip(hipStreamSynchronize(c->stream), "sync before download");
+    if (field == F_ro1 || field == F_ro2 || field == F_ro3) s = ark_refresh_rho(c);
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync before download");
     if (s != ARK_OK) return s;
     return copy_planes(c, c->cur[field], nullptr, host, e1, e2, e3);
 }
@@ -353,18 +361,66 @@ static ark_status require_fluid(ark_ctx* c) {
     return ARK_OK;
 }
 
-static ark_status time_step_size(ark_ctx* c, int advance) {
+ark_status ark_cfl_finalize(ark_ctx* c, int advance) {
+    FinArgs f{c, advance};
+    return c->launch("cfl_finalize", run_cfl_fin, &f);
+}
+
+ark_status ark_cfl_step(ark_ctx* c, int advance) {
     CflArgs a{c};
     ark_status s = c->launch("cfl", run_cfl, &a);
-    FinArgs f{c, advance};
-    if (s == ARK_OK) s = c->launch("cfl_finalize", run_cfl_fin, &f);
+    return s == ARK_OK ? ark_cfl_finalize(c, advance) : s;
+}
+
+ark_status ark_begin_step(ark_ctx* c) {
+    if (c->dt_pending) {
+        c->dt_pending = false;
+        return ark_cfl_finalize(c, 1);
+    }
+    return ark_cfl_step(c, 1);
+}
+
+// A phase call after fused steps: the pending dt no longer describes Con.
+static ark_status drop_pending(ark_ctx* c) {
+    if (!c->dt_pending) return ARK_OK;
+    c->dt_pending = false;
+    return ark_cfl_finalize(c, 2);
+}
+
+hipError_t run_rho(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_rho_faces(c->stream, c->geo, c->fields(), c->K);
+}
+
+ark_status ark_refresh_rho(ark_ctx* c) {
+    if (c->rho_stored) return ARK_OK;
+    ark_status s = c->launch("rho_faces", run_rho, c);
+    if (s == ARK_OK) c->rho_stored = true;
+    return s;
+}
+
+ark_status ark_phase_body(ark_ctx* c) {
+    ark_status s = ark_phase1(c);
+    if (s == ARK_OK) s = ark_stress_tensor(c);
+    if (s == ARK_OK) s = ark_use_forces(c);
+    if (s == ARK_OK) s = ark_phase2(c);
+    if (s == ARK_OK) s = ark_phase1(c);
+    if (s == ARK_OK) s = ark_use_forces(c);
     return s;
 }
 
 ark_status ark_time_step_size(ark_ctx* c, double* dt_out) {
     if (!c) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
-    if (s == ARK_OK) s = time_step_size(c, 0);
+    if (s == ARK_OK) {
+        if (c->dt_pending) {
+            // after fused steps Con is not stored; their MIN is TimeStepSize's result
+            c->dt_pending = false;
+            s = ark_cfl_finalize(c, 0);
+        } else {
+            s = ark_cfl_step(c, 0);
+        }
+    }
     if (s == ARK_OK && dt_out) s = ark_get_dt(c, dt_out);
     return s;
 }
@@ -372,12 +428,15 @@ ark_status ark_time_step_size(ark_ctx* c, double* dt_out) {
 ark_status ark_phase1(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = drop_pending(c);
+    if (s == ARK_OK) s = ark_refresh_rho(c);
     return s == ARK_OK ? c->launch("phase1", run_phase1, c) : s;
 }
 
 ark_status ark_stress_tensor(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = drop_pending(c);
     if (s == ARK_OK && c->sigma_dirty) {
         s = c->launch("sigma_zero", run_sigma_zero, c);
         if (s == ARK_OK) c->sigma_dirty = false;
@@ -391,20 +450,25 @@ ark_status ark_stress_tensor(ark_ctx* c) {
 ark_status ark_use_forces(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = drop_pending(c);
     return s == ARK_OK ? c->launch("use_forces", run_use_forces, c) : s;
 }
 
 ark_status ark_phase2(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = drop_pending(c);
     if (s == ARK_OK) s = c->launch("ghost_phase2", run_phase2_ghost, c);
     static const char* names[3] = {"phase2_x", "phase2_y", "phase2_z"};
     for (int d = 0; d < 3 && s == ARK_OK; ++d) {
         P2Args a{c, d};
         s = c->launch(names[d], run_phase2_axis, &a);
     }
-    if (s == ARK_OK)
+    if (s == ARK_OK) {
         for (int f = F_ro1; f <= F_p3; ++f) std::swap(c->cur[f], c->alt[f]);
+        c->rho_consistent = !c->has_solid;
+        c->rho_stored = true;
+    }
     return s;
 }
 
@@ -412,13 +476,8 @@ ark_status ark_step_phases(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
     for (int n = 0; n < nsteps && s == ARK_OK; ++n) {
-        s = time_step_size(c, 1);
-        if (s == ARK_OK) s = ark_phase1(c);
-        if (s == ARK_OK) s = ark_stress_tensor(c);
-        if (s == ARK_OK) s = ark_use_forces(c);
-        if (s == ARK_OK) s = ark_phase2(c);
-        if (s == ARK_OK) s = ark_phase1(c);
-        if (s == ARK_OK) s = ark_use_forces(c);
+        s = ark_begin_step(c);
+        if (s == ARK_OK) s = ark_phase_body(c);
     }
     return s;
 }
@@ -493,6 +552,9 @@ ark_status ark_init_tgv(ark_ctx* c, double U0) {
     if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     hipFree(dtab);
     c->has_solid = false;
+    c->rho_consistent = true;
+    c->rho_stored = true;
+    c->sigma_dirty = false;
     if (s == ARK_OK) {
         // fresh run: TIME = 0, nStep = 0, running MIN reset
         Scalars hs{};

@@ -75,6 +75,22 @@ __device__ __forceinline__ double max3d(double x1, double x2, double x3) {
     return x1;
 }
 
+__device__ __forceinline__ unsigned long long umin64(unsigned long long a, unsigned long long b) {
+    return a < b ? a : b;
+}
+
+// Block-wide MIN of `bits` into *slot: an LDS atomic per thread (valid for
+// any block size, including a partial last wavefront, whose inactive lanes a
+// cross-lane shuffle would read), then one global atomic per block.
+__device__ __forceinline__ void block_min_atomic(unsigned long long bits, unsigned long long* slot) {
+    __shared__ unsigned long long block_min;
+    if (threadIdx.x == 0) block_min = ~0ull;
+    __syncthreads();
+    if (bits != ~0ull) atomicMin(&block_min, bits);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_min != ~0ull) atomicMin(slot, block_min);
+}
+
 // Numeric constants of a context.
 struct Consts {
     double dx[3];

@@ -1,4 +1,468 @@
-// Fused step driver.  (Placeholder body: the phase-by-phase sequence.)
+// Fused CABARET/ARK step: one kernel per time step.
+//
+// Reads the persistent state once (five level-n cell fields and the five
+// face fields u1,u2,u3,p,t of each orientation) and writes it once (level
+// n+1 cells into the other cell buffer, new faces into the other face
+// buffer).  Everything the reference stores in between -- the half-level
+// cells, the stresses, the forces, the face densities -- stays on chip.
+//
+// Blocking ("2.5-D"): a block owns TX x TY columns of cells and marches a
+// chunk of LZ planes in z.  Its threads cover the owned columns plus a ring
+// of one cell (RX x RY threads), because the new faces on the tile's edge
+// need the half-level state of the neighbouring column, which the ring
+// threads recompute.  Per plane the block computes, in order:
+//   stress force f and the predictor (Phase1 + UseForces) for every thread's
+//   cell, the Phase2 extrapolations of every cell (forward and backward, per
+//   axis), the new x- and y-faces (LDS exchange of the neighbour's forward
+//   extrapolation), the new z-face (registers: the previous plane's forward
+//   extrapolation), and the corrector (Phase1 + UseForces) of the previous
+//   plane, whose top face has just been formed.
+// Flux terms such as rho*u1*u2 are computed once per face and used by both
+// cells that share it; the operands and their association are the
+// reference's, so FP64 results are bitwise equal to the phase-by-phase path.
+//
+// Periodic box: cell indices wrap (ghost cell 0 is cell N, N+1 is cell 1,
+// which is what the reference's halo exchange delivers), face N+1 is a real
+// face, u/p ghost faces 0 and N+2 map to faces N and 2, and t ghost faces are
+// read from storage because the reference never refreshes them.
 #include "ark_internal.hpp"
+#include "ark_kernels.hpp"
 
-ark_status ark_step_fused(ark_ctx* c, int nsteps) { return ark_step_phases(c, nsteps); }
+namespace arkd {
+
+constexpr int TX = 32, TY = 8;
+constexpr int RX = TX + 2, RY = TY + 2;
+constexpr int NTHR = RX * RY;   // 340 threads: 6 wavefronts
+
+struct FusedArgs {
+    Geo g;
+    Consts K;
+    double cfl_dx[3];          // CFL*dx_d, the numerators of the next step's CFL bound
+    const double* A[5];        // level-n cells: rho, u1, u2, u3, t
+    double* Ao[5];             // level n+1 cells
+    const double* F[3][5];     // old faces normal to d: u1, u2, u3, p, t
+    double* Fo[3][5];          // new faces
+    const Scalars* S;          // S->dt is this step's dt
+    unsigned long long* dt_next;
+    int LZ;
+};
+
+__device__ __forceinline__ int wrapc(int q, int N) { return q < 1 ? q + N : (q > N ? q - N : q); }
+__device__ __forceinline__ int mapf(int f, int N) { return f == 0 ? N : (f == N + 2 ? 2 : f); }
+
+struct Cell5 { double ro, u1, u2, u3, t; };
+struct Face5 { double u1, u2, u3, p, t; };
+struct Flux5 { double m[5]; };
+
+__device__ __forceinline__ Cell5 load_cell(const FusedArgs& a, int idx) {
+    Cell5 c;
+    c.ro = a.A[0][idx]; c.u1 = a.A[1][idx]; c.u2 = a.A[2][idx]; c.u3 = a.A[3][idx]; c.t = a.A[4][idx];
+    return c;
+}
+
+__device__ __forceinline__ Face5 load_face(const FusedArgs& a, int d, int idx) {
+    Face5 f;
+    f.u1 = a.F[d][0][idx]; f.u2 = a.F[d][1][idx]; f.u3 = a.F[d][2][idx]; f.p = a.F[d][3][idx]; f.t = a.F[d][4][idx];
+    return f;
+}
+
+// The five flux terms of a face normal to D with density rho (the operands of
+// source/main.cpp:530-557, one face at a time).
+template <int D>
+__device__ __forceinline__ Flux5 flux_terms(double ro, double u1, double u2, double u3, double p, double t) {
+    Flux5 q;
+    const double un = D == 0 ? u1 : D == 1 ? u2 : u3;
+    q.m[0] = ro * un;
+    q.m[1] = D == 0 ? ro * u1 * u1 + p : ro * u1 * un;
+    q.m[2] = D == 1 ? ro * u2 * u2 + p : ro * u2 * un;
+    q.m[3] = D == 2 ? ro * u3 * u3 + p : ro * u3 * un;
+    q.m[4] = ro * t * un;
+    return q;
+}
+
+template <int D>
+__device__ __forceinline__ Flux5 old_flux(const Face5& f, const Consts& K) {
+    double ro = K.ro0 + f.p / (K.sound * K.sound);   // the reference's face density, source/main.cpp:1930
+    return flux_terms<D>(ro, f.u1, f.u2, f.u3, f.p, f.t);
+}
+
+// Conservative half-step from the flux differences (Phase1's update) followed
+// by the force half-step (UseForces).
+__device__ __forceinline__ Cell5 conservative_update(const Cell5& c, const double S[5], const double f[3], double dt,
+                                                     const Consts& K) {
+    const double dvc = K.dvc;
+    Cell5 n;
+    double ro_cn = (c.ro * dvc - 0.5 * dt * S[0]) / dvc;
+    double u1_cn = (c.ro * c.u1 * dvc - 0.5 * dt * S[1]) / (ro_cn * dvc);
+    double u2_cn = (c.ro * c.u2 * dvc - 0.5 * dt * S[2]) / (ro_cn * dvc);
+    double u3_cn = (c.ro * c.u3 * dvc - 0.5 * dt * S[3]) / (ro_cn * dvc);
+    double t_cn = (c.ro * c.t * dvc - 0.5 * dt * S[4]) / (dvc * ro_cn);
+    n.ro = ro_cn;
+    n.u1 = (ro_cn * dvc * u1_cn + 0.5 * dt * f[0]) / (dvc * ro_cn);
+    n.u2 = (ro_cn * dvc * u2_cn + 0.5 * dt * f[1]) / (dvc * ro_cn);
+    n.u3 = (ro_cn * dvc * u3_cn + 0.5 * dt * f[2]) / (dvc * ro_cn);
+    n.t = t_cn;
+    return n;
+}
+
+__device__ __forceinline__ double ucomp(const Cell5& c, int m) { return m == 0 ? c.u1 : m == 1 ? c.u2 : c.u3; }
+
+// CellSide of a cell for the sweep along D from its level-n (a) and
+// half-level (h) values and its two faces along D.
+template <int D>
+__device__ __forceinline__ CellSide make_side(const Cell5& a, const Cell5& h, const Face5& fb, const Face5& ff) {
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    auto fu = [](const Face5& f, int m) { return m == 0 ? f.u1 : m == 1 ? f.u2 : f.u3; };
+    CellSide s;
+    s.un_f = fu(ff, D); s.un_b = fu(fb, D); s.un_cn = ucomp(h, D); s.un_c = ucomp(a, D);
+    s.ua_f = fu(ff, TA); s.ua_b = fu(fb, TA); s.ua_cn = ucomp(h, TA); s.ua_c = ucomp(a, TA);
+    s.ub_f = fu(ff, TB); s.ub_b = fu(fb, TB); s.ub_cn = ucomp(h, TB); s.ub_c = ucomp(a, TB);
+    s.ro_cn = h.ro; s.ro_c = a.ro;
+    s.t_f = ff.t; s.t_b = fb.t; s.t_cn = h.t; s.t_c = a.t;
+    s.p_f = ff.p; s.p_b = fb.p;
+    return s;
+}
+
+// New face along D from the forward half of the cell behind (fw, hb) and the
+// backward half of the cell ahead (bw, hf); old: the face's old values.
+template <int D>
+__device__ __forceinline__ FaceOut new_face(const Fwd& fw, const Cell5& hb, const Bwd& bw, const Cell5& hf,
+                                            const Face5& old, const Consts& K) {
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    auto fu = [](const Face5& f, int m) { return m == 0 ? f.u1 : m == 1 ? f.u2 : f.u3; };
+    return face_combine(fw, bw, ucomp(hb, D), ucomp(hf, D), hb.t, hf.t, ucomp(hb, TA), ucomp(hf, TA), ucomp(hb, TB),
+                        ucomp(hf, TB), old.t, fu(old, TA), fu(old, TB), K);
+}
+
+template <int D>
+__device__ __forceinline__ void store_face(const FusedArgs& a, int idx, const FaceOut& o) {
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    a.Fo[D][D][idx] = o.un;
+    a.Fo[D][TA][idx] = o.a;
+    a.Fo[D][TB][idx] = o.b;
+    a.Fo[D][3][idx] = o.p;
+    a.Fo[D][4][idx] = o.t;
+}
+
+template <int D>
+__device__ __forceinline__ Flux5 new_flux(const FaceOut& o) {
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    double u[3];
+    u[D] = o.un; u[TA] = o.a; u[TB] = o.b;
+    return flux_terms<D>(o.ro, u[0], u[1], u[2], o.p, o.t);
+}
+
+// Per-thread LDS record of one plane.
+struct LdsCell {
+    double h[5];        // half-level rho, u1, u2, u3, t
+    double fx[4];       // forward x-extrapolation (r, t, a, b)
+    double fy[4];       // forward y-extrapolation
+};
+struct LdsFlux {
+    double mx[5], my[5];
+};
+union LdsSlot {
+    LdsCell c;
+    LdsFlux f;
+};
+
+__global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
+    __shared__ LdsSlot lds[NTHR];
+    const Geo& g = a.g;
+    const Consts& K = a.K;
+    const double dt = a.S->dt;
+    const int tid = threadIdx.x;
+    const int tx = tid % RX, ty = tid / RX;
+    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1, k0 = blockIdx.z * a.LZ + 1;
+    const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = min(k0 + a.LZ, g.N3 + 1);
+    // logical cell of this thread (clamped into the region so every access stays in bounds)
+    const int i = min(i0 - 1 + tx, iend), j = min(j0 - 1 + ty, jend);
+    const bool in_region = (i0 - 1 + tx <= iend) && (j0 - 1 + ty <= jend);
+    const bool own_x = i >= i0 && i < iend, own_y = j >= j0 && j < jend;
+    const bool owned = in_region && own_x && own_y;
+    const bool xface = in_region && own_y && i >= i0 && i <= iend;     // computes the x-face at i
+    const bool yface = in_region && own_x && j >= j0 && j <= jend;     // computes the y-face at j
+    const bool xface_store = xface && (i < iend || iend == g.N1 + 1);
+    const bool yface_store = yface && (j < jend || jend == g.N2 + 1);
+    const int iw = wrapc(i, g.N1), jw = wrapc(j, g.N2);
+    const int iwm = wrapc(i - 1, g.N1), iwp = wrapc(i + 1, g.N1);
+    const int jwm = wrapc(j - 1, g.N2), jwp = wrapc(j + 1, g.N2);
+    const int px = g.px, sxy = g.sxy;
+    const double VIS = K.VIS;
+
+    // rolling state along z
+    Cell5 A_cur = load_cell(a, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
+    double sz_lo[3];
+    {
+        Cell5 A_below = load_cell(a, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
+        sz_lo[0] = VIS * (A_cur.u1 - A_below.u1) / K.dx[2];
+        sz_lo[1] = VIS * (A_cur.u2 - A_below.u2) / K.dx[2];
+        sz_lo[2] = VIS * (A_cur.u3 - A_below.u3) / K.dx[2];
+    }
+    Cell5 H_prev{};
+    Fwd fz_prev{};
+    double f_prev[3] = {0, 0, 0};
+    double S_prev[5] = {0, 0, 0, 0, 0};
+    Flux5 mz_prev{};
+    unsigned long long dtbits = ~0ull;
+
+    for (int p = k0 - 1; p <= kend; ++p) {
+        const int pw = wrapc(p, g.N3);
+        const int c = at(g, iw, jw, pw);
+        const Cell5 A_next = load_cell(a, at(g, iw, jw, wrapc(p + 1, g.N3)));
+
+        // ---- stress force f (StressTensor, source/main.cpp:856-1023) -------
+        double f[3];
+        {
+            const Cell5 Aw = load_cell(a, at(g, iwm, jw, pw)), Ae = load_cell(a, at(g, iwp, jw, pw));
+            const Cell5 As = load_cell(a, at(g, iw, jwm, pw)), An = load_cell(a, at(g, iw, jwp, pw));
+            double sz_hi[3];
+            for (int m = 0; m < 3; ++m) {
+                const double uc = ucomp(A_cur, m);
+                double sx_lo = VIS * (uc - ucomp(Aw, m)) / K.dx[0];
+                double sx_hi = VIS * (ucomp(Ae, m) - uc) / K.dx[0];
+                double sy_lo = VIS * (uc - ucomp(As, m)) / K.dx[1];
+                double sy_hi = VIS * (ucomp(An, m) - uc) / K.dx[1];
+                sz_hi[m] = VIS * (ucomp(A_next, m) - uc) / K.dx[2];
+                f[m] = ((sx_hi - sx_lo) * K.ds1 + (sy_hi - sy_lo) * K.ds2 + (sz_hi[m] - sz_lo[m]) * K.ds3) * A_cur.ro;
+            }
+            for (int m = 0; m < 3; ++m) sz_lo[m] = sz_hi[m];
+        }
+
+        // ---- predictor: Phase1 + UseForces with the old faces --------------
+        const Face5 Xw = load_face(a, 0, c), Xe = load_face(a, 0, c + 1);
+        const Face5 Ys = load_face(a, 1, c), Yn = load_face(a, 1, c + px);
+        const Face5 Zb = load_face(a, 2, c), Zt = load_face(a, 2, c + sxy);
+        Cell5 H;
+        {
+            const Flux5 fxw = old_flux<0>(Xw, K), fxe = old_flux<0>(Xe, K);
+            const Flux5 fys = old_flux<1>(Ys, K), fyn = old_flux<1>(Yn, K);
+            const Flux5 fzb = old_flux<2>(Zb, K), fzt = old_flux<2>(Zt, K);
+            double S[5];
+            for (int v = 0; v < 5; ++v)
+                S[v] = (fxe.m[v] - fxw.m[v]) * K.ds1 + (fyn.m[v] - fys.m[v]) * K.ds2 + (fzt.m[v] - fzb.m[v]) * K.ds3;
+            H = conservative_update(A_cur, S, f, dt, K);
+        }
+        if (owned && p >= k0 && p < kend) {
+            // the next step's CFL bound from the post-force half level (TimeStepSize, source/main.cpp:379-405)
+            double d1 = a.cfl_dx[0] / (K.sound + fabs(H.u1));
+            double d2 = a.cfl_dx[1] / (K.sound + fabs(H.u2));
+            double d3 = a.cfl_dx[2] / (K.sound + fabs(H.u3));
+            unsigned long long b = (unsigned long long)__double_as_longlong(min3d(d1, d2, d3));
+            dtbits = b < dtbits ? b : dtbits;
+        }
+
+        // ---- Phase2 extrapolations of this cell along z ---------------------
+        // faces of the logical cell p: ghost faces map for u/p, t is read as stored
+        Face5 Pzb = Zb, Pzf = Zt;
+        if (p != pw) {
+            const int cl = at(g, iw, jw, 0);
+            Pzb = load_face(a, 2, cl + mapf(p, g.N3) * sxy);
+            Pzf = load_face(a, 2, cl + mapf(p + 1, g.N3) * sxy);
+            Pzb.t = a.F[2][4][cl + p * sxy];
+            Pzf.t = a.F[2][4][cl + (p + 1) * sxy];
+        }
+        const CellSide sz = make_side<2>(A_cur, H, Pzb, Pzf);
+        const Fwd fz = cell_forward(sz, dt, K.dx[2], K);
+
+        // ---- x/y faces and the in-plane part of the corrector ---------------
+        double S_xy[5] = {0, 0, 0, 0, 0};
+        if (p >= k0 && p < kend) {
+            Face5 Pxb = Xw, Pxf = Xe, Pyb = Ys, Pyf = Yn;
+            if (i != iw) {
+                const int cl = at(g, 0, jw, pw);
+                Pxb = load_face(a, 0, cl + mapf(i, g.N1));
+                Pxf = load_face(a, 0, cl + mapf(i + 1, g.N1));
+                Pxb.t = a.F[0][4][cl + i];
+                Pxf.t = a.F[0][4][cl + i + 1];
+            }
+            if (j != jw) {
+                const int cl = at(g, iw, 0, pw);
+                Pyb = load_face(a, 1, cl + mapf(j, g.N2) * px);
+                Pyf = load_face(a, 1, cl + mapf(j + 1, g.N2) * px);
+                Pyb.t = a.F[1][4][cl + j * px];
+                Pyf.t = a.F[1][4][cl + (j + 1) * px];
+            }
+            const CellSide sx = make_side<0>(A_cur, H, Pxb, Pxf);
+            const CellSide sy = make_side<1>(A_cur, H, Pyb, Pyf);
+            const Fwd fx = cell_forward(sx, dt, K.dx[0], K);
+            const Fwd fy = cell_forward(sy, dt, K.dx[1], K);
+            LdsCell& me = lds[tid].c;
+            me.h[0] = H.ro; me.h[1] = H.u1; me.h[2] = H.u2; me.h[3] = H.u3; me.h[4] = H.t;
+            me.fx[0] = fx.r; me.fx[1] = fx.t; me.fx[2] = fx.a; me.fx[3] = fx.b;
+            me.fy[0] = fy.r; me.fy[1] = fy.t; me.fy[2] = fy.a; me.fy[3] = fy.b;
+            __syncthreads();
+            Flux5 mx{}, my{};
+            if (xface) {
+                const LdsCell& nb = lds[tid - 1].c;
+                Cell5 hb{nb.h[0], nb.h[1], nb.h[2], nb.h[3], nb.h[4]};
+                Fwd fwb{nb.fx[0], nb.fx[1], nb.fx[2], nb.fx[3]};
+                const Bwd bw = cell_backward<true>(sx, dt, K.dx[0], K);
+                const FaceOut o = new_face<0>(fwb, hb, bw, H, Pxb, K);
+                if (xface_store) store_face<0>(a, at(g, i, j, p), o);
+                mx = new_flux<0>(o);
+            }
+            if (yface) {
+                const LdsCell& nb = lds[tid - RX].c;
+                Cell5 hb{nb.h[0], nb.h[1], nb.h[2], nb.h[3], nb.h[4]};
+                Fwd fwb{nb.fy[0], nb.fy[1], nb.fy[2], nb.fy[3]};
+                const Bwd bw = cell_backward<false>(sy, dt, K.dx[1], K);
+                const FaceOut o = new_face<1>(fwb, hb, bw, H, Pyb, K);
+                if (yface_store) store_face<1>(a, at(g, i, j, p), o);
+                my = new_flux<1>(o);
+            }
+            __syncthreads();
+            LdsFlux& mf = lds[tid].f;
+            for (int v = 0; v < 5; ++v) { mf.mx[v] = mx.m[v]; mf.my[v] = my.m[v]; }
+            __syncthreads();
+            if (owned) {
+                const LdsFlux& e = lds[tid + 1].f;
+                const LdsFlux& n = lds[tid + RX].f;
+                for (int v = 0; v < 5; ++v) S_xy[v] = (e.mx[v] - mx.m[v]) * K.ds1 + (n.my[v] - my.m[v]) * K.ds2;
+            }
+            __syncthreads();
+        }
+
+        // ---- z-face p and the corrector of plane p-1 ------------------------
+        if (p >= k0) {
+            const Bwd bz = cell_backward<true>(sz, dt, K.dx[2], K);
+            const FaceOut o = new_face<2>(fz_prev, H_prev, bz, H, Pzb, K);
+            const Flux5 mz = new_flux<2>(o);
+            if (owned) {
+                if (p < kend || kend == g.N3 + 1) store_face<2>(a, at(g, i, j, p), o);
+                if (p > k0) {
+                    double S[5];
+                    for (int v = 0; v < 5; ++v) S[v] = S_prev[v] + (mz.m[v] - mz_prev.m[v]) * K.ds3;
+                    const Cell5 An = conservative_update(H_prev, S, f_prev, dt, K);
+                    const int co = at(g, i, j, p - 1);
+                    a.Ao[0][co] = An.ro; a.Ao[1][co] = An.u1; a.Ao[2][co] = An.u2; a.Ao[3][co] = An.u3;
+                    a.Ao[4][co] = An.t;
+                }
+            }
+            mz_prev = mz;
+        }
+        H_prev = H;
+        fz_prev = fz;
+        for (int m = 0; m < 3; ++m) f_prev[m] = f[m];
+        for (int v = 0; v < 5; ++v) S_prev[v] = S_xy[v];
+        A_cur = A_next;
+    }
+    block_min_atomic(dtbits, a.dt_next);
+}
+
+}  // namespace arkd
+
+// ---------------------------------------------------------------------------
+namespace {
+
+using namespace arkd;
+
+struct FusedLaunch {
+    ark_ctx* c;
+    int LZ;
+};
+
+hipError_t run_fused(void* p) {
+    FusedLaunch* fl = (FusedLaunch*)p;
+    ark_ctx* c = fl->c;
+    FusedArgs a;
+    a.g = c->geo;
+    a.K = c->K;
+    a.cfl_dx[0] = c->prm.CFL * c->prm.dx1;
+    a.cfl_dx[1] = c->prm.CFL * c->prm.dx2;
+    a.cfl_dx[2] = c->prm.CFL * c->prm.dx3;
+    const int cells[5] = {F_ronCon, F_u1nCon, F_u2nCon, F_u3nCon, F_tnCon};
+    for (int v = 0; v < 5; ++v) {
+        a.A[v] = c->cur[cells[v]];
+        a.Ao[v] = c->alt[cells[v]];
+    }
+    for (int d = 0; d < 3; ++d) {
+        const int slots[5] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_p(d), face_t(d)};
+        for (int v = 0; v < 5; ++v) {
+            a.F[d][v] = c->cur[slots[v]];
+            a.Fo[d][v] = c->alt[slots[v]];
+        }
+    }
+    a.S = c->S;
+    a.dt_next = &c->S->dt_bits;
+    a.LZ = fl->LZ;
+    dim3 grid((c->geo.N1 + TX - 1) / TX, (c->geo.N2 + TY - 1) / TY, (c->geo.N3 + fl->LZ - 1) / fl->LZ);
+    hipLaunchKernelGGL(k_fused_step, grid, dim3(NTHR), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+// Planes per block: enough blocks to fill 256 CUs several times over, long
+// enough chunks that the two extra half-level planes per chunk stay cheap.
+int choose_lz(const Geo& g) {
+    long long cols = (long long)((g.N1 + TX - 1) / TX) * ((g.N2 + TY - 1) / TY);
+    int lz = 64;
+    while (lz > 8 && cols * ((g.N3 + lz - 1) / lz) < 2048) lz /= 2;
+    return lz < g.N3 ? lz : g.N3;
+}
+
+}  // namespace
+
+bool ark_fused_eligible(const ark_ctx* c) {
+    return c->geo.N1 >= 2 && c->geo.N2 >= 2 && c->geo.N3 >= 2 && c->rho_consistent && !c->has_solid &&
+           c->prm.precision == ARK_PREC_FP64;
+}
+
+// One fused step per iteration: begin the step (dt), run the kernel, swap
+// the double-buffered persistent fields.  The kernel leaves the next step's
+// dt as the running MIN in S->dt_bits (dt_pending).
+static ark_status fused_steps(ark_ctx* c, int nsteps) {
+    ark_status s = ARK_OK;
+    FusedLaunch fl{c, choose_lz(c->geo)};
+    const int cells[5] = {F_ronCon, F_u1nCon, F_u2nCon, F_u3nCon, F_tnCon};
+    for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
+        s = ark_begin_step(c);
+        if (s == ARK_OK) s = c->launch("fused_step", run_fused, &fl);
+        if (s != ARK_OK) break;
+        for (int v = 0; v < 5; ++v) std::swap(c->cur[cells[v]], c->alt[cells[v]]);
+        for (int d = 0; d < 3; ++d) {
+            const int slots[5] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_p(d), face_t(d)};
+            for (int v = 0; v < 5; ++v) std::swap(c->cur[slots[v]], c->alt[slots[v]]);
+        }
+        c->rho_stored = false;
+        c->dt_pending = true;
+    }
+    return s;
+}
+
+// nsteps steps: nsteps-1 fused steps, then the last step through the
+// phase-by-phase kernels, so that on return every array -- including the
+// reference's scratch arrays Con, f, sigma and the ghost copies -- holds what
+// the reference holds after the same steps.  A state the fused kernel does
+// not cover (face densities not known to equal ro0 + p/c^2, a dimension
+// below 2) takes its first step phase by phase, which makes it covered.
+ark_status ark_step_fused(ark_ctx* c, int nsteps) {
+    ark_status s = ARK_OK;
+    int done = 0;
+    if (nsteps > 1 && !ark_fused_eligible(c)) {
+        s = ark_begin_step(c);
+        if (s == ARK_OK) s = ark_phase_body(c);
+        done = 1;
+    }
+    if (s == ARK_OK && done < nsteps - 1 && ark_fused_eligible(c)) {
+        s = fused_steps(c, nsteps - 1 - done);
+        done = nsteps - 1;
+    }
+    while (s == ARK_OK && done < nsteps) {
+        s = ark_refresh_rho(c);
+        if (s == ARK_OK) s = ark_begin_step(c);
+        if (s == ARK_OK) s = ark_phase_body(c);
+        ++done;
+    }
+    return s;
+}
+
+// ark_advance: fused steps only.
+extern "C" ark_status ark_advance(ark_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return ARK_ERR_INVALID;
+    if (c->has_solid) return c->fail(ARK_ERR_UNSUPPORTED, "solid cells are not implemented on the GPU path");
+    if (!ark_fused_eligible(c))
+        return c->fail(ARK_ERR_STATE,
+                       "ark_advance needs face densities equal to ro0 + p/c^2 (after ark_init_tgv or any step) and "
+                       "N1, N2, N3 >= 2 in FP64 mode; call ark_step first");
+    return fused_steps(c, nsteps);
+}

@@ -27,6 +27,9 @@ struct ark_ctx {
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;
+    bool rho_consistent = false;       // face densities 1..N+1 equal ro0 + p/c^2 bit for bit
+    bool rho_stored = true;            // the stored face-density arrays are current
+    bool dt_pending = false;           // S->dt_bits holds the next step's dt from a fused step
     bool sigma_dirty = false;          // an upload left non-zero values in a stress array
     double** sigma_ptrs = nullptr;     // device array: allocation bases of the nine stress arrays
     bool profiling = false;
@@ -42,5 +45,15 @@ struct ark_ctx {
     arkd::Fields fields() const;
 };
 
+// Step building blocks shared by the drivers (ark_api.hip; internal, not in ark.h).
+extern "C" {
+ark_status ark_cfl_step(ark_ctx* c, int advance);       // TimeStepSize on Con, then dt := MIN
+ark_status ark_cfl_finalize(ark_ctx* c, int mode);      // see k_cfl_finalize
+ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pending MIN or Con), TIME, nStep
+ark_status ark_refresh_rho(ark_ctx* c);                 // store face densities if they are derived
+ark_status ark_phase_body(ark_ctx* c);                  // Phase1 .. UseForces of one step, dt already set
+}
+
 // The fused step (ark_fused.hip).
 ark_status ark_step_fused(ark_ctx* c, int nsteps);
+bool ark_fused_eligible(const ark_ctx* c);

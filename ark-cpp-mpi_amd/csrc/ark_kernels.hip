@@ -19,41 +19,36 @@ namespace arkd {
 __global__ void __launch_bounds__(256) k_cfl(Geo g, const double* __restrict__ u1, const double* __restrict__ u2,
                                               const double* __restrict__ u3, double c1, double c2, double c3,
                                               double sound, Scalars* S) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    int k = blockIdx.z + 1;
+    const int ncell = g.N1 * g.N2 * g.N3;
     unsigned long long bits = ~0ull;
     int bad = 0;
-    if (i <= g.N1 && j <= g.N2 && k <= g.N3) {
+    for (int lin = blockIdx.x * blockDim.x + threadIdx.x; lin < ncell; lin += gridDim.x * blockDim.x) {
+        int i = lin % g.N1 + 1, r = lin / g.N1;
+        int j = r % g.N2 + 1, k = r / g.N2 + 1;
         int c = at(g, i, j, k);
         double a1 = u1[c], a2 = u2[c], a3 = u3[c];
         double dtu1 = c1 / (sound + fabs(a1));
         double dtu2 = c2 / (sound + fabs(a2));
         double dtu3 = c3 / (sound + fabs(a3));
         double dtu = min3d(dtu1, dtu2, dtu3);
-        bits = (unsigned long long)__double_as_longlong(dtu);
-        bad = !(isfinite(a1) && isfinite(a2) && isfinite(a3));
+        bits = umin64(bits, (unsigned long long)__double_as_longlong(dtu));
+        bad += !(isfinite(a1) && isfinite(a2) && isfinite(a3));
     }
-    // wavefront MIN, then one atomic per wavefront
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(bits, off, 64);
-        bits = o < bits ? o : bits;
-    }
-    unsigned long long badmask = __ballot(bad);
-    if ((threadIdx.x & 63) == 0) {
-        if (bits != ~0ull) atomicMin(&S->dt_bits, bits);
-        if (badmask) atomicAdd((unsigned long long*)&S->nonfinite, (unsigned long long)__popcll(badmask));
-    }
+    block_min_atomic(bits, &S->dt_bits);
+    for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off, 64);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd((unsigned long long*)&S->nonfinite, (unsigned long long)bad);
 }
 
-// dt := MIN, reset the running MIN; optionally the driver's bookkeeping
-// TIME += dt/2 twice and nStep++ (source/main.cpp:83-84, 91).
-__global__ void k_cfl_finalize(Scalars* S, int advance) {
+// mode 0: dt := MIN and reset the running MIN; mode 1: the same plus the
+// driver's bookkeeping TIME += dt/2 twice and nStep++ (source/main.cpp:83-84,
+// 91); mode 2: only reset the running MIN (discard it).
+__global__ void k_cfl_finalize(Scalars* S, int mode) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double dt = __longlong_as_double((long long)S->dt_bits);
-        S->dt = dt;
         S->dt_bits = S->dt_init;
-        if (advance) {
+        if (mode == 2) return;
+        S->dt = dt;
+        if (mode == 1) {
             S->nstep += 1;
             double t = S->time + 0.5 * dt;
             S->time = t + 0.5 * dt;
@@ -345,6 +340,23 @@ template __global__ void k_phase2<1>(Geo, Fields, NextFaces, Consts, const Scala
 template __global__ void k_phase2<2>(Geo, Fields, NextFaces, Consts, const Scalars*);
 
 // ---------------------------------------------------------------------------
+// Face densities from face pressures, rho = ro0 + p/c^2 (source/main.cpp:1930),
+// on the faces a Phase2 sweep writes (1..N_d+1 along d, interior across).
+// The fused step does not store face densities (it derives them); this
+// restores them before the phase-by-phase kernels read them.
+__global__ void __launch_bounds__(256) k_rho_faces(Geo g, Fields F, Consts K) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 + 1 || j > g.N2 + 1 || k > g.N3 + 1) return;
+    const int c = at(g, i, j, k);
+    const double s2 = K.sound * K.sound;
+    if (j <= g.N2 && k <= g.N3) F.a[F_ro1][c] = K.ro0 + F.a[F_p1][c] / s2;
+    if (i <= g.N1 && k <= g.N3) F.a[F_ro2][c] = K.ro0 + F.a[F_p2][c] / s2;
+    if (i <= g.N1 && j <= g.N2) F.a[F_ro3][c] = K.ro0 + F.a[F_p3][c] / s2;
+}
+
+// ---------------------------------------------------------------------------
 // Diagnostics (WriteEnergy, source/main.cpp:2851-2876, plus mass and a
 // non-finite scan): fixed block->cell assignment and a fixed-order final
 // reduction, so repeated calls return identical sums.
@@ -426,6 +438,10 @@ __global__ void __launch_bounds__(256) k_init_tgv(Geo g, Fields F, Fields F2, Tg
         double ro = K.ro0 + p / s2;
         a[F_u1nCon][c] = u; a[F_u2nCon][c] = v; a[F_u3nCon][c] = 0.0;
         a[F_ronCon][c] = ro; a[F_tnCon][c] = K.t0;
+        if (F2.a[F_ronCon]) {   // second buffer of the level-n fields: identical, edges included
+            F2.a[F_u1nCon][c] = u; F2.a[F_u2nCon][c] = v; F2.a[F_u3nCon][c] = 0.0;
+            F2.a[F_ronCon][c] = ro; F2.a[F_tnCon][c] = K.t0;
+        }
         a[F_u1Con][c] = u; a[F_u2Con][c] = v; a[F_u3Con][c] = 0.0;
         a[F_roCon][c] = ro; a[F_tCon][c] = K.t0;
         a[F_condition][c] = 0.0;
@@ -465,7 +481,7 @@ static const dim3 kBlock(64, 4, 1);
 
 hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, double c2, double c3, double sound,
                       Scalars* S) {
-    hipLaunchKernelGGL(k_cfl, cell_grid(g.N1, g.N2, g.N3), kBlock, 0, st, g, F.a[F_u1Con], F.a[F_u2Con],
+    hipLaunchKernelGGL(k_cfl, dim3(1024), dim3(256), 0, st, g, F.a[F_u1Con], F.a[F_u2Con],
                        F.a[F_u3Con], c1, c2, c3, sound, S);
     return hipGetLastError();
 }
@@ -517,6 +533,11 @@ hipError_t launch_phase2(hipStream_t st, int axis, const Geo& g, const Fields& F
         hipLaunchKernelGGL(k_phase2<1>, cell_grid(g.N1, g.N2 + 1, g.N3), kBlock, 0, st, g, F, nx, K, S);
     else
         hipLaunchKernelGGL(k_phase2<2>, cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_rho_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K) {
+    hipLaunchKernelGGL(k_rho_faces, cell_grid(g.N1 + 1, g.N2 + 1, g.N3 + 1), kBlock, 0, st, g, F, K);
     return hipGetLastError();
 }
 

@@ -40,6 +40,7 @@ hipError_t launch_forces(hipStream_t st, const Geo& g, const Fields& F, const Co
 hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_phase2(hipStream_t st, int axis, const Geo& g, const Fields& F, const NextFaces& nx,
                          const Consts& K, const Scalars* S);
+hipError_t launch_rho_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_diag(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, double* part, int nblocks,
                        double* out);
 hipError_t launch_init_tgv(hipStream_t st, const Geo& g, const Fields& F, const Fields& F2, const TgvTables& T,

@@ -123,6 +123,17 @@ ark_status ark_phase2(ark_ctx* ctx);
  * device between steps. */
 ark_status ark_step(ark_ctx* ctx, int nsteps);
 
+/* nsteps through the fused kernels only, maintaining the persistent state:
+ * the level-n cell fields (ronCon, u1nCon, u2nCon, u3nCon, tnCon) on the
+ * interior cells, the face fields u*, p*, t* on the faces 1..N_d+1 along
+ * their axis, dt, TIME and nStep -- bitwise as the reference.  The
+ * reference's scratch arrays (Con, f, sigma), the ghost copies and the face
+ * densities ro1..ro3 are NOT brought up to date (the next ark_step or phase
+ * call works from the persistent state and refreshes them).  No counterpart in
+ * the reference: it is ark_step without the final phase-by-phase step, for
+ * callers that only read the persistent fields. */
+ark_status ark_advance(ark_ctx* ctx, int nsteps);
+
 /* The same nsteps through the phase-by-phase kernels (one launch per
  * reference phase): the unfused baseline that ark_step is checked against. */
 ark_status ark_step_phases(ark_ctx* ctx, int nsteps);
