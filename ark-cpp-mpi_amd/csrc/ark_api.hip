@@ -37,6 +37,28 @@ const char* kFieldNames[F_NUM] = {
 
 }  // namespace
 
+const int kGroupFields[4][5] = {
+    {F_ronCon, F_u1nCon, F_u2nCon, F_u3nCon, F_tnCon},
+    {F_u11, F_u21, F_u31, F_p1, F_t1},
+    {F_u12, F_u22, F_u32, F_p2, F_t2},
+    {F_u13, F_u23, F_u33, F_p3, F_t3},
+};
+
+void ark_swap_group(ark_ctx* c, int g) {
+    for (int v = 0; v < 5; ++v) std::swap(c->cur[kGroupFields[g][v]], c->alt[kGroupFields[g][v]]);
+    c->gpar[g] ^= 1;
+}
+
+static int group_of(int f, int* slot) {
+    for (int g = 0; g < 4; ++g)
+        for (int v = 0; v < 5; ++v)
+            if (kGroupFields[g][v] == f) {
+                *slot = v;
+                return g;
+            }
+    return -1;
+}
+
 // ---------------------------------------------------------------------------
 ark_status ark_ctx::fail(ark_status s, const std::string& msg) {
     err = msg;
@@ -117,7 +139,25 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
 
     ark_status s = c->hip(hipSetDevice(p->device), "hipSetDevice");
     if (s == ARK_OK) s = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    c->field_bytes = (c->elems * sizeof(double) + 4095) / 4096 * 4096;
+    for (int g = 0; g < 4 && s == ARK_OK; ++g)
+        for (int b = 0; b < 2 && s == ARK_OK; ++b) {
+            char* mem = nullptr;
+            size_t nb = 5 * c->field_bytes;
+            s = c->hip(hipMalloc(&mem, nb), "hipMalloc field group");
+            if (s != ARK_OK) break;
+            c->allocs.push_back(mem);
+            c->bytes += nb;
+            s = c->hip(hipMemsetAsync(mem, 0, nb, c->stream), "hipMemset field group");
+            c->gbase[g][b] = mem + kOriginPad * sizeof(double);
+        }
     for (int f = 0; f < F_NUM && s == ARK_OK; ++f) {
+        int slot = 0, g = group_of(f, &slot);
+        if (g >= 0) {
+            c->cur[f] = (double*)(c->gbase[g][0] + slot * c->field_bytes);
+            c->alt[f] = (double*)(c->gbase[g][1] + slot * c->field_bytes);
+            continue;
+        }
         int nbuf = is_double_buffered(f) ? 2 : 1;
         for (int b = 0; b < nbuf && s == ARK_OK; ++b) {
             double* mem = nullptr;
@@ -141,6 +181,14 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
         if (s == ARK_OK) {
             c->allocs.push_back(c->sigma_ptrs);
             s = c->hip(hipMemcpy(c->sigma_ptrs, bases, sizeof bases, hipMemcpyHostToDevice), "sigma table");
+        }
+    }
+    if (s == ARK_OK && p->precision == ARK_PREC_MIXED) {
+        s = c->hip(hipMalloc(&c->staging, c->elems * sizeof(double)), "hipMalloc staging");
+        if (s == ARK_OK) {
+            c->allocs.push_back(c->staging);
+            c->bytes += c->elems * sizeof(double);
+            c->staging += kOriginPad;
         }
     }
     if (s == ARK_OK) {
@@ -231,6 +279,14 @@ ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2,
         for (long long q = 0; q < n; ++q)
             if (host[q] != 0.0) { c->has_solid = true; break; }
     }
+    if (c->prm.precision == ARK_PREC_MIXED && is_face_state(field)) {
+        // FP32 storage: face densities are derived from p and not stored
+        if (field == F_ro1 || field == F_ro2 || field == F_ro3) return ARK_OK;
+        s = copy_planes(c, c->staging, host, nullptr, e1, e2, e3);
+        if (s == ARK_OK) s = ark_mixed_face_from_host_layout(c, field, c->staging);
+        if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+        return s;
+    }
     if (field >= F_sigm11) c->sigma_dirty = true;
     if (field == F_ro1 || field == F_ro2 || field == F_ro3 || field == F_p1 || field == F_p2 || field == F_p3) {
         c->rho_consistent = false;
@@ -245,6 +301,12 @@ ark_status ark_download(ark_ctx* c, int field, double* host, int e1, int e2, int
     if (!c || !host) return ARK_ERR_INVALID;
     ark_status s = check_extents(c, field, e1, e2, e3);
     if (s != ARK_OK) return s;
+    if (c->prm.precision == ARK_PREC_MIXED && is_face_state(field)) {
+        s = ark_mixed_face_to_host_layout(c, field, c->staging);
+        if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync before download");
+        if (s != ARK_OK) return s;
+        return copy_planes(c, c->staging, nullptr, host, e1, e2, e3);
+    }
     if (field == F_ro1 || field == F_ro2 || field == F_ro3) s = ark_refresh_rho(c);
     if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync before download");
     if (s != ARK_OK) return s;
@@ -354,6 +416,10 @@ hipError_t run_phase2_axis(void* p) {
 }  // namespace
 
 static ark_status require_fluid(ark_ctx* c) {
+    if (c->prm.precision == ARK_PREC_MIXED)
+        return c->fail(ARK_ERR_UNSUPPORTED,
+                       "the phase-by-phase entry points are FP64; a mixed-precision context advances with ark_step "
+                       "or ark_advance");
     if (c->has_solid)
         return c->fail(ARK_ERR_UNSUPPORTED,
                        "solid cells (condition != 0) are not implemented on the GPU path; the periodic box must be "
@@ -411,7 +477,7 @@ ark_status ark_phase_body(ark_ctx* c) {
 
 ark_status ark_time_step_size(ark_ctx* c, double* dt_out) {
     if (!c) return ARK_ERR_INVALID;
-    ark_status s = require_fluid(c);
+    ark_status s = c->has_solid ? require_fluid(c) : ARK_OK;
     if (s == ARK_OK) {
         if (c->dt_pending) {
             // after fused steps Con is not stored; their MIN is TimeStepSize's result
@@ -465,7 +531,10 @@ ark_status ark_phase2(ark_ctx* c) {
         s = c->launch(names[d], run_phase2_axis, &a);
     }
     if (s == ARK_OK) {
-        for (int f = F_ro1; f <= F_p3; ++f) std::swap(c->cur[f], c->alt[f]);
+        for (int g = 1; g < 4; ++g) ark_swap_group(c, g);
+        std::swap(c->cur[F_ro1], c->alt[F_ro1]);
+        std::swap(c->cur[F_ro2], c->alt[F_ro2]);
+        std::swap(c->cur[F_ro3], c->alt[F_ro3]);
         c->rho_consistent = !c->has_solid;
         c->rho_stored = true;
     }
@@ -484,6 +553,10 @@ ark_status ark_step_phases(ark_ctx* c, int nsteps) {
 
 ark_status ark_step(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
+    if (c->prm.precision == ARK_PREC_MIXED) {
+        if (c->has_solid) return c->fail(ARK_ERR_UNSUPPORTED, "solid cells are not implemented on the GPU path");
+        return ark_mixed_steps(c, nsteps);
+    }
     ark_status s = require_fluid(c);
     if (s != ARK_OK) return s;
     return ark_step_fused(c, nsteps);
@@ -551,6 +624,19 @@ ark_status ark_init_tgv(ark_ctx* c, double U0) {
     if (s == ARK_OK) s = c->hip(launch_init_tgv(c->stream, c->geo, c->fields(), F2, T, U0, amp, c->K), "init_tgv");
     if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     hipFree(dtab);
+    if (s == ARK_OK && c->prm.precision == ARK_PREC_MIXED) {
+        // FP32 face storage: convert each face field (both buffers hold it)
+        for (int g = 1; g < 4 && s == ARK_OK; ++g)
+            for (int v = 0; v < 5 && s == ARK_OK; ++v) {
+                const int f = kGroupFields[g][v];
+                // FP64 values in cur -> staging -> FP32 into cur and alt (never in place)
+                s = c->hip(hipMemcpyAsync(c->staging - kOriginPad, c->cur[f] - kOriginPad,
+                                          c->elems * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                           "copy to staging");
+                if (s == ARK_OK) s = ark_mixed_face_from_host_layout(c, f, c->staging);
+            }
+        if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    }
     c->has_solid = false;
     c->rho_consistent = true;
     c->rho_stored = true;

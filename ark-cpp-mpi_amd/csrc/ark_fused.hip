@@ -34,18 +34,31 @@ constexpr int TX = 32, TY = 8;
 constexpr int RX = TX + 2, RY = TY + 2;
 constexpr int NTHR = RX * RY;   // 340 threads: 6 wavefronts
 
+// Field groups: five fields `fb` bytes apart, element (0,0,0) of the first
+// at the group pointer.  A field is read as group + v*fb + 8*idx with the
+// group pointer in scalar registers and a 32-bit per-thread offset.
 struct FusedArgs {
     Geo g;
     Consts K;
     double cfl_dx[3];          // CFL*dx_d, the numerators of the next step's CFL bound
-    const double* A[5];        // level-n cells: rho, u1, u2, u3, t
-    double* Ao[5];             // level n+1 cells
-    const double* F[3][5];     // old faces normal to d: u1, u2, u3, p, t
-    double* Fo[3][5];          // new faces
+    const char* A;             // level-n cells: rho, u1, u2, u3, t
+    char* Ao;                  // level n+1 cells
+    const char* F[3];          // old faces normal to d: u1, u2, u3, p, t
+    char* Fo[3];               // new faces
+    unsigned long long fb;     // bytes between the fields of a group
     const Scalars* S;          // S->dt is this step's dt
     unsigned long long* dt_next;
     int LZ;
 };
+
+__device__ __forceinline__ double gld(const char* grp, unsigned long long fb, int v, int idx) {
+    const char* base = grp + v * fb;
+    return *(const double*)(base + (unsigned)idx * 8u);
+}
+__device__ __forceinline__ void gst(char* grp, unsigned long long fb, int v, int idx, double x) {
+    char* base = grp + v * fb;
+    *(double*)(base + (unsigned)idx * 8u) = x;
+}
 
 __device__ __forceinline__ int wrapc(int q, int N) { return q < 1 ? q + N : (q > N ? q - N : q); }
 __device__ __forceinline__ int mapf(int f, int N) { return f == 0 ? N : (f == N + 2 ? 2 : f); }
@@ -56,13 +69,15 @@ struct Flux5 { double m[5]; };
 
 __device__ __forceinline__ Cell5 load_cell(const FusedArgs& a, int idx) {
     Cell5 c;
-    c.ro = a.A[0][idx]; c.u1 = a.A[1][idx]; c.u2 = a.A[2][idx]; c.u3 = a.A[3][idx]; c.t = a.A[4][idx];
+    c.ro = gld(a.A, a.fb, 0, idx); c.u1 = gld(a.A, a.fb, 1, idx); c.u2 = gld(a.A, a.fb, 2, idx);
+    c.u3 = gld(a.A, a.fb, 3, idx); c.t = gld(a.A, a.fb, 4, idx);
     return c;
 }
 
 __device__ __forceinline__ Face5 load_face(const FusedArgs& a, int d, int idx) {
     Face5 f;
-    f.u1 = a.F[d][0][idx]; f.u2 = a.F[d][1][idx]; f.u3 = a.F[d][2][idx]; f.p = a.F[d][3][idx]; f.t = a.F[d][4][idx];
+    f.u1 = gld(a.F[d], a.fb, 0, idx); f.u2 = gld(a.F[d], a.fb, 1, idx); f.u3 = gld(a.F[d], a.fb, 2, idx);
+    f.p = gld(a.F[d], a.fb, 3, idx); f.t = gld(a.F[d], a.fb, 4, idx);
     return f;
 }
 
@@ -137,11 +152,11 @@ __device__ __forceinline__ FaceOut new_face(const Fwd& fw, const Cell5& hb, cons
 template <int D>
 __device__ __forceinline__ void store_face(const FusedArgs& a, int idx, const FaceOut& o) {
     constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
-    a.Fo[D][D][idx] = o.un;
-    a.Fo[D][TA][idx] = o.a;
-    a.Fo[D][TB][idx] = o.b;
-    a.Fo[D][3][idx] = o.p;
-    a.Fo[D][4][idx] = o.t;
+    gst(a.Fo[D], a.fb, D, idx, o.un);
+    gst(a.Fo[D], a.fb, TA, idx, o.a);
+    gst(a.Fo[D], a.fb, TB, idx, o.b);
+    gst(a.Fo[D], a.fb, 3, idx, o.p);
+    gst(a.Fo[D], a.fb, 4, idx, o.t);
 }
 
 template <int D>
@@ -189,12 +204,13 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
     const int jwm = wrapc(j - 1, g.N2), jwp = wrapc(j + 1, g.N2);
     const int px = g.px, sxy = g.sxy;
     const double VIS = K.VIS;
+    FusedArgs A = a;
 
     // rolling state along z
-    Cell5 A_cur = load_cell(a, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
+    Cell5 A_cur = load_cell(A, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
     double sz_lo[3];
     {
-        Cell5 A_below = load_cell(a, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
+        const Cell5 A_below = load_cell(A, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
         sz_lo[0] = VIS * (A_cur.u1 - A_below.u1) / K.dx[2];
         sz_lo[1] = VIS * (A_cur.u2 - A_below.u2) / K.dx[2];
         sz_lo[2] = VIS * (A_cur.u3 - A_below.u3) / K.dx[2];
@@ -207,86 +223,131 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
     unsigned long long dtbits = ~0ull;
 
     for (int p = k0 - 1; p <= kend; ++p) {
+        // keep the per-field addresses (group + v*stride) from being hoisted
+        // out of the loop into scalar registers: they are cheap to recompute
+        asm volatile("" : "+s"(A.fb));
         const int pw = wrapc(p, g.N3);
         const int c = at(g, iw, jw, pw);
-        const Cell5 A_next = load_cell(a, at(g, iw, jw, wrapc(p + 1, g.N3)));
+        const Cell5 A_next = load_cell(A, at(g, iw, jw, wrapc(p + 1, g.N3)));
 
         // ---- stress force f (StressTensor, source/main.cpp:856-1023) -------
         double f[3];
         {
-            const Cell5 Aw = load_cell(a, at(g, iwm, jw, pw)), Ae = load_cell(a, at(g, iwp, jw, pw));
-            const Cell5 As = load_cell(a, at(g, iw, jwm, pw)), An = load_cell(a, at(g, iw, jwp, pw));
-            double sz_hi[3];
+            const int cw = at(g, iwm, jw, pw), ce = at(g, iwp, jw, pw);
+            const int cs = at(g, iw, jwm, pw), cn = at(g, iw, jwp, pw);
             for (int m = 0; m < 3; ++m) {
                 const double uc = ucomp(A_cur, m);
-                double sx_lo = VIS * (uc - ucomp(Aw, m)) / K.dx[0];
-                double sx_hi = VIS * (ucomp(Ae, m) - uc) / K.dx[0];
-                double sy_lo = VIS * (uc - ucomp(As, m)) / K.dx[1];
-                double sy_hi = VIS * (ucomp(An, m) - uc) / K.dx[1];
-                sz_hi[m] = VIS * (ucomp(A_next, m) - uc) / K.dx[2];
-                f[m] = ((sx_hi - sx_lo) * K.ds1 + (sy_hi - sy_lo) * K.ds2 + (sz_hi[m] - sz_lo[m]) * K.ds3) * A_cur.ro;
+                const double uw = gld(A.A, A.fb, 1 + m, cw), ue = gld(A.A, A.fb, 1 + m, ce);
+                const double us = gld(A.A, A.fb, 1 + m, cs), un = gld(A.A, A.fb, 1 + m, cn);
+                double sx_lo = VIS * (uc - uw) / K.dx[0];
+                double sx_hi = VIS * (ue - uc) / K.dx[0];
+                double sy_lo = VIS * (uc - us) / K.dx[1];
+                double sy_hi = VIS * (un - uc) / K.dx[1];
+                double sz_hi = VIS * (ucomp(A_next, m) - uc) / K.dx[2];
+                f[m] = ((sx_hi - sx_lo) * K.ds1 + (sy_hi - sy_lo) * K.ds2 + (sz_hi - sz_lo[m]) * K.ds3) * A_cur.ro;
+                sz_lo[m] = sz_hi;
             }
-            for (int m = 0; m < 3; ++m) sz_lo[m] = sz_hi[m];
         }
 
-        // ---- predictor: Phase1 + UseForces with the old faces --------------
-        const Face5 Xw = load_face(a, 0, c), Xe = load_face(a, 0, c + 1);
-        const Face5 Ys = load_face(a, 1, c), Yn = load_face(a, 1, c + px);
-        const Face5 Zb = load_face(a, 2, c), Zt = load_face(a, 2, c + sxy);
+        // ---- predictor (Phase1 + UseForces, old faces) and the Phase2 -------
+        // ---- extrapolations of this cell along x, y, z -----------------------
         Cell5 H;
+        Fwd fx, fy, fz;
+        Bwd bx, by, bz;
+        double ox[3], oy[3], oz[3];   // old t, tangential a, tangential b of the low faces
         {
-            const Flux5 fxw = old_flux<0>(Xw, K), fxe = old_flux<0>(Xe, K);
-            const Flux5 fys = old_flux<1>(Ys, K), fyn = old_flux<1>(Yn, K);
-            const Flux5 fzb = old_flux<2>(Zb, K), fzt = old_flux<2>(Zt, K);
-            double S[5];
-            for (int v = 0; v < 5; ++v)
-                S[v] = (fxe.m[v] - fxw.m[v]) * K.ds1 + (fyn.m[v] - fys.m[v]) * K.ds2 + (fzt.m[v] - fzb.m[v]) * K.ds3;
-            H = conservative_update(A_cur, S, f, dt, K);
+            const Face5 Xw = load_face(A, 0, c), Xe = load_face(A, 0, c + 1);
+            const Face5 Ys = load_face(A, 1, c), Yn = load_face(A, 1, c + px);
+            const Face5 Zb = load_face(A, 2, c), Zt = load_face(A, 2, c + sxy);
+            {
+                const Flux5 fxw = old_flux<0>(Xw, K), fxe = old_flux<0>(Xe, K);
+                const Flux5 fys = old_flux<1>(Ys, K), fyn = old_flux<1>(Yn, K);
+                const Flux5 fzb = old_flux<2>(Zb, K), fzt = old_flux<2>(Zt, K);
+                double S[5];
+                for (int v = 0; v < 5; ++v)
+                    S[v] = (fxe.m[v] - fxw.m[v]) * K.ds1 + (fyn.m[v] - fys.m[v]) * K.ds2 +
+                           (fzt.m[v] - fzb.m[v]) * K.ds3;
+                H = conservative_update(A_cur, S, f, dt, K);
+            }
+            // Phase2 uses the faces of the logical cell: the same as the
+            // predictor's except for a ring cell just outside the box, whose
+            // u/p ghost faces map into the box and whose t ghost faces are read
+            // as stored
+            Face5 Pxb = Xw, Pxf = Xe, Pyb = Ys, Pyf = Yn, Pzb = Zb, Pzf = Zt;
+            if (i != iw) {
+                const int cl = at(g, 0, jw, pw);
+                Pxb = load_face(A, 0, cl + mapf(i, g.N1));
+                Pxf = load_face(A, 0, cl + mapf(i + 1, g.N1));
+                Pxb.t = gld(A.F[0], A.fb, 4, cl + i);
+                Pxf.t = gld(A.F[0], A.fb, 4, cl + i + 1);
+            }
+            if (j != jw) {
+                const int cl = at(g, iw, 0, pw);
+                Pyb = load_face(A, 1, cl + mapf(j, g.N2) * px);
+                Pyf = load_face(A, 1, cl + mapf(j + 1, g.N2) * px);
+                Pyb.t = gld(A.F[1], A.fb, 4, cl + j * px);
+                Pyf.t = gld(A.F[1], A.fb, 4, cl + (j + 1) * px);
+            }
+            if (p != pw) {
+                const int cl = at(g, iw, jw, 0);
+                Pzb = load_face(A, 2, cl + mapf(p, g.N3) * sxy);
+                Pzf = load_face(A, 2, cl + mapf(p + 1, g.N3) * sxy);
+                Pzb.t = gld(A.F[2], A.fb, 4, cl + p * sxy);
+                Pzf.t = gld(A.F[2], A.fb, 4, cl + (p + 1) * sxy);
+            }
+            {
+                const CellSide s = make_side<0>(A_cur, H, Pxb, Pxf);
+                fx = cell_forward(s, dt, K.dx[0], K);
+                bx = cell_backward<true>(s, dt, K.dx[0], K);
+            }
+            {
+                const CellSide s = make_side<1>(A_cur, H, Pyb, Pyf);
+                fy = cell_forward(s, dt, K.dx[1], K);
+                by = cell_backward<false>(s, dt, K.dx[1], K);
+            }
+            {
+                const CellSide s = make_side<2>(A_cur, H, Pzb, Pzf);
+                fz = cell_forward(s, dt, K.dx[2], K);
+                bz = cell_backward<true>(s, dt, K.dx[2], K);
+            }
+            ox[0] = Pxb.t; ox[1] = Pxb.u2; ox[2] = Pxb.u3;
+            oy[0] = Pyb.t; oy[1] = Pyb.u1; oy[2] = Pyb.u3;
+            oz[0] = Pzb.t; oz[1] = Pzb.u1; oz[2] = Pzb.u2;
         }
         if (owned && p >= k0 && p < kend) {
             // the next step's CFL bound from the post-force half level (TimeStepSize, source/main.cpp:379-405)
-            double d1 = a.cfl_dx[0] / (K.sound + fabs(H.u1));
-            double d2 = a.cfl_dx[1] / (K.sound + fabs(H.u2));
-            double d3 = a.cfl_dx[2] / (K.sound + fabs(H.u3));
+            double d1 = A.cfl_dx[0] / (K.sound + fabs(H.u1));
+            double d2 = A.cfl_dx[1] / (K.sound + fabs(H.u2));
+            double d3 = A.cfl_dx[2] / (K.sound + fabs(H.u3));
             unsigned long long b = (unsigned long long)__double_as_longlong(min3d(d1, d2, d3));
             dtbits = b < dtbits ? b : dtbits;
         }
 
-        // ---- Phase2 extrapolations of this cell along z ---------------------
-        // faces of the logical cell p: ghost faces map for u/p, t is read as stored
-        Face5 Pzb = Zb, Pzf = Zt;
-        if (p != pw) {
-            const int cl = at(g, iw, jw, 0);
-            Pzb = load_face(a, 2, cl + mapf(p, g.N3) * sxy);
-            Pzf = load_face(a, 2, cl + mapf(p + 1, g.N3) * sxy);
-            Pzb.t = a.F[2][4][cl + p * sxy];
-            Pzf.t = a.F[2][4][cl + (p + 1) * sxy];
+        // ---- z-face p and the corrector of plane p-1 ------------------------
+        Flux5 mz{};
+        if (p >= k0) {
+            const FaceOut o = face_combine(fz_prev, bz, H_prev.u3, H.u3, H_prev.t, H.t, H_prev.u1, H.u1, H_prev.u2,
+                                           H.u2, oz[0], oz[1], oz[2], K);
+            mz = new_flux<2>(o);
+            if (owned) {
+                if (p < kend || kend == g.N3 + 1) store_face<2>(A, at(g, i, j, p), o);
+                if (p > k0) {
+                    double S[5];
+                    for (int v = 0; v < 5; ++v) S[v] = S_prev[v] + (mz.m[v] - mz_prev.m[v]) * K.ds3;
+                    const Cell5 An = conservative_update(H_prev, S, f_prev, dt, K);
+                    const int co = at(g, i, j, p - 1);
+                    gst(A.Ao, A.fb, 0, co, An.ro);
+                    gst(A.Ao, A.fb, 1, co, An.u1);
+                    gst(A.Ao, A.fb, 2, co, An.u2);
+                    gst(A.Ao, A.fb, 3, co, An.u3);
+                    gst(A.Ao, A.fb, 4, co, An.t);
+                }
+            }
         }
-        const CellSide sz = make_side<2>(A_cur, H, Pzb, Pzf);
-        const Fwd fz = cell_forward(sz, dt, K.dx[2], K);
 
         // ---- x/y faces and the in-plane part of the corrector ---------------
         double S_xy[5] = {0, 0, 0, 0, 0};
         if (p >= k0 && p < kend) {
-            Face5 Pxb = Xw, Pxf = Xe, Pyb = Ys, Pyf = Yn;
-            if (i != iw) {
-                const int cl = at(g, 0, jw, pw);
-                Pxb = load_face(a, 0, cl + mapf(i, g.N1));
-                Pxf = load_face(a, 0, cl + mapf(i + 1, g.N1));
-                Pxb.t = a.F[0][4][cl + i];
-                Pxf.t = a.F[0][4][cl + i + 1];
-            }
-            if (j != jw) {
-                const int cl = at(g, iw, 0, pw);
-                Pyb = load_face(a, 1, cl + mapf(j, g.N2) * px);
-                Pyf = load_face(a, 1, cl + mapf(j + 1, g.N2) * px);
-                Pyb.t = a.F[1][4][cl + j * px];
-                Pyf.t = a.F[1][4][cl + (j + 1) * px];
-            }
-            const CellSide sx = make_side<0>(A_cur, H, Pxb, Pxf);
-            const CellSide sy = make_side<1>(A_cur, H, Pyb, Pyf);
-            const Fwd fx = cell_forward(sx, dt, K.dx[0], K);
-            const Fwd fy = cell_forward(sy, dt, K.dx[1], K);
             LdsCell& me = lds[tid].c;
             me.h[0] = H.ro; me.h[1] = H.u1; me.h[2] = H.u2; me.h[3] = H.u3; me.h[4] = H.t;
             me.fx[0] = fx.r; me.fx[1] = fx.t; me.fx[2] = fx.a; me.fx[3] = fx.b;
@@ -295,20 +356,18 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
             Flux5 mx{}, my{};
             if (xface) {
                 const LdsCell& nb = lds[tid - 1].c;
-                Cell5 hb{nb.h[0], nb.h[1], nb.h[2], nb.h[3], nb.h[4]};
-                Fwd fwb{nb.fx[0], nb.fx[1], nb.fx[2], nb.fx[3]};
-                const Bwd bw = cell_backward<true>(sx, dt, K.dx[0], K);
-                const FaceOut o = new_face<0>(fwb, hb, bw, H, Pxb, K);
-                if (xface_store) store_face<0>(a, at(g, i, j, p), o);
+                const Fwd fwb{nb.fx[0], nb.fx[1], nb.fx[2], nb.fx[3]};
+                const FaceOut o = face_combine(fwb, bx, nb.h[1], H.u1, nb.h[4], H.t, nb.h[2], H.u2, nb.h[3], H.u3,
+                                               ox[0], ox[1], ox[2], K);
+                if (xface_store) store_face<0>(A, at(g, i, j, p), o);
                 mx = new_flux<0>(o);
             }
             if (yface) {
                 const LdsCell& nb = lds[tid - RX].c;
-                Cell5 hb{nb.h[0], nb.h[1], nb.h[2], nb.h[3], nb.h[4]};
-                Fwd fwb{nb.fy[0], nb.fy[1], nb.fy[2], nb.fy[3]};
-                const Bwd bw = cell_backward<false>(sy, dt, K.dx[1], K);
-                const FaceOut o = new_face<1>(fwb, hb, bw, H, Pyb, K);
-                if (yface_store) store_face<1>(a, at(g, i, j, p), o);
+                const Fwd fwb{nb.fy[0], nb.fy[1], nb.fy[2], nb.fy[3]};
+                const FaceOut o = face_combine(fwb, by, nb.h[2], H.u2, nb.h[4], H.t, nb.h[1], H.u1, nb.h[3], H.u3,
+                                               oy[0], oy[1], oy[2], K);
+                if (yface_store) store_face<1>(A, at(g, i, j, p), o);
                 my = new_flux<1>(o);
             }
             __syncthreads();
@@ -323,31 +382,14 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
             __syncthreads();
         }
 
-        // ---- z-face p and the corrector of plane p-1 ------------------------
-        if (p >= k0) {
-            const Bwd bz = cell_backward<true>(sz, dt, K.dx[2], K);
-            const FaceOut o = new_face<2>(fz_prev, H_prev, bz, H, Pzb, K);
-            const Flux5 mz = new_flux<2>(o);
-            if (owned) {
-                if (p < kend || kend == g.N3 + 1) store_face<2>(a, at(g, i, j, p), o);
-                if (p > k0) {
-                    double S[5];
-                    for (int v = 0; v < 5; ++v) S[v] = S_prev[v] + (mz.m[v] - mz_prev.m[v]) * K.ds3;
-                    const Cell5 An = conservative_update(H_prev, S, f_prev, dt, K);
-                    const int co = at(g, i, j, p - 1);
-                    a.Ao[0][co] = An.ro; a.Ao[1][co] = An.u1; a.Ao[2][co] = An.u2; a.Ao[3][co] = An.u3;
-                    a.Ao[4][co] = An.t;
-                }
-            }
-            mz_prev = mz;
-        }
         H_prev = H;
         fz_prev = fz;
         for (int m = 0; m < 3; ++m) f_prev[m] = f[m];
         for (int v = 0; v < 5; ++v) S_prev[v] = S_xy[v];
+        mz_prev = mz;
         A_cur = A_next;
     }
-    block_min_atomic(dtbits, a.dt_next);
+    block_min_atomic(dtbits, A.dt_next);
 }
 
 }  // namespace arkd
@@ -371,18 +413,13 @@ hipError_t run_fused(void* p) {
     a.cfl_dx[0] = c->prm.CFL * c->prm.dx1;
     a.cfl_dx[1] = c->prm.CFL * c->prm.dx2;
     a.cfl_dx[2] = c->prm.CFL * c->prm.dx3;
-    const int cells[5] = {F_ronCon, F_u1nCon, F_u2nCon, F_u3nCon, F_tnCon};
-    for (int v = 0; v < 5; ++v) {
-        a.A[v] = c->cur[cells[v]];
-        a.Ao[v] = c->alt[cells[v]];
-    }
+    a.A = c->gbase[0][c->gpar[0]];
+    a.Ao = c->gbase[0][c->gpar[0] ^ 1];
     for (int d = 0; d < 3; ++d) {
-        const int slots[5] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_p(d), face_t(d)};
-        for (int v = 0; v < 5; ++v) {
-            a.F[d][v] = c->cur[slots[v]];
-            a.Fo[d][v] = c->alt[slots[v]];
-        }
+        a.F[d] = c->gbase[1 + d][c->gpar[1 + d]];
+        a.Fo[d] = c->gbase[1 + d][c->gpar[1 + d] ^ 1];
     }
+    a.fb = c->field_bytes;
     a.S = c->S;
     a.dt_next = &c->S->dt_bits;
     a.LZ = fl->LZ;
@@ -413,16 +450,11 @@ bool ark_fused_eligible(const ark_ctx* c) {
 static ark_status fused_steps(ark_ctx* c, int nsteps) {
     ark_status s = ARK_OK;
     FusedLaunch fl{c, choose_lz(c->geo)};
-    const int cells[5] = {F_ronCon, F_u1nCon, F_u2nCon, F_u3nCon, F_tnCon};
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
         if (s == ARK_OK) s = c->launch("fused_step", run_fused, &fl);
         if (s != ARK_OK) break;
-        for (int v = 0; v < 5; ++v) std::swap(c->cur[cells[v]], c->alt[cells[v]]);
-        for (int d = 0; d < 3; ++d) {
-            const int slots[5] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_p(d), face_t(d)};
-            for (int v = 0; v < 5; ++v) std::swap(c->cur[slots[v]], c->alt[slots[v]]);
-        }
+        for (int g = 0; g < 4; ++g) ark_swap_group(c, g);
         c->rho_stored = false;
         c->dt_pending = true;
     }
@@ -460,6 +492,7 @@ ark_status ark_step_fused(ark_ctx* c, int nsteps) {
 extern "C" ark_status ark_advance(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
     if (c->has_solid) return c->fail(ARK_ERR_UNSUPPORTED, "solid cells are not implemented on the GPU path");
+    if (c->prm.precision == ARK_PREC_MIXED) return ark_mixed_steps(c, nsteps);
     if (!ark_fused_eligible(c))
         return c->fail(ARK_ERR_STATE,
                        "ark_advance needs face densities equal to ro0 + p/c^2 (after ark_init_tgv or any step) and "

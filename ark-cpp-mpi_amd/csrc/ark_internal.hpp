@@ -20,10 +20,18 @@ struct ark_ctx {
     arkd::Consts K{};
     hipStream_t stream = nullptr;
     size_t elems = 0;                  // elements per field allocation
+    size_t field_bytes = 0;            // bytes between consecutive fields of a group
     double* cur[arkd::F_NUM] = {};     // element (0,0,0) of each field
-    double* alt[arkd::F_NUM] = {};     // second buffer of the 18 face-state fields
+    double* alt[arkd::F_NUM] = {};     // second buffer of the double-buffered fields
+    // The persistent fields live in four groups of five consecutive fields
+    // (level-n cells rho,u1,u2,u3,t; per orientation the faces u1,u2,u3,p,t),
+    // each group twice.  gbase[g][b] is element (0,0,0) of the group's first
+    // field in buffer b; gpar[g] is the buffer that is current.
+    char* gbase[4][2] = {};
+    int gpar[4] = {0, 0, 0, 0};
     arkd::Scalars* S = nullptr;        // device scalars
     double* diag_part = nullptr;
+    double* staging = nullptr;         // one FP64 field (mixed precision conversions)
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;
@@ -53,6 +61,17 @@ ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pen
 ark_status ark_refresh_rho(ark_ctx* c);                 // store face densities if they are derived
 ark_status ark_phase_body(ark_ctx* c);                  // Phase1 .. UseForces of one step, dt already set
 }
+
+// Field slots of the four groups, in group order.
+extern const int kGroupFields[4][5];
+// Exchange the current and the other buffer of group g.
+void ark_swap_group(ark_ctx* c, int g);
+
+// Mixed precision (ark_mixed.hip): fused steps, and the conversion of one
+// face field between its FP32 storage and an FP64 field in the common layout.
+ark_status ark_mixed_steps(ark_ctx* c, int nsteps);
+ark_status ark_mixed_face_to_host_layout(ark_ctx* c, int f, double* staging);
+ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* staging);
 
 // The fused step (ark_fused.hip).
 ark_status ark_step_fused(ark_ctx* c, int nsteps);

@@ -1,0 +1,581 @@
+// Mixed-precision fused CABARET/ARK step.
+//
+// Storage: the level-n cell fields stay FP64 (rho, u1, u2, u3, t); the face
+// fields are FP32: u1, u2, u3, p and t' = t - t0.  The face density is
+// rho0 + p/c^2 and is never stored; p itself is the density perturbation
+// c^2 (rho - rho0), so the weakly compressible equation of state
+// p = c^2 (rho - rho0) (source/main.cpp:1774-1775) never subtracts two
+// numbers near rho0 in FP32: cell densities are turned into FP32
+// perturbations rho - rho0 after an FP64 subtraction.
+//
+// Arithmetic: fluxes, stresses, Riemann invariants, the limiter and the new
+// face values are FP32 (divisions by step constants become multiplications by
+// reciprocals).  The closing conservative update is FP64: with c = dt/(2 dvc)
+// and the predictor's and corrector's flux sums S, S' (FP32), the
+// reference's predictor-force-corrector-force sequence (source/main.cpp:
+// 530-557, 1038-1040, applied twice) is exactly
+//     rho'   = rho - c (S_rho + S'_rho)
+//     (rho u)' = rho u - c (S_u + S'_u - 2 f)
+//     (rho t)' = rho t - c (S_t + S'_t)
+// so level n+1 is level n plus two FP32 increments accumulated in FP64.  The
+// half-level state the Phase2 sweeps need is formed in FP32 from the
+// predictor's increments.
+//
+// Blocking and the order of work per plane are those of the FP64 fused
+// kernel (ark_fused.hip): 32x8 owned columns plus a one-cell ring, z-march.
+#include "ark_internal.hpp"
+#include "ark_kernels.hpp"
+
+namespace arkd {
+namespace mixed {
+
+constexpr int TX = 32, TY = 8;
+constexpr int RX = TX + 2, RY = TY + 2;
+constexpr int NTHR = RX * RY;
+
+struct Args {
+    Geo g;
+    // FP32 step constants
+    float VIS, c, c2, inv_c2, inv_rc, half_rc, ro0;
+    float inv_dx[3], ds[3];
+    float cfl_dx[3];
+    // FP64 constants of the closing update
+    double ro0d, t0d, dvc;
+    const char* A;      // FP64 level-n cells: rho, u1, u2, u3, t
+    char* Ao;
+    const char* F[3];   // FP32 faces normal to d: u1, u2, u3, p, t'
+    char* Fo[3];
+    unsigned long long fb;
+    const Scalars* S;
+    unsigned long long* dt_next;
+    int LZ;
+};
+
+__device__ __forceinline__ double ldd(const char* grp, unsigned long long fb, int v, int idx) {
+    return *(const double*)(grp + v * fb + (unsigned)idx * 8u);
+}
+__device__ __forceinline__ void std_(char* grp, unsigned long long fb, int v, int idx, double x) {
+    *(double*)(grp + v * fb + (unsigned)idx * 8u) = x;
+}
+__device__ __forceinline__ float ldf(const char* grp, unsigned long long fb, int v, int idx) {
+    return *(const float*)(grp + v * fb + (unsigned)idx * 4u);
+}
+__device__ __forceinline__ void stf(char* grp, unsigned long long fb, int v, int idx, float x) {
+    *(float*)(grp + v * fb + (unsigned)idx * 4u) = x;
+}
+
+__device__ __forceinline__ int wrapc(int q, int N) { return q < 1 ? q + N : (q > N ? q - N : q); }
+__device__ __forceinline__ int mapf(int f, int N) { return f == 0 ? N : (f == N + 2 ? 2 : f); }
+
+struct CellD { double ro, u[3], t; };                 // FP64 level-n cell
+struct CellF { float dr, u[3], tp; };                 // FP32 view: rho - rho0, u, t - t0
+struct FaceF { float u[3], p, tp; };
+
+__device__ __forceinline__ CellD load_cell(const Args& a, int idx) {
+    CellD c;
+    c.ro = ldd(a.A, a.fb, 0, idx);
+    c.u[0] = ldd(a.A, a.fb, 1, idx);
+    c.u[1] = ldd(a.A, a.fb, 2, idx);
+    c.u[2] = ldd(a.A, a.fb, 3, idx);
+    c.t = ldd(a.A, a.fb, 4, idx);
+    return c;
+}
+__device__ __forceinline__ CellF to_f(const CellD& c, const Args& a) {
+    CellF f;
+    f.dr = (float)(c.ro - a.ro0d);
+    f.u[0] = (float)c.u[0]; f.u[1] = (float)c.u[1]; f.u[2] = (float)c.u[2];
+    f.tp = (float)(c.t - a.t0d);
+    return f;
+}
+__device__ __forceinline__ FaceF load_face(const Args& a, int d, int idx) {
+    FaceF f;
+    f.u[0] = ldf(a.F[d], a.fb, 0, idx);
+    f.u[1] = ldf(a.F[d], a.fb, 1, idx);
+    f.u[2] = ldf(a.F[d], a.fb, 2, idx);
+    f.p = ldf(a.F[d], a.fb, 3, idx);
+    f.tp = ldf(a.F[d], a.fb, 4, idx);
+    return f;
+}
+
+struct Flux { float m[5]; };
+
+// Flux terms of a face normal to D (momentum flux carries p on the normal
+// component); rho = rho0 + p/c^2.
+template <int D>
+__device__ __forceinline__ Flux flux(const float u[3], float p, float tp, const Args& a) {
+    const float ro = a.ro0 + p * a.inv_c2;
+    const float mro = ro * u[D];
+    Flux q;
+    q.m[0] = mro;
+    q.m[1] = mro * u[0] + (D == 0 ? p : 0.f);
+    q.m[2] = mro * u[1] + (D == 1 ? p : 0.f);
+    q.m[3] = mro * u[2] + (D == 2 ? p : 0.f);
+    q.m[4] = mro * tp;
+    return q;
+}
+
+struct Side {   // one cell along the sweep axis: level n (c), half level (cn), faces f and b
+    float un_f, un_b, un_cn, un_c;
+    float ua_f, ua_b, ua_cn, ua_c;
+    float ub_f, ub_b, ub_cn, ub_c;
+    float p_f, p_b, p_cn, p_c;
+    float t_f, t_b, t_cn, t_c;
+};
+struct FwdF { float r, t, a, b; };
+struct BwdF { float q, t, a, b; };
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+__device__ __forceinline__ float max3f(float x, float y, float z) { return fmaxf(fmaxf(x, y), z); }
+__device__ __forceinline__ float min3f(float x, float y, float z) { return fminf(fminf(x, y), z); }
+
+// Limited extrapolations (CABARET min-max limiter, source/main.cpp:1777-1873).
+__device__ __forceinline__ FwdF forward(const Side& s, float dt, float inv_dt2, float inv_dx, const Args& a) {
+    const float r_f = s.un_f + s.p_f * a.inv_rc, r_b = s.un_b + s.p_b * a.inv_rc;
+    const float r_cn = s.un_cn + s.p_cn * a.inv_rc, r_c = s.un_c + s.p_c * a.inv_rc;
+    const float gr = (r_cn - r_c) * inv_dt2 + (s.un_cn + a.c) * (r_f - r_b) * inv_dx;
+    const float gt = (s.t_cn - s.t_c) * inv_dt2 + s.un_cn * (s.t_f - s.t_b) * inv_dx;
+    const float ga = (s.ua_cn - s.ua_c) * inv_dt2 + s.un_cn * (s.ua_f - s.ua_b) * inv_dx;
+    const float gb = (s.ub_cn - s.ub_c) * inv_dt2 + s.un_cn * (s.ub_f - s.ub_b) * inv_dx;
+    FwdF o;
+    o.r = clampf(2.f * r_cn - r_b, min3f(r_f, r_c, r_b) + dt * gr, max3f(r_f, r_c, r_b) + dt * gr);
+    o.t = clampf(2.f * s.t_cn - s.t_b, min3f(s.t_f, s.t_c, s.t_b) + dt * gt, max3f(s.t_f, s.t_c, s.t_b) + dt * gt);
+    o.a = clampf(2.f * s.ua_cn - s.ua_b, min3f(s.ua_f, s.ua_c, s.ua_b) + dt * ga,
+                 max3f(s.ua_f, s.ua_c, s.ua_b) + dt * ga);
+    o.b = clampf(2.f * s.ub_cn - s.ub_b, min3f(s.ub_f, s.ub_c, s.ub_b) + dt * gb,
+                 max3f(s.ub_f, s.ub_c, s.ub_b) + dt * gb);
+    return o;
+}
+
+// GqUsesR: the X1 and X3 sweeps take gq's time derivative from r
+// (source/main.cpp:1804, 2321), the X2 sweep from q (2083).
+template <bool GqUsesR>
+__device__ __forceinline__ BwdF backward(const Side& s, float dt, float inv_dt2, float inv_dx, const Args& a) {
+    const float q_f = s.un_f - s.p_f * a.inv_rc, q_b = s.un_b - s.p_b * a.inv_rc;
+    const float q_cn = s.un_cn - s.p_cn * a.inv_rc, q_c = s.un_c - s.p_c * a.inv_rc;
+    float dq;
+    if (GqUsesR) {
+        const float r_cn = s.un_cn + s.p_cn * a.inv_rc, r_c = s.un_c + s.p_c * a.inv_rc;
+        dq = (r_cn - r_c) * inv_dt2;
+    } else {
+        dq = (q_cn - q_c) * inv_dt2;
+    }
+    const float gq = dq + (s.un_cn - a.c) * (q_f - q_b) * inv_dx;
+    const float gt = (s.t_cn - s.t_c) * inv_dt2 + s.un_cn * (s.t_f - s.t_b) * inv_dx;
+    const float ga = (s.ua_cn - s.ua_c) * inv_dt2 + s.un_cn * (s.ua_f - s.ua_b) * inv_dx;
+    const float gb = (s.ub_cn - s.ub_c) * inv_dt2 + s.un_cn * (s.ub_f - s.ub_b) * inv_dx;
+    BwdF o;
+    o.q = clampf(2.f * q_cn - q_f, min3f(q_f, q_c, q_b) + dt * gq, max3f(q_f, q_c, q_b) + dt * gq);
+    o.t = clampf(2.f * s.t_cn - s.t_f, min3f(s.t_f, s.t_c, s.t_b) + dt * gt, max3f(s.t_f, s.t_c, s.t_b) + dt * gt);
+    o.a = clampf(2.f * s.ua_cn - s.ua_f, min3f(s.ua_f, s.ua_c, s.ua_b) + dt * ga,
+                 max3f(s.ua_f, s.ua_c, s.ua_b) + dt * ga);
+    o.b = clampf(2.f * s.ub_cn - s.ub_f, min3f(s.ub_f, s.ub_c, s.ub_b) + dt * gb,
+                 max3f(s.ub_f, s.ub_c, s.ub_b) + dt * gb);
+    return o;
+}
+
+struct FaceOutF { float un, p, t, a, b; };
+
+// New face from the cell behind (forward half fw, half-level hb: un, t, a, b)
+// and the cell ahead (bw, hf); upwinding of the passive quantities as in
+// source/main.cpp:1932-1961.
+__device__ __forceinline__ FaceOutF combine(const FwdF& fw, const BwdF& bw, float ucb, float ucf, float tb_,
+                                            float tf_, float ab_, float af_, float bb_, float bf_, float t_old,
+                                            float a_old, float b_old, const Args& a) {
+    FaceOutF o;
+    o.un = (fw.r + bw.q) * 0.5f;
+    o.p = (fw.r - bw.q) * a.half_rc;
+    if (ucf >= 0.f && ucb >= 0.f) {
+        o.t = fw.t; o.a = fw.a; o.b = fw.b;
+    } else if (ucf <= 0.f && ucb <= 0.f) {
+        o.t = bw.t; o.a = bw.a; o.b = bw.b;
+    } else if (ucb >= 0.f && ucf <= 0.f) {
+        if (ucb > -ucf) { o.t = fw.t; o.a = fw.a; o.b = fw.b; }
+        else { o.t = bw.t; o.a = bw.a; o.b = bw.b; }
+    } else {
+        o.t = tf_ + tb_ - t_old;
+        o.a = af_ + ab_ - a_old;
+        o.b = bf_ + bb_ - b_old;
+    }
+    return o;
+}
+
+template <int D>
+__device__ __forceinline__ Flux out_flux(const FaceOutF& o, const Args& a) {
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    float u[3];
+    u[D] = o.un; u[TA] = o.a; u[TB] = o.b;
+    return flux<D>(u, o.p, o.t, a);
+}
+
+template <int D>
+__device__ __forceinline__ void store_face(const Args& a, int idx, const FaceOutF& o) {
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    stf(a.Fo[D], a.fb, D, idx, o.un);
+    stf(a.Fo[D], a.fb, TA, idx, o.a);
+    stf(a.Fo[D], a.fb, TB, idx, o.b);
+    stf(a.Fo[D], a.fb, 3, idx, o.p);
+    stf(a.Fo[D], a.fb, 4, idx, o.t);
+}
+
+template <int D>
+__device__ __forceinline__ Side make_side(const CellF& ac, const CellF& h, const FaceF& fb, const FaceF& ff,
+                                          const Args& a) {
+    constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
+    Side s;
+    s.un_f = ff.u[D]; s.un_b = fb.u[D]; s.un_cn = h.u[D]; s.un_c = ac.u[D];
+    s.ua_f = ff.u[TA]; s.ua_b = fb.u[TA]; s.ua_cn = h.u[TA]; s.ua_c = ac.u[TA];
+    s.ub_f = ff.u[TB]; s.ub_b = fb.u[TB]; s.ub_cn = h.u[TB]; s.ub_c = ac.u[TB];
+    s.p_f = ff.p; s.p_b = fb.p; s.p_cn = a.c2 * h.dr; s.p_c = a.c2 * ac.dr;
+    s.t_f = ff.tp; s.t_b = fb.tp; s.t_cn = h.tp; s.t_c = ac.tp;
+    return s;
+}
+
+struct LdsCell { float h[4]; float fx[4]; float fy[4]; };   // h: u1, u2, u3, t'
+struct LdsFlux { float mx[5], my[5]; };
+union LdsSlot { LdsCell c; LdsFlux f; };
+
+__global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
+    __shared__ LdsSlot lds[NTHR];
+    const Geo& g = a.g;
+    const double dtd = a.S->dt;
+    const float dt = (float)dtd, inv_dt2 = (float)(2.0 / dtd);
+    const double cd = 0.5 * dtd / a.dvc;       // FP64 closing-update factor
+    const float cf = (float)cd;
+    const int tid = threadIdx.x;
+    const int tx = tid % RX, ty = tid / RX;
+    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1, k0 = blockIdx.z * a.LZ + 1;
+    const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = min(k0 + a.LZ, g.N3 + 1);
+    const int i = min(i0 - 1 + tx, iend), j = min(j0 - 1 + ty, jend);
+    const bool in_region = (i0 - 1 + tx <= iend) && (j0 - 1 + ty <= jend);
+    const bool own_x = i >= i0 && i < iend, own_y = j >= j0 && j < jend;
+    const bool owned = in_region && own_x && own_y;
+    const bool xface = in_region && own_y && i >= i0 && i <= iend;
+    const bool yface = in_region && own_x && j >= j0 && j <= jend;
+    const bool xface_store = xface && (i < iend || iend == g.N1 + 1);
+    const bool yface_store = yface && (j < jend || jend == g.N2 + 1);
+    const int iw = wrapc(i, g.N1), jw = wrapc(j, g.N2);
+    const int iwm = wrapc(i - 1, g.N1), iwp = wrapc(i + 1, g.N1);
+    const int jwm = wrapc(j - 1, g.N2), jwp = wrapc(j + 1, g.N2);
+    const int px = g.px, sxy = g.sxy;
+    Args A = a;
+
+    CellD Acur = load_cell(A, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
+    float sz_lo[3];
+    {
+        const CellD Ab = load_cell(A, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
+        for (int m = 0; m < 3; ++m) sz_lo[m] = A.VIS * (float)(Acur.u[m] - Ab.u[m]) * A.inv_dx[2];
+    }
+    // carried from plane p-1 to plane p
+    CellD Aprev{};
+    float Sp_prev[5] = {0, 0, 0, 0, 0};     // predictor flux sums of plane p-1
+    float f_prev[3] = {0, 0, 0};
+    float Sxy_prev[5] = {0, 0, 0, 0, 0};    // corrector in-plane flux sums of plane p-1
+    Flux mz_prev{};
+    CellF Hprev{};
+    FwdF fz_prev{};
+    unsigned long long dtbits = ~0ull;
+
+    for (int p = k0 - 1; p <= kend; ++p) {
+        asm volatile("" : "+s"(A.fb));
+        const int pw = wrapc(p, g.N3);
+        const int c = at(g, iw, jw, pw);
+        const CellD Anext = load_cell(A, at(g, iw, jw, wrapc(p + 1, g.N3)));
+        const CellF Af = to_f(Acur, A);
+
+        // stress force (FP32 differences of FP64 velocities)
+        float f[3];
+        {
+            const int cw = at(g, iwm, jw, pw), ce = at(g, iwp, jw, pw);
+            const int cs = at(g, iw, jwm, pw), cn = at(g, iw, jwp, pw);
+            const float roc = (float)Acur.ro;
+            for (int m = 0; m < 3; ++m) {
+                const double uc = Acur.u[m];
+                const float sxl = A.VIS * (float)(uc - ldd(A.A, A.fb, 1 + m, cw)) * A.inv_dx[0];
+                const float sxh = A.VIS * (float)(ldd(A.A, A.fb, 1 + m, ce) - uc) * A.inv_dx[0];
+                const float syl = A.VIS * (float)(uc - ldd(A.A, A.fb, 1 + m, cs)) * A.inv_dx[1];
+                const float syh = A.VIS * (float)(ldd(A.A, A.fb, 1 + m, cn) - uc) * A.inv_dx[1];
+                const float szh = A.VIS * (float)(Anext.u[m] - uc) * A.inv_dx[2];
+                f[m] = ((sxh - sxl) * A.ds[0] + (syh - syl) * A.ds[1] + (szh - sz_lo[m]) * A.ds[2]) * roc;
+                sz_lo[m] = szh;
+            }
+        }
+
+        // predictor increments, half level (FP32) and the Phase2 extrapolations
+        float Sp[5];
+        CellF H;
+        FwdF fx, fy, fz;
+        BwdF bx, by, bz;
+        float ox[3], oy[3], oz[3];
+        {
+            const FaceF Xw = load_face(A, 0, c), Xe = load_face(A, 0, c + 1);
+            const FaceF Ys = load_face(A, 1, c), Yn = load_face(A, 1, c + px);
+            const FaceF Zb = load_face(A, 2, c), Zt = load_face(A, 2, c + sxy);
+            {
+                const Flux a0 = flux<0>(Xw.u, Xw.p, Xw.tp, A), a1 = flux<0>(Xe.u, Xe.p, Xe.tp, A);
+                const Flux b0 = flux<1>(Ys.u, Ys.p, Ys.tp, A), b1 = flux<1>(Yn.u, Yn.p, Yn.tp, A);
+                const Flux c0 = flux<2>(Zb.u, Zb.p, Zb.tp, A), c1 = flux<2>(Zt.u, Zt.p, Zt.tp, A);
+                for (int v = 0; v < 5; ++v)
+                    Sp[v] = (a1.m[v] - a0.m[v]) * A.ds[0] + (b1.m[v] - b0.m[v]) * A.ds[1] +
+                            (c1.m[v] - c0.m[v]) * A.ds[2];
+            }
+            {
+                const float roA = A.ro0 + Af.dr;
+                H.dr = Af.dr - cf * Sp[0];
+                const float inv_roH = 1.f / (A.ro0 + H.dr);
+                for (int m = 0; m < 3; ++m) H.u[m] = (roA * Af.u[m] - cf * (Sp[1 + m] - f[m])) * inv_roH;
+                H.tp = (roA * Af.tp - cf * Sp[4]) * inv_roH;
+            }
+            FaceF Pxb = Xw, Pxf = Xe, Pyb = Ys, Pyf = Yn, Pzb = Zb, Pzf = Zt;
+            if (i != iw) {
+                const int cl = at(g, 0, jw, pw);
+                Pxb = load_face(A, 0, cl + mapf(i, g.N1));
+                Pxf = load_face(A, 0, cl + mapf(i + 1, g.N1));
+                Pxb.tp = ldf(A.F[0], A.fb, 4, cl + i);
+                Pxf.tp = ldf(A.F[0], A.fb, 4, cl + i + 1);
+            }
+            if (j != jw) {
+                const int cl = at(g, iw, 0, pw);
+                Pyb = load_face(A, 1, cl + mapf(j, g.N2) * px);
+                Pyf = load_face(A, 1, cl + mapf(j + 1, g.N2) * px);
+                Pyb.tp = ldf(A.F[1], A.fb, 4, cl + j * px);
+                Pyf.tp = ldf(A.F[1], A.fb, 4, cl + (j + 1) * px);
+            }
+            if (p != pw) {
+                const int cl = at(g, iw, jw, 0);
+                Pzb = load_face(A, 2, cl + mapf(p, g.N3) * sxy);
+                Pzf = load_face(A, 2, cl + mapf(p + 1, g.N3) * sxy);
+                Pzb.tp = ldf(A.F[2], A.fb, 4, cl + p * sxy);
+                Pzf.tp = ldf(A.F[2], A.fb, 4, cl + (p + 1) * sxy);
+            }
+            {
+                const Side s = make_side<0>(Af, H, Pxb, Pxf, A);
+                fx = forward(s, dt, inv_dt2, A.inv_dx[0], A);
+                bx = backward<true>(s, dt, inv_dt2, A.inv_dx[0], A);
+            }
+            {
+                const Side s = make_side<1>(Af, H, Pyb, Pyf, A);
+                fy = forward(s, dt, inv_dt2, A.inv_dx[1], A);
+                by = backward<false>(s, dt, inv_dt2, A.inv_dx[1], A);
+            }
+            {
+                const Side s = make_side<2>(Af, H, Pzb, Pzf, A);
+                fz = forward(s, dt, inv_dt2, A.inv_dx[2], A);
+                bz = backward<true>(s, dt, inv_dt2, A.inv_dx[2], A);
+            }
+            ox[0] = Pxb.tp; ox[1] = Pxb.u[1]; ox[2] = Pxb.u[2];
+            oy[0] = Pyb.tp; oy[1] = Pyb.u[0]; oy[2] = Pyb.u[2];
+            oz[0] = Pzb.tp; oz[1] = Pzb.u[0]; oz[2] = Pzb.u[1];
+        }
+        if (owned && p >= k0 && p < kend) {
+            const float d1 = A.cfl_dx[0] / (A.c + fabsf(H.u[0]));
+            const float d2 = A.cfl_dx[1] / (A.c + fabsf(H.u[1]));
+            const float d3 = A.cfl_dx[2] / (A.c + fabsf(H.u[2]));
+            const double dmin = (double)fminf(fminf(d1, d2), d3);
+            const unsigned long long b = (unsigned long long)__double_as_longlong(dmin);
+            dtbits = b < dtbits ? b : dtbits;
+        }
+
+        // z-face p, then the FP64 closing update of plane p-1
+        Flux mz{};
+        if (p >= k0) {
+            const FaceOutF o = combine(fz_prev, bz, Hprev.u[2], H.u[2], Hprev.tp, H.tp, Hprev.u[0], H.u[0],
+                                       Hprev.u[1], H.u[1], oz[0], oz[1], oz[2], A);
+            mz = out_flux<2>(o, A);
+            if (owned) {
+                if (p < kend || kend == g.N3 + 1) store_face<2>(A, at(g, i, j, p), o);
+                if (p > k0) {
+                    double Sc[5];
+                    for (int v = 0; v < 5; ++v)
+                        Sc[v] = (double)Sp_prev[v] + (double)(Sxy_prev[v] + (mz.m[v] - mz_prev.m[v]) * A.ds[2]);
+                    const double ro1 = Aprev.ro - cd * Sc[0];
+                    const double inv = 1.0 / ro1;
+                    const int co = at(g, i, j, p - 1);
+                    std_(A.Ao, A.fb, 0, co, ro1);
+                    for (int m = 0; m < 3; ++m) {
+                        const double mom = Aprev.ro * Aprev.u[m] - cd * (Sc[1 + m] - 2.0 * (double)f_prev[m]);
+                        std_(A.Ao, A.fb, 1 + m, co, mom * inv);
+                    }
+                    const double rt = Aprev.ro * (Aprev.t - A.t0d) - cd * Sc[4];
+                    std_(A.Ao, A.fb, 4, co, A.t0d + rt * inv);
+                }
+            }
+        }
+
+        // x/y faces and the in-plane corrector flux sums
+        float Sxy[5] = {0, 0, 0, 0, 0};
+        if (p >= k0 && p < kend) {
+            LdsCell& me = lds[tid].c;
+            me.h[0] = H.u[0]; me.h[1] = H.u[1]; me.h[2] = H.u[2]; me.h[3] = H.tp;
+            me.fx[0] = fx.r; me.fx[1] = fx.t; me.fx[2] = fx.a; me.fx[3] = fx.b;
+            me.fy[0] = fy.r; me.fy[1] = fy.t; me.fy[2] = fy.a; me.fy[3] = fy.b;
+            __syncthreads();
+            Flux mx{}, my{};
+            if (xface) {
+                const LdsCell& nb = lds[tid - 1].c;
+                const FwdF fwb{nb.fx[0], nb.fx[1], nb.fx[2], nb.fx[3]};
+                const FaceOutF o = combine(fwb, bx, nb.h[0], H.u[0], nb.h[3], H.tp, nb.h[1], H.u[1], nb.h[2], H.u[2],
+                                           ox[0], ox[1], ox[2], A);
+                if (xface_store) store_face<0>(A, at(g, i, j, p), o);
+                mx = out_flux<0>(o, A);
+            }
+            if (yface) {
+                const LdsCell& nb = lds[tid - RX].c;
+                const FwdF fwb{nb.fy[0], nb.fy[1], nb.fy[2], nb.fy[3]};
+                const FaceOutF o = combine(fwb, by, nb.h[1], H.u[1], nb.h[3], H.tp, nb.h[0], H.u[0], nb.h[2], H.u[2],
+                                           oy[0], oy[1], oy[2], A);
+                if (yface_store) store_face<1>(A, at(g, i, j, p), o);
+                my = out_flux<1>(o, A);
+            }
+            __syncthreads();
+            LdsFlux& mf = lds[tid].f;
+            for (int v = 0; v < 5; ++v) { mf.mx[v] = mx.m[v]; mf.my[v] = my.m[v]; }
+            __syncthreads();
+            if (owned) {
+                const LdsFlux& e = lds[tid + 1].f;
+                const LdsFlux& n = lds[tid + RX].f;
+                for (int v = 0; v < 5; ++v)
+                    Sxy[v] = (e.mx[v] - mx.m[v]) * A.ds[0] + (n.my[v] - my.m[v]) * A.ds[1];
+            }
+            __syncthreads();
+        }
+
+        Aprev = Acur;
+        Hprev = H;
+        fz_prev = fz;
+        for (int m = 0; m < 3; ++m) f_prev[m] = f[m];
+        for (int v = 0; v < 5; ++v) {
+            Sp_prev[v] = Sp[v];
+            Sxy_prev[v] = Sxy[v];
+        }
+        mz_prev = mz;
+        Acur = Anext;
+    }
+    block_min_atomic(dtbits, A.dt_next);
+}
+
+// ---------------------------------------------------------------------------
+// Conversions between the FP64 API arrays and the FP32 face storage.
+// mode 0: u or p (plain), mode 1: t (t' = t - t0).
+__global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, long long n, int mode, double t0) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+        dst[q] = (float)(mode == 1 ? src[q] - t0 : src[q]);
+}
+// mode 0: plain, mode 1: t = t0 + t', mode 2: rho = rho0 + p/c^2 from p
+__global__ void k_to_double(const float* __restrict__ src, double* __restrict__ dst, long long n, int mode,
+                            double t0, double ro0, double c2) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        const double v = (double)src[q];
+        dst[q] = mode == 1 ? t0 + v : (mode == 2 ? ro0 + v / c2 : v);
+    }
+}
+
+}  // namespace mixed
+}  // namespace arkd
+
+// ---------------------------------------------------------------------------
+using namespace arkd;
+
+namespace {
+
+struct MixedLaunch {
+    ark_ctx* c;
+    int LZ;
+};
+
+hipError_t run_mixed(void* p) {
+    MixedLaunch* ml = (MixedLaunch*)p;
+    ark_ctx* c = ml->c;
+    const ark_params& P = c->prm;
+    mixed::Args a;
+    a.g = c->geo;
+    a.VIS = (float)P.VIS;
+    a.c = (float)P.sound;
+    a.c2 = (float)(P.sound * P.sound);
+    a.inv_c2 = (float)(1.0 / (P.sound * P.sound));
+    a.inv_rc = (float)(1.0 / (P.ro0 * P.sound));
+    a.half_rc = (float)(0.5 * P.sound * P.ro0);
+    a.ro0 = (float)P.ro0;
+    const double dx[3] = {P.dx1, P.dx2, P.dx3};
+    for (int d = 0; d < 3; ++d) {
+        a.inv_dx[d] = (float)(1.0 / dx[d]);
+        a.cfl_dx[d] = (float)(P.CFL * dx[d]);
+    }
+    a.ds[0] = (float)c->K.ds1;
+    a.ds[1] = (float)c->K.ds2;
+    a.ds[2] = (float)c->K.ds3;
+    a.ro0d = P.ro0;
+    a.t0d = P.t0;
+    a.dvc = c->K.dvc;
+    a.A = c->gbase[0][c->gpar[0]];
+    a.Ao = c->gbase[0][c->gpar[0] ^ 1];
+    for (int d = 0; d < 3; ++d) {
+        a.F[d] = c->gbase[1 + d][c->gpar[1 + d]];
+        a.Fo[d] = c->gbase[1 + d][c->gpar[1 + d] ^ 1];
+    }
+    a.fb = c->field_bytes;
+    a.S = c->S;
+    a.dt_next = &c->S->dt_bits;
+    a.LZ = ml->LZ;
+    dim3 grid((c->geo.N1 + mixed::TX - 1) / mixed::TX, (c->geo.N2 + mixed::TY - 1) / mixed::TY,
+              (c->geo.N3 + ml->LZ - 1) / ml->LZ);
+    hipLaunchKernelGGL(mixed::k_fused_mixed, grid, dim3(mixed::NTHR), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+int choose_lz_mixed(const Geo& g) {
+    long long cols = (long long)((g.N1 + mixed::TX - 1) / mixed::TX) * ((g.N2 + mixed::TY - 1) / mixed::TY);
+    int lz = 64;
+    while (lz > 8 && cols * ((g.N3 + lz - 1) / lz) < 2048) lz /= 2;
+    return lz < g.N3 ? lz : g.N3;
+}
+
+int face_mode(int f) { return (f == F_t1 || f == F_t2 || f == F_t3) ? 1 : 0; }
+
+}  // namespace
+
+ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
+    if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)
+        return c->fail(ARK_ERR_UNSUPPORTED, "mixed precision needs N1, N2, N3 >= 2");
+    ark_status s = ARK_OK;
+    MixedLaunch ml{c, choose_lz_mixed(c->geo)};
+    for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
+        s = ark_begin_step(c);
+        if (s == ARK_OK) s = c->launch("fused_mixed", run_mixed, &ml);
+        if (s != ARK_OK) break;
+        for (int g = 0; g < 4; ++g) ark_swap_group(c, g);
+        c->dt_pending = true;
+    }
+    return s;
+}
+
+// The FP32 storage of one face field, as seen through the FP64 API.  A face
+// field's FP32 element i sits 4*i bytes from the field's element (0,0,0),
+// its FP64 element i 8*i bytes; both views cover elems elements starting
+// 15 elements before (0,0,0).
+ark_status ark_mixed_face_to_host_layout(ark_ctx* c, int f, double* staging) {
+    const ark_params& P = c->prm;
+    const long long n = (long long)c->elems;
+    if (f == F_ro1 || f == F_ro2 || f == F_ro3) {
+        const int pf = f == F_ro1 ? F_p1 : f == F_ro2 ? F_p2 : F_p3;
+        hipLaunchKernelGGL(mixed::k_to_double, dim3(2048), dim3(256), 0, c->stream,
+                           (const float*)c->cur[pf] - 15, staging - 15, n, 2, P.t0, P.ro0, P.sound * P.sound);
+    } else {
+        hipLaunchKernelGGL(mixed::k_to_double, dim3(2048), dim3(256), 0, c->stream,
+                           (const float*)c->cur[f] - 15, staging - 15, n, face_mode(f), P.t0, P.ro0,
+                           P.sound * P.sound);
+    }
+    return c->hip(hipGetLastError(), "to_double");
+}
+
+// FP64 values (staging, field layout) into the FP32 storage of face field f,
+// both buffers.
+ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* staging) {
+    const long long n = (long long)c->elems;
+    for (double* dst : {c->cur[f], c->alt[f]}) {
+        hipLaunchKernelGGL(mixed::k_to_float, dim3(2048), dim3(256), 0, c->stream, staging - 15,
+                           (float*)dst - 15, n, face_mode(f), c->prm.t0);
+        ark_status s = c->hip(hipGetLastError(), "to_float");
+        if (s != ARK_OK) return s;
+    }
+    return ARK_OK;
+}

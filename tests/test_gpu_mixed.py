@@ -1,0 +1,112 @@
+"""Mixed-precision mode (FP32 fluxes/invariants/stress, FP64 closing update
+and global sums) against the FP64 path, which is itself bitwise equal to the
+reference.  Tolerances are written per test.
+"""
+import numpy as np
+import pytest
+
+import arkamd
+
+pytestmark = pytest.mark.gpu
+
+CELL = ["ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon"]
+FACE = ["u11", "u21", "u31", "p1", "t1", "u12", "u22", "u32", "p2", "t2", "u13", "u23", "u33", "p3", "t3"]
+
+
+def interior(name, a, N):
+    N1, N2, N3 = N
+    if name.endswith("Con"):
+        return a[1:N3 + 1, 1:N2 + 1, 1:N1 + 1]
+    d = int(name[-1]) - 1
+    e = [N1 + (d == 0), N2 + (d == 1), N3 + (d == 2)]
+    return a[1:e[2] + 1, 1:e[1] + 1, 1:e[0] + 1]
+
+
+def run_pair(N, steps):
+    out = {}
+    for prec in (arkamd.PREC_FP64, arkamd.PREC_MIXED):
+        with arkamd.Solver(*N, precision=prec) as s:
+            s.init_tgv()
+            s.step(steps)
+            out[prec] = ({n: interior(n, s.download(n), N) for n in CELL + FACE}, s.diagnostics())
+    return out
+
+
+def rel_l2(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def test_mixed_init_roundtrip():
+    N = (16, 12, 10)
+    with arkamd.Solver(*N) as d, arkamd.Solver(*N, precision=arkamd.PREC_MIXED) as m:
+        d.init_tgv()
+        m.init_tgv()
+        for n in CELL:   # FP64 cell storage: identical
+            assert np.array_equal(d.download(n), m.download(n)), n
+        for n in FACE:   # FP32 face storage: FP64 values rounded once
+            a, b = d.download(n), m.download(n)
+            if n.startswith("t"):
+                assert np.array_equal(a, b)          # t' = 0 is exact
+            else:
+                assert np.array_equal(a.astype(np.float32).astype(np.float64), b), n
+        # face densities are derived from p
+        ro = m.download("ro2")
+        p = m.download("p2")
+        assert np.allclose(ro[1:-1, 1:-1, 1:-1], 1.0 + p[1:-1, 1:-1, 1:-1] / 100.0, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("N,steps,tol", [((32, 32, 32), 50, 1e-5), ((24, 20, 16), 30, 1e-5)])
+def test_mixed_close_to_fp64(N, steps, tol):
+    """Relative L2 error per field after `steps` steps; tolerance tol on the
+    velocities and density (relative to the field), looser on p (relative to
+    the pressure perturbation itself, which is Ma^2 smaller)."""
+    r = run_pair(N, steps)
+    f64, dmix = r[arkamd.PREC_FP64], r[arkamd.PREC_MIXED]
+    errs = {n: rel_l2(dmix[0][n], f64[0][n]) for n in CELL + FACE}
+    for n in ("ronCon", "u1nCon", "u2nCon", "u11", "u22"):
+        assert errs[n] < tol, (n, errs[n])
+    for n in ("p1", "p2", "p3"):
+        assert errs[n] < 1e-3, (n, errs[n])
+    # the density perturbation rho - 1 itself
+    dr_err = rel_l2(dmix[0]["ronCon"] - 1.0, f64[0]["ronCon"] - 1.0)
+    assert dr_err < 1e-3, dr_err
+    assert abs(dmix[1]["dt"] - f64[1]["dt"]) / f64[1]["dt"] < 1e-6
+    assert dmix[1]["nstep"] == f64[1]["nstep"] == steps
+
+
+def test_mixed_mass_conservation():
+    with arkamd.Solver(32, 32, 32, precision=arkamd.PREC_MIXED) as s:
+        s.init_tgv()
+        m0 = s.diagnostics()["mass"]
+        s.step(40)
+        m1 = s.diagnostics()["mass"]
+    assert abs(m1 - m0) / m0 < 1e-9
+
+
+def test_mixed_uniform_state_stays_uniform():
+    N = (12, 12, 12)
+    with arkamd.Solver(*N, precision=arkamd.PREC_MIXED) as s:
+        for n in arkamd.FIELDS:
+            shp = s.shape(n)
+            if n in ("ronCon", "roCon") or n.startswith("ro"):
+                s.upload(n, np.ones(shp))
+            elif n.startswith("t"):
+                s.upload(n, np.ones(shp))
+            elif n in ("u1nCon", "u1Con") or n in ("u11", "u12", "u13"):
+                s.upload(n, np.full(shp, 0.5))
+            else:
+                s.upload(n, np.zeros(shp))
+        s.step(5)
+        u = interior("u1nCon", s.download("u1nCon"), N)
+        ro = interior("ronCon", s.download("ronCon"), N)
+        assert np.abs(u - 0.5).max() < 1e-6 and np.abs(ro - 1.0).max() < 1e-6
+
+
+def test_mixed_rejects_phase_calls():
+    with arkamd.Solver(8, 8, 8, precision=arkamd.PREC_MIXED) as s:
+        s.init_tgv()
+        with pytest.raises(arkamd.ArkError) as e:
+            s.phase1()
+        assert e.value.status == arkamd.ARK_ERR_UNSUPPORTED
+        s.step(2)   # the fused mixed path works on the same context
+        assert s.diagnostics()["nstep"] == 2
