@@ -40,6 +40,7 @@ FACE_FIRST = FIELD_ID["ro1"]
 
 ARK_OK, ARK_ERR_INVALID, ARK_ERR_HIP, ARK_ERR_UNSUPPORTED, ARK_ERR_NONFINITE, ARK_ERR_STATE = range(6)
 PREC_FP64, PREC_MIXED = 0, 1
+SCHEME_REFERENCE, SCHEME_CORRECTED = 0, 1
 
 EXPORTED_SYMBOLS = [
     "ark_create", "ark_destroy", "ark_last_error", "ark_upload", "ark_download", "ark_set_dt",
@@ -63,7 +64,7 @@ class Params(ctypes.Structure):
         ("CFL", ctypes.c_double), ("VIS", ctypes.c_double), ("sound", ctypes.c_double),
         ("ro0", ctypes.c_double), ("t0", ctypes.c_double),
         ("periodic", ctypes.c_int * 3),
-        ("precision", ctypes.c_int), ("device", ctypes.c_int),
+        ("precision", ctypes.c_int), ("device", ctypes.c_int), ("scheme", ctypes.c_int),
     ]
 
 
@@ -138,7 +139,7 @@ class Solver:
 
     def __init__(self, N1: int, N2: int, N3: int, dx=None, CFL: float = 0.3,
                  VIS: float = 1.0 / 1600.0, sound: float = 10.0, ro0: float = 1.0, t0: float = 1.0,
-                 precision: int = PREC_FP64, device: int = 0):
+                 precision: int = PREC_FP64, device: int = 0, scheme: int = SCHEME_REFERENCE):
         self.N = (int(N1), int(N2), int(N3))
         if dx is None:
             two_pi = 2.0 * np.pi
@@ -149,7 +150,7 @@ class Solver:
         p.dx1, p.dx2, p.dx3 = self.dx
         p.CFL, p.VIS, p.sound, p.ro0, p.t0 = CFL, VIS, sound, ro0, t0
         p.periodic[0] = p.periodic[1] = p.periodic[2] = 1
-        p.precision, p.device = precision, device
+        p.precision, p.device, p.scheme = precision, device, scheme
         self.params = p
         self._L = lib()
         h = ctypes.c_void_p()

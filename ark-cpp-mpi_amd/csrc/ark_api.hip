@@ -106,6 +106,11 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
         *out = c;
         return ARK_ERR_UNSUPPORTED;
     }
+    if (p->scheme != ARK_SCHEME_REFERENCE && p->scheme != ARK_SCHEME_CORRECTED) {
+        c->err = "scheme must be ARK_SCHEME_REFERENCE or ARK_SCHEME_CORRECTED";
+        *out = c;
+        return ARK_ERR_INVALID;
+    }
     if (p->precision != ARK_PREC_FP64 && p->precision != ARK_PREC_MIXED) {
         c->err = "precision must be ARK_PREC_FP64 or ARK_PREC_MIXED";
         *out = c;
@@ -410,7 +415,7 @@ hipError_t run_phase2_axis(void* p) {
     nx.p[3] = c->alt[face_p(d)];
     nx.p[4] = c->alt[face_ro(d)];
     nx.p[5] = c->alt[face_t(d)];
-    return launch_phase2(c->stream, d, c->geo, c->fields(), nx, c->K, c->S);
+    return launch_phase2(c->stream, d, c->prm.scheme, c->geo, c->fields(), nx, c->K, c->S);
 }
 
 }  // namespace

@@ -181,6 +181,7 @@ union LdsSlot {
     LdsFlux f;
 };
 
+template <bool Quirk>
 __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
     __shared__ LdsSlot lds[NTHR];
     const Geo& g = a.g;
@@ -298,7 +299,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
             {
                 const CellSide s = make_side<0>(A_cur, H, Pxb, Pxf);
                 fx = cell_forward(s, dt, K.dx[0], K);
-                bx = cell_backward<true>(s, dt, K.dx[0], K);
+                bx = cell_backward<Quirk>(s, dt, K.dx[0], K);
             }
             {
                 const CellSide s = make_side<1>(A_cur, H, Pyb, Pyf);
@@ -308,7 +309,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
             {
                 const CellSide s = make_side<2>(A_cur, H, Pzb, Pzf);
                 fz = cell_forward(s, dt, K.dx[2], K);
-                bz = cell_backward<true>(s, dt, K.dx[2], K);
+                bz = cell_backward<Quirk>(s, dt, K.dx[2], K);
             }
             ox[0] = Pxb.t; ox[1] = Pxb.u2; ox[2] = Pxb.u3;
             oy[0] = Pyb.t; oy[1] = Pyb.u1; oy[2] = Pyb.u3;
@@ -424,7 +425,10 @@ hipError_t run_fused(void* p) {
     a.dt_next = &c->S->dt_bits;
     a.LZ = fl->LZ;
     dim3 grid((c->geo.N1 + TX - 1) / TX, (c->geo.N2 + TY - 1) / TY, (c->geo.N3 + fl->LZ - 1) / fl->LZ);
-    hipLaunchKernelGGL(k_fused_step, grid, dim3(NTHR), 0, c->stream, a);
+    if (c->prm.scheme == ARK_SCHEME_REFERENCE)
+        hipLaunchKernelGGL(k_fused_step<true>, grid, dim3(NTHR), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_fused_step<false>, grid, dim3(NTHR), 0, c->stream, a);
     return hipGetLastError();
 }
 

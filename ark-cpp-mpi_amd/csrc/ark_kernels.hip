@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(256) k_use_forces(Geo g, Fields F, Consts K, c
 // read from the current buffer and written to the next one (the reference's
 // in-place update reads only old values).  Threads of the first and last face
 // also carry the ghost faces 0 and N_D+2, set by the ghost fill, across.
-template <int D>
+template <int D, bool GqUsesR>
 __global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, Consts K, const Scalars* S) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, C
     sf.p_f = P[cf]; sf.p_b = P[c];
 
     Fwd fw = cell_forward(sb, dt, dx, K);
-    Bwd bw = cell_backward<D != 1>(sf, dt, dx, K);
+    Bwd bw = cell_backward<GqUsesR>(sf, dt, dx, K);
     FaceOut o = face_combine(fw, bw, sb.un_cn, sf.un_cn, sb.t_cn, sf.t_cn, sb.ua_cn, sf.ua_cn, sb.ub_cn,
                              sf.ub_cn, T[c], Ua[c], Ub[c], K);
     nx.p[0][c] = o.un;
@@ -335,9 +335,6 @@ __global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, C
     }
 }
 
-template __global__ void k_phase2<0>(Geo, Fields, NextFaces, Consts, const Scalars*);
-template __global__ void k_phase2<1>(Geo, Fields, NextFaces, Consts, const Scalars*);
-template __global__ void k_phase2<2>(Geo, Fields, NextFaces, Consts, const Scalars*);
 
 // ---------------------------------------------------------------------------
 // Face densities from face pressures, rho = ro0 + p/c^2 (source/main.cpp:1930),
@@ -525,14 +522,19 @@ hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, cons
     return hipGetLastError();
 }
 
-hipError_t launch_phase2(hipStream_t st, int axis, const Geo& g, const Fields& F, const NextFaces& nx,
+hipError_t launch_phase2(hipStream_t st, int axis, int scheme, const Geo& g, const Fields& F, const NextFaces& nx,
                          const Consts& K, const Scalars* S) {
-    if (axis == 0)
-        hipLaunchKernelGGL(k_phase2<0>, cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S);
-    else if (axis == 1)
-        hipLaunchKernelGGL(k_phase2<1>, cell_grid(g.N1, g.N2 + 1, g.N3), kBlock, 0, st, g, F, nx, K, S);
-    else
-        hipLaunchKernelGGL(k_phase2<2>, cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S);
+    // reference scheme: gq from R on the X1 and X3 sweeps; corrected: from Q everywhere
+    const bool quirk = scheme == 0;
+    if (axis == 0) {
+        if (quirk) hipLaunchKernelGGL((k_phase2<0, true>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S);
+        else hipLaunchKernelGGL((k_phase2<0, false>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S);
+    } else if (axis == 1) {
+        hipLaunchKernelGGL((k_phase2<1, false>), cell_grid(g.N1, g.N2 + 1, g.N3), kBlock, 0, st, g, F, nx, K, S);
+    } else {
+        if (quirk) hipLaunchKernelGGL((k_phase2<2, true>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S);
+        else hipLaunchKernelGGL((k_phase2<2, false>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S);
+    }
     return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, co
 hipError_t launch_zero(hipStream_t st, double* const* dev_ptrs, int narr, long long n);
 hipError_t launch_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
-hipError_t launch_phase2(hipStream_t st, int axis, const Geo& g, const Fields& F, const NextFaces& nx,
+hipError_t launch_phase2(hipStream_t st, int axis, int scheme, const Geo& g, const Fields& F, const NextFaces& nx,
                          const Consts& K, const Scalars* S);
 hipError_t launch_rho_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_diag(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, double* part, int nblocks,

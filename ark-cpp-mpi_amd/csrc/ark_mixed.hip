@@ -234,6 +234,7 @@ struct LdsCell { float h[4]; float fx[4]; float fy[4]; };   // h: u1, u2, u3, t'
 struct LdsFlux { float mx[5], my[5]; };
 union LdsSlot { LdsCell c; LdsFlux f; };
 
+template <bool Quirk>
 __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     __shared__ LdsSlot lds[NTHR];
     const Geo& g = a.g;
@@ -350,7 +351,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
             {
                 const Side s = make_side<0>(Af, H, Pxb, Pxf, A);
                 fx = forward(s, dt, inv_dt2, A.inv_dx[0], A);
-                bx = backward<true>(s, dt, inv_dt2, A.inv_dx[0], A);
+                bx = backward<Quirk>(s, dt, inv_dt2, A.inv_dx[0], A);
             }
             {
                 const Side s = make_side<1>(Af, H, Pyb, Pyf, A);
@@ -360,7 +361,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
             {
                 const Side s = make_side<2>(Af, H, Pzb, Pzf, A);
                 fz = forward(s, dt, inv_dt2, A.inv_dx[2], A);
-                bz = backward<true>(s, dt, inv_dt2, A.inv_dx[2], A);
+                bz = backward<Quirk>(s, dt, inv_dt2, A.inv_dx[2], A);
             }
             ox[0] = Pxb.tp; ox[1] = Pxb.u[1]; ox[2] = Pxb.u[2];
             oy[0] = Pyb.tp; oy[1] = Pyb.u[0]; oy[2] = Pyb.u[2];
@@ -518,7 +519,10 @@ hipError_t run_mixed(void* p) {
     a.LZ = ml->LZ;
     dim3 grid((c->geo.N1 + mixed::TX - 1) / mixed::TX, (c->geo.N2 + mixed::TY - 1) / mixed::TY,
               (c->geo.N3 + ml->LZ - 1) / ml->LZ);
-    hipLaunchKernelGGL(mixed::k_fused_mixed, grid, dim3(mixed::NTHR), 0, c->stream, a);
+    if (c->prm.scheme == ARK_SCHEME_REFERENCE)
+        hipLaunchKernelGGL(mixed::k_fused_mixed<true>, grid, dim3(mixed::NTHR), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(mixed::k_fused_mixed<false>, grid, dim3(mixed::NTHR), 0, c->stream, a);
     return hipGetLastError();
 }
 

@@ -34,6 +34,17 @@ typedef enum {
     ARK_ERR_STATE = 5         /* call out of order for the context's state */
 } ark_status;
 
+/* Which CABARET variant to run.  REFERENCE reproduces the reference bit for
+ * bit, including its X1/X3 sweeps taking gq's time derivative from the
+ * R invariant (source/main.cpp:1804, 2321; X2 uses Q, 2083).  On a periodic
+ * Taylor-Green vortex that variant grows spurious velocities after some
+ * hundred steps.  CORRECTED takes gq from Q on every axis, as the reference's
+ * X2 sweep does; it has no counterpart in the reference. */
+typedef enum {
+    ARK_SCHEME_REFERENCE = 0,
+    ARK_SCHEME_CORRECTED = 1
+} ark_scheme;
+
 typedef enum {
     ARK_PREC_FP64 = 0,   /* every operation in IEEE binary64, bitwise equal to the reference */
     ARK_PREC_MIXED = 1   /* FP32 fluxes/invariants/stress, FP64 closing update and sums */
@@ -67,6 +78,7 @@ typedef struct {
     int periodic[3];
     int precision;   /* ark_precision */
     int device;      /* HIP device ordinal */
+    int scheme;      /* ark_scheme */
 } ark_params;
 
 typedef struct ark_ctx ark_ctx;

@@ -285,7 +285,7 @@ static void phase2_sweep(ark_oracle_state* s, const geom* g, int d, line_buffers
     const int tb = d == 2 ? 1 : 2;
     const double dx = d == 0 ? s->dx1 : (d == 1 ? s->dx2 : s->dx3);
     const double sound = s->sound, ro0_g = s->ro0, dt = s->dt, t0 = s->t0;
-    const int gq_uses_r = (d != 1);
+    const int gq_uses_r = (d != 1) && s->scheme == 0;
     const int n = g->N[d] + 1; /* the reference's node count along d */
 
     const double* Un = f[uface[d][d]];

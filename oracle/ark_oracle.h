@@ -37,6 +37,7 @@ typedef struct {
     double dx1, dx2, dx3;
     double dt, CFL, VIS, sound, ro0, t0;
     double* f[ORC_NUM_FIELDS];
+    int scheme;   /* 0: the reference (gq from R on X1/X3); 1: gq from Q on every axis */
 } ark_oracle_state;
 
 /* 1 for face/stress-sized fields, 0 for cell-sized ones. */

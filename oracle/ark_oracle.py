@@ -51,6 +51,7 @@ class _State(ctypes.Structure):
         ("dt", ctypes.c_double), ("CFL", ctypes.c_double), ("VIS", ctypes.c_double),
         ("sound", ctypes.c_double), ("ro0", ctypes.c_double), ("t0", ctypes.c_double),
         ("f", ctypes.POINTER(ctypes.c_double) * len(FIELDS)),
+        ("scheme", ctypes.c_int),
     ]
 
 
@@ -90,8 +91,9 @@ class OracleState:
 
     def __init__(self, N1: int, N2: int, N3: int, dx=(1.0, 1.0, 1.0), dt: float = 0.0,
                  CFL: float = 0.3, VIS: float = 1.0 / 1600.0, sound: float = 10.0,
-                 ro0: float = 1.0, t0: float = 1.0):
+                 ro0: float = 1.0, t0: float = 1.0, scheme: int = 0):
         self.N = (int(N1), int(N2), int(N3))
+        self.scheme = int(scheme)
         self.dx = tuple(float(v) for v in dx)
         self.dt, self.CFL, self.VIS = float(dt), float(CFL), float(VIS)
         self.sound, self.ro0, self.t0 = float(sound), float(ro0), float(t0)
@@ -116,7 +118,7 @@ class OracleState:
 
     def copy(self) -> "OracleState":
         o = OracleState(*self.N, dx=self.dx, dt=self.dt, CFL=self.CFL, VIS=self.VIS,
-                        sound=self.sound, ro0=self.ro0, t0=self.t0)
+                        sound=self.sound, ro0=self.ro0, t0=self.t0, scheme=self.scheme)
         for k, v in self.arrays.items():
             o.arrays[k] = v.copy()
         return o
@@ -127,6 +129,7 @@ class OracleState:
         s.dx1, s.dx2, s.dx3 = self.dx
         s.dt, s.CFL, s.VIS, s.sound, s.ro0, s.t0 = (self.dt, self.CFL, self.VIS,
                                                    self.sound, self.ro0, self.t0)
+        s.scheme = self.scheme
         for q, name in enumerate(FIELDS):
             a = self.arrays[name]
             assert a.flags.c_contiguous and a.dtype == np.float64
@@ -179,12 +182,12 @@ class OracleState:
     @classmethod
     def tgv(cls, N1: int, N2: Optional[int] = None, N3: Optional[int] = None,
             VIS: float = 1.0 / 1600.0, CFL: float = 0.3, sound: float = 10.0,
-            U0: float = 1.0) -> "OracleState":
+            U0: float = 1.0, scheme: int = 0) -> "OracleState":
         N2 = N1 if N2 is None else N2
         N3 = N1 if N3 is None else N3
         two_pi = 2.0 * np.pi
         dx = ((two_pi - 0.0) / N1, (two_pi - 0.0) / N2, (two_pi - 0.0) / N3)
-        st = cls(N1, N2, N3, dx=dx, CFL=CFL, VIS=VIS, sound=sound)
+        st = cls(N1, N2, N3, dx=dx, CFL=CFL, VIS=VIS, sound=sound, scheme=scheme)
         s = st._struct()
         lib().ark_oracle_init_tgv(ctypes.byref(s), U0)
         return st

@@ -22,10 +22,10 @@ def interior(name, a, N):
     return a[1:e[2] + 1, 1:e[1] + 1, 1:e[0] + 1]
 
 
-def run_pair(N, steps):
+def run_pair(N, steps, scheme=arkamd.SCHEME_CORRECTED):
     out = {}
     for prec in (arkamd.PREC_FP64, arkamd.PREC_MIXED):
-        with arkamd.Solver(*N, precision=prec) as s:
+        with arkamd.Solver(*N, precision=prec, scheme=scheme) as s:
             s.init_tgv()
             s.step(steps)
             out[prec] = ({n: interior(n, s.download(n), N) for n in CELL + FACE}, s.diagnostics())
@@ -55,11 +55,12 @@ def test_mixed_init_roundtrip():
         assert np.allclose(ro[1:-1, 1:-1, 1:-1], 1.0 + p[1:-1, 1:-1, 1:-1] / 100.0, rtol=0, atol=1e-15)
 
 
-@pytest.mark.parametrize("N,steps,tol", [((32, 32, 32), 50, 1e-5), ((24, 20, 16), 30, 1e-5)])
+@pytest.mark.parametrize("N,steps,tol", [((32, 32, 32), 100, 1e-5), ((24, 20, 16), 60, 1e-5)])
 def test_mixed_close_to_fp64(N, steps, tol):
-    """Relative L2 error per field after `steps` steps; tolerance tol on the
-    velocities and density (relative to the field), looser on p (relative to
-    the pressure perturbation itself, which is Ma^2 smaller)."""
+    """Relative L2 error per field after `steps` steps of the corrected scheme
+    (the reference scheme goes unstable on this vortex, after which any
+    rounding difference grows to O(1)); tolerance tol on velocities and
+    density, looser on p and rho - 1, which are Ma^2 smaller than u."""
     r = run_pair(N, steps)
     f64, dmix = r[arkamd.PREC_FP64], r[arkamd.PREC_MIXED]
     errs = {n: rel_l2(dmix[0][n], f64[0][n]) for n in CELL + FACE}
@@ -110,3 +111,11 @@ def test_mixed_rejects_phase_calls():
         assert e.value.status == arkamd.ARK_ERR_UNSUPPORTED
         s.step(2)   # the fused mixed path works on the same context
         assert s.diagnostics()["nstep"] == 2
+
+
+def test_mixed_reference_scheme_before_instability():
+    """The reference scheme too, over a stretch where it is still stable."""
+    r = run_pair((32, 32, 32), 30, scheme=arkamd.SCHEME_REFERENCE)
+    f64, dmix = r[arkamd.PREC_FP64], r[arkamd.PREC_MIXED]
+    for n in ("ronCon", "u1nCon", "u2nCon", "u11", "u22"):
+        assert rel_l2(dmix[0][n], f64[0][n]) < 1e-5, n

@@ -225,3 +225,31 @@ def test_errors_are_reported():
     with pytest.raises(arkamd.ArkError) as e:
         arkamd.Solver(0, 4, 4, dx=(1.0, 1.0, 1.0))
     assert e.value.status == arkamd.ARK_ERR_INVALID
+
+
+# ---- corrected scheme (gq from Q on every axis) ------------------------------
+def test_corrected_scheme_phases_vs_oracle():
+    st = random_fluid_state((13, 9, 7), 21)
+    st.scheme = 1
+    for phase in PHASES:
+        want = st.copy()
+        want.phase(phase)
+        with arkamd.Solver(*st.N, dx=st.dx, CFL=st.CFL, VIS=st.VIS, sound=st.sound, ro0=st.ro0, t0=st.t0,
+                           scheme=arkamd.SCHEME_CORRECTED) as s:
+            s.upload_all(st.arrays)
+            s.dt = st.dt
+            if phase == "TimeStepSize":
+                assert s.time_step_size() == want.dt
+            else:
+                s.phase(phase)
+            assert_state_equal(s, want, what=f"corrected {phase}")
+
+
+@pytest.mark.parametrize("N", [(24, 20, 16), (40, 33, 9)])
+def test_corrected_scheme_fused_vs_oracle(N):
+    ref = OracleState.tgv(*N, scheme=1)
+    ref.step(6)
+    with arkamd.Solver(*N, scheme=arkamd.SCHEME_CORRECTED) as s:
+        s.init_tgv()
+        s.step(6)
+        assert_state_equal(s, ref, what=f"corrected fused {N}")

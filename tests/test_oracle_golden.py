@@ -135,3 +135,82 @@ def test_live_reference_nonpow2(tmp_path, N):
         ref = OracleState.from_dir(os.path.join(tmp_path, f"s{s}"))
         for name in FIELDS:
             assert bits_equal(st[name], ref[name]), f"step {s}: " + describe_mismatch(name, st[name], ref[name])
+
+
+# ---------------------------------------------------------------------------
+# The corrected scheme (gq from Q on every axis) has no counterpart in the
+# reference, so it is pinned by a symmetry instead: exchanging the x and y
+# axes (arrays transposed, velocity components, face orientations and dx
+# exchanged) must commute with a step, bit for bit.  The reference scheme
+# breaks this symmetry (its X1 sweep uses R where X2 uses Q), which the
+# second test shows, so the check has teeth.
+
+def _swap_name(n):
+    sw = {"1": "2", "2": "1", "3": "3"}
+    if n in ("roCon", "tCon", "ronCon", "tnCon", "condition"):
+        return n
+    if n in ("u1Con", "u2Con", "u3Con"):
+        return "u" + sw[n[1]] + "Con"
+    if n in ("u1nCon", "u2nCon", "u3nCon"):
+        return "u" + sw[n[1]] + "nCon"
+    if n in ("f1", "f2", "f3"):
+        return "f" + sw[n[1]]
+    if n.startswith("sigm"):
+        return "sigm" + sw[n[4]] + sw[n[5]]
+    if n[0] == "u":                    # u{m}{d}
+        return "u" + sw[n[1]] + sw[n[2]]
+    return n[:-1] + sw[n[-1]]          # ro{d}, t{d}, p{d}
+
+
+def _swap_xy(st):
+    N1, N2, N3 = st.N
+    o = OracleState(N2, N1, N3, dx=(st.dx[1], st.dx[0], st.dx[2]), dt=st.dt, CFL=st.CFL, VIS=st.VIS,
+                    sound=st.sound, ro0=st.ro0, t0=st.t0, scheme=st.scheme)
+    for n in FIELDS:
+        o.arrays[_swap_name(n)] = np.ascontiguousarray(np.swapaxes(st[n], 1, 2))
+    return o
+
+
+def _random_fluid(N, seed, scheme):
+    rng = np.random.default_rng(seed)
+    st = OracleState(*N, dx=(0.3, 0.3, 0.2), dt=0.01, CFL=0.3, VIS=0.02, scheme=scheme)
+    for n in FIELDS:
+        shp = st.shape(n)
+        if n == "condition":
+            continue
+        if n.startswith("ro"):
+            st.arrays[n] = 1.0 + 0.02 * rng.uniform(-1, 1, shp)
+        elif n.startswith("p"):
+            st.arrays[n] = 2.0 * rng.uniform(-1, 1, shp)
+        elif n.startswith("t"):
+            st.arrays[n] = 1.0 + 0.1 * rng.uniform(-1, 1, shp)
+        else:
+            st.arrays[n] = rng.uniform(-1, 1, shp)
+    return st
+
+
+def test_corrected_scheme_is_xy_symmetric():
+    st = _random_fluid((7, 5, 4), 7, scheme=1)
+    a = _swap_xy(st)
+    st.step(2)
+    a.step(2)
+    b = _swap_xy(st)
+    for n in FIELDS:
+        assert bits_equal(a[n], b[n]), describe_mismatch(n, a[n], b[n])
+
+
+def test_reference_scheme_is_not_xy_symmetric():
+    st = _random_fluid((7, 5, 4), 7, scheme=0)
+    a = _swap_xy(st)
+    st.step(1)
+    a.step(1)
+    b = _swap_xy(st)
+    assert not all(bits_equal(a[n], b[n]) for n in ("u11", "u12", "p1", "p2"))
+
+
+def test_corrected_scheme_keeps_tgv_bounded():
+    """64 steps at 24^3: the corrected scheme's max |u| stays below U0."""
+    st = OracleState.tgv(24, scheme=1)
+    st.step(150)
+    umax = max(float(np.abs(st[n][1:-1, 1:-1, 1:-1]).max()) for n in ("u1nCon", "u2nCon", "u3nCon"))
+    assert umax < 1.0
