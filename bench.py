@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X CABARET/ARK step: Mcell-steps/s on the 256^3
-Taylor-Green vortex (periodic box of side 2 pi, c=10, Re=1600, CFL 0.3).
+"""Benchmark of the MI355X CABARET/ARK step: Mcell-steps/s at 256^3.
 
-One "step" is one iteration of the reference's driver loop
-(source/main.cpp:82-91) over the whole box, inputs resident in HBM.
-Prints ONE JSON line.  Extra objects:
-  roofline      the dominant kernel's algorithmic HBM bytes per launch over its
-                average launch time (HIP events on the library's own stream)
-  cpu_baseline  the unmodified reference (oracle/_ref, built from the
-                reference's sources) under mpirun on this host's cores, on a
-                bounded sample of the same workload
+Workload: the Taylor-Green vortex on a periodic box of side 2 pi, N=256 cells
+per side, c=10 (Ma=0.1), Re=1600, CFL 0.3, 500 steps, inputs resident in HBM.
+One "step" is one iteration of the reference's driver loop (source/main.cpp:
+82-91) over the whole box.  The headline value is the mixed-precision step;
+the FP64 step (bitwise equal to the reference) is timed beside it.
+
+Printed: ONE JSON line.  Beyond the common fields it carries
+  fp64          the FP64 step's throughput and roofline
+  roofline      the mixed step's fused kernel: algorithmic HBM bytes per launch
+                (200 B per cell: 5 FP64 cell fields and 15 FP32 face fields,
+                each read once and written once) over its average launch time,
+                timed with HIP events on the library's stream
+  accuracy      mixed vs FP64 after the same steps, per field (corrected
+                scheme: the reference scheme goes unstable on this vortex)
+  cpu_baseline  the unmodified reference (oracle/_ref, built from its own
+                sources) under mpirun on this host's cores, bounded sample
+  --calibrate-copy prints the HBM bytes a plain device copy of known size
+  reports to rocprofv3's FETCH_SIZE/WRITE_SIZE counters (used to scale them).
 """
 import argparse
+import hashlib
 import json
 import os
 import shutil
@@ -23,33 +33,17 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "ark-cpp-mpi_amd"))
 
 HBM_PEAK_GBS = 8000.0
-
-# Algorithmic bytes per unit of work for each kernel: every array element
-# the kernel must read or write, counted once.  Units are cells, or faces of
-# one orientation for the Phase2 sweeps.
-def kernel_bytes(prec):
-    d = 8  # FP64 storage
-    return {
-        # fused predictor: reads 5 level-n cell fields + 15 face fields (u1,u2,u3,p,t
-        # per axis), writes 5 half-level fields and 3 forces
-        "fused_predictor": (5 + 15 + 5 + 3) * d,
-        # fused corrector: reads 5 level-n + 5 half-level cell fields, 15 old faces,
-        # 3 forces; writes 15 new faces and 5 level-(n+1) cell fields
-        "fused_corrector": (5 + 5 + 15 + 3 + 15 + 5) * d,
-        "phase1": (5 + 18 + 5 + 5) * d,
-        "stress_faces": (3 + 9) * d,
-        "forces": (9 + 1 + 3) * d,
-        "use_forces": (4 + 3 + 3) * d,
-        "phase2_x": (10 + 5 + 6) * d,
-        "phase2_y": (10 + 5 + 6) * d,
-        "phase2_z": (10 + 5 + 6) * d,
-        "cfl": 3 * d,
-    }
+# Algorithmic HBM bytes per cell per step (each persistent array read once and
+# written once): FP64 = (5 cell + 15 face fields) x 8 B x 2; mixed = 5 cell
+# fields x 8 B x 2 + 15 face fields x 4 B x 2.
+BYTES_PER_CELL = {"fp64": 320, "mixed": 200}
+FUSED_KERNEL = {"fp64": "fused_step", "mixed": "fused_mixed"}
 
 
-def run_gpu(N, steps, warmup, precision, profile_steps):
+def time_steps(N, steps, warmup, precision, scheme, profile_steps):
     import arkamd
-    s = arkamd.Solver(N, N, N, precision=precision)
+    prec = arkamd.PREC_MIXED if precision == "mixed" else arkamd.PREC_FP64
+    s = arkamd.Solver(N, N, N, precision=prec, scheme=scheme)
     s.init_tgv(1.0)
     s.step(warmup)
     s.synchronize()
@@ -62,12 +56,58 @@ def run_gpu(N, steps, warmup, precision, profile_steps):
     if profile_steps > 0:
         s.profile(True)
         s.profile_reset()
-        s.step(profile_steps)
+        s.advance(profile_steps)
         prof = s.profile_read()
         s.profile(False)
     dev_bytes = s.device_bytes()
     s.close()
     return elapsed, diag, prof, dev_bytes
+
+
+def roofline_of(prof, precision, cells):
+    name = FUSED_KERNEL[precision]
+    if name not in prof:
+        return None
+    launches, total_ms = prof[name]
+    avg_s = total_ms / launches / 1e3
+    algo = BYTES_PER_CELL[precision] * cells
+    achieved = algo / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_of(name),
+            "kernel": name, "algorithmic_bytes_per_launch": algo, "avg_launch_ms": round(avg_s * 1e3, 4),
+            "launches_timed": launches}
+
+
+def traffic_of(kernel):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
+    (profiles/pmc_traffic.json: per-dispatch KiB scaled by what the counters
+    read for a 1 GiB device copy), or None when no record exists or the
+    kernel's source has changed since it was taken."""
+    try:
+        rec = json.load(open(os.path.join(REPO, "profiles", "pmc_traffic.json")))["kernels"][kernel]
+        src = open(os.path.join(REPO, rec["source"]), "rb").read()
+        if hashlib.sha256(src).hexdigest() != rec["source_sha256"]:
+            return None
+        return rec["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def accuracy(N, steps):
+    import numpy as np
+    import arkamd
+    out = {}
+    for prec in (arkamd.PREC_FP64, arkamd.PREC_MIXED):
+        with arkamd.Solver(N, N, N, precision=prec, scheme=arkamd.SCHEME_CORRECTED) as s:
+            s.init_tgv(1.0)
+            s.step(steps)
+            out[prec] = {n: s.download(n)[1:-1, 1:-1, 1:-1] for n in ("ronCon", "u1nCon", "u2nCon", "u3nCon")}
+    res = {"scheme": "corrected", "steps": steps}
+    for n, a in out[arkamd.PREC_MIXED].items():
+        b = out[arkamd.PREC_FP64][n]
+        res[n] = {"rel_l2": float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())),
+                  "rel_linf": float(np.abs(a - b).max() / np.abs(b).max())}
+    return res
 
 
 def cpu_baseline(N, steps, ranks):
@@ -82,77 +122,97 @@ def cpu_baseline(N, steps, ranks):
         return {"error": str(e)[:200]}
     rec = json.loads([l for l in out.splitlines() if l.startswith("{")][-1])
     return {
-        "value": rec["mcell_steps_per_s"], "unit": "Mcell-steps/s", "cores": ranks, "kind": "reference",
+        "value": round(rec["mcell_steps_per_s"], 3), "unit": "Mcell-steps/s", "cores": ranks, "kind": "reference",
         "sample": f"TGV {N}^3 x {steps} steps, FP64, mpirun -n {ranks} (dims {rec['dims']}), "
                   f"step loop only, max over ranks ({rec['loop_seconds']:.2f} s)",
     }
 
 
+def calibrate_copy(nbytes):
+    """Three plain device copies of nbytes (torch's elementwise copy kernel),
+    whose FETCH_SIZE/WRITE_SIZE readings scale the counters to bytes."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--accuracy-steps", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-ranks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="time one precision only (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs one subdomain on one GPU; multi-GPU runs are not part of it "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import arkamd
-    prec = arkamd.PREC_MIXED if args.precision == "mixed" else arkamd.PREC_FP64
     N = args.size
     cells = N ** 3
-    elapsed, diag, prof, dev_bytes = run_gpu(N, args.steps, args.warmup, prec, args.profile_steps)
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = cells * args.steps / elapsed / 1e6
+    runs = {}
+    for prec in ("mixed", "fp64"):
+        if args.only and prec != args.only:
+            continue
+        elapsed, diag, prof, dev_bytes = time_steps(N, args.steps, args.warmup, prec, arkamd.SCHEME_REFERENCE,
+                                                    args.profile_steps)
+        runs[prec] = {"elapsed": elapsed, "diag": diag, "prof": prof, "device_bytes": dev_bytes,
+                      "value": cells * args.steps / elapsed / 1e6, "ms_per_step": elapsed * 1e3 / args.steps,
+                      "roofline": roofline_of(prof, prec, cells)}
 
-    roofline = None
-    kernels = {}
-    if prof:
-        kb = kernel_bytes(prec)
-        for name, (launches, total_ms) in prof.items():
-            kernels[name] = {"launches": launches, "avg_ms": total_ms / launches}
-        dom = max(prof.items(), key=lambda kv: kv[1][1])
-        name, (launches, total_ms) = dom
-        avg_s = total_ms / launches / 1e3
-        if name in kb:
-            unit_count = cells
-            algo = kb[name] * unit_count
-            achieved = algo / avg_s / 1e9
-            roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
-                        "algorithmic_bytes_per_launch": algo, "avg_launch_ms": round(avg_s * 1e3, 4)}
-
+    head = "mixed" if "mixed" in runs else "fp64"
+    H = runs[head]
+    line = {
+        "metric": "Mcell-steps/s at 256^3 (FP64 vs mixed)",
+        "value": round(H["value"], 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(H["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 (mixed)" if head == "mixed" else "f64",
+        "data": "synthetic (Taylor-Green vortex initial condition, generated on device)",
+        "config": {"workload": f"Taylor-Green vortex {N}^3 periodic, Re=1600, c=10, CFL 0.3, "
+                               f"{args.steps} steps, {head} precision, reference scheme",
+                   "grid": [N, N, N], "precision": head},
+        "roofline": H["roofline"],
+    }
+    if "fp64" in runs and head != "fp64":
+        F = runs["fp64"]
+        line["fp64"] = {"value": round(F["value"], 2), "ms_per_step": round(F["ms_per_step"], 4),
+                        "roofline": F["roofline"], "dtype": "f64",
+                        "note": "bitwise equal to the reference step"}
+        line["speedup_mixed_vs_fp64"] = round(H["value"] / F["value"], 3)
+    line["kernels"] = {p: {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4)} for k, v in r["prof"].items()}
+                       for p, r in runs.items()}
+    line["final"] = {p: {"dt": r["diag"]["dt"], "nstep": r["diag"]["nstep"], "nonfinite": r["diag"]["nonfinite"]}
+                     for p, r in runs.items()}
+    line["device_bytes"] = {p: r["device_bytes"] for p, r in runs.items()}
+    if args.accuracy_steps > 0 and not args.only:
+        line["accuracy"] = accuracy(N, args.accuracy_steps)
     base = None
     if not args.no_cpu_baseline:
         ranks = args.cpu_ranks or min(16, os.cpu_count() or 1)
         while ranks > 1 and (N % ranks):
             ranks -= 1
         base = cpu_baseline(N, args.cpu_steps, ranks)
-
-    line = {
-        "metric": "Mcell-steps/s at 256^3 (FP64 vs mixed)",
-        "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == arkamd.PREC_FP64 else "f32+f64",
-        "data": "synthetic (Taylor-Green vortex initial condition, generated on device)",
-        "config": {"workload": f"TGV {N}^3 periodic, Re=1600, c=10, CFL 0.3, {args.precision}",
-                   "grid": [N, N, N], "precision": args.precision},
-        "roofline": roofline,
-        "cpu_baseline": base,
-        "kernels": kernels,
-        "final": {"dt": diag["dt"], "nstep": diag["nstep"], "ke_physical": diag["ke_physical"],
-                  "nonfinite": diag["nonfinite"]},
-        "device_bytes": dev_bytes,
-    }
+    line["cpu_baseline"] = base
     print(json.dumps(line))
 
 
