@@ -30,9 +30,12 @@
 
 namespace arkd {
 
-constexpr int TX = 32, TY = 8;
+// 62x6 owned columns plus the ring: 64x8 = 512 threads, eight full
+// wavefronts, two per SIMD (256 VGPRs).  A multiple of four wavefronts keeps
+// the four SIMDs of a CU equally loaded.
+constexpr int TX = 62, TY = 6;
 constexpr int RX = TX + 2, RY = TY + 2;
-constexpr int NTHR = RX * RY;   // 340 threads: 6 wavefronts
+constexpr int NTHR = RX * RY;
 
 // Field groups: five fields `fb` bytes apart, element (0,0,0) of the first
 // at the group pointer.  A field is read as group + v*fb + 8*idx with the
@@ -189,7 +192,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
     const double dt = a.S->dt;
     const int tid = threadIdx.x;
     const int tx = tid % RX, ty = tid / RX;
-    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1, k0 = blockIdx.z * a.LZ + 1;
+    const TileId tile = die_major_tile((g.N1 + TX - 1) / TX, (g.N2 + TY - 1) / TY, (g.N3 + a.LZ - 1) / a.LZ);
+    const int i0 = tile.x * TX + 1, j0 = tile.y * TY + 1, k0 = tile.z * a.LZ + 1;
     const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = min(k0 + a.LZ, g.N3 + 1);
     // logical cell of this thread (clamped into the region so every access stays in bounds)
     const int i = min(i0 - 1 + tx, iend), j = min(j0 - 1 + ty, jend);
@@ -424,7 +428,7 @@ hipError_t run_fused(void* p) {
     a.S = c->S;
     a.dt_next = &c->S->dt_bits;
     a.LZ = fl->LZ;
-    dim3 grid((c->geo.N1 + TX - 1) / TX, (c->geo.N2 + TY - 1) / TY, (c->geo.N3 + fl->LZ - 1) / fl->LZ);
+    dim3 grid(((c->geo.N1 + TX - 1) / TX) * ((c->geo.N2 + TY - 1) / TY) * ((c->geo.N3 + fl->LZ - 1) / fl->LZ));
     if (c->prm.scheme == ARK_SCHEME_REFERENCE)
         hipLaunchKernelGGL(k_fused_step<true>, grid, dim3(NTHR), 0, c->stream, a);
     else

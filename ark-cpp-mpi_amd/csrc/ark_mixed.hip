@@ -22,14 +22,30 @@
 // predictor's increments.
 //
 // Blocking and the order of work per plane are those of the FP64 fused
-// kernel (ark_fused.hip): 32x8 owned columns plus a one-cell ring, z-march.
+// kernel (ark_fused.hip): owned columns plus a one-cell ring, z-march.
 #include "ark_internal.hpp"
 #include "ark_kernels.hpp"
+
+// This file is not bound to the reference's rounding (its results are
+// checked against the FP64 path with a tolerance), so a*b+c may become one
+// fused multiply-add and divisions by cell quantities use the hardware
+// reciprocal (1 ulp in FP32; refined to full FP64 precision by two Newton
+// steps in the closing update).
+#pragma clang fp contract(fast)
 
 namespace arkd {
 namespace mixed {
 
-constexpr int TX = 32, TY = 8;
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpd(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return fma(y, fma(-x, y, 1.0), y);
+}
+
+// 30x6 owned columns plus the ring: 32x8 = 256 threads, four full
+// wavefronts, so three workgroups fill a CU evenly at three waves per SIMD.
+constexpr int TX = 30, TY = 6;
 constexpr int RX = TX + 2, RY = TY + 2;
 constexpr int NTHR = RX * RY;
 
@@ -234,8 +250,9 @@ struct LdsCell { float h[4]; float fx[4]; float fy[4]; };   // h: u1, u2, u3, t'
 struct LdsFlux { float mx[5], my[5]; };
 union LdsSlot { LdsCell c; LdsFlux f; };
 
+// At most 168 VGPRs: three waves per SIMD.
 template <bool Quirk>
-__global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) k_fused_mixed(Args a) {
     __shared__ LdsSlot lds[NTHR];
     const Geo& g = a.g;
     const double dtd = a.S->dt;
@@ -244,7 +261,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     const float cf = (float)cd;
     const int tid = threadIdx.x;
     const int tx = tid % RX, ty = tid / RX;
-    const int i0 = blockIdx.x * TX + 1, j0 = blockIdx.y * TY + 1, k0 = blockIdx.z * a.LZ + 1;
+    const TileId tile = die_major_tile((g.N1 + TX - 1) / TX, (g.N2 + TY - 1) / TY, (g.N3 + a.LZ - 1) / a.LZ);
+    const int i0 = tile.x * TX + 1, j0 = tile.y * TY + 1, k0 = tile.z * a.LZ + 1;
     const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = min(k0 + a.LZ, g.N3 + 1);
     const int i = min(i0 - 1 + tx, iend), j = min(j0 - 1 + ty, jend);
     const bool in_region = (i0 - 1 + tx <= iend) && (j0 - 1 + ty <= jend);
@@ -267,11 +285,11 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
         for (int m = 0; m < 3; ++m) sz_lo[m] = A.VIS * (float)(Acur.u[m] - Ab.u[m]) * A.inv_dx[2];
     }
     // carried from plane p-1 to plane p
-    CellD Aprev{};
-    float Sp_prev[5] = {0, 0, 0, 0, 0};     // predictor flux sums of plane p-1
-    float f_prev[3] = {0, 0, 0};
-    float Sxy_prev[5] = {0, 0, 0, 0, 0};    // corrector in-plane flux sums of plane p-1
-    Flux mz_prev{};
+    // Conservative variables of plane p-1 (rho, rho u, rho t') after every
+    // increment except the flux through its top z-face: level n minus c times
+    // predictor sums, corrector in-plane sums, bottom z-face flux and
+    // (momentum) twice the force, in FP64.
+    double cons_prev[5] = {0, 0, 0, 0, 0};
     CellF Hprev{};
     FwdF fz_prev{};
     unsigned long long dtbits = ~0ull;
@@ -322,7 +340,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
             {
                 const float roA = A.ro0 + Af.dr;
                 H.dr = Af.dr - cf * Sp[0];
-                const float inv_roH = 1.f / (A.ro0 + H.dr);
+                const float inv_roH = rcpf(A.ro0 + H.dr);
                 for (int m = 0; m < 3; ++m) H.u[m] = (roA * Af.u[m] - cf * (Sp[1 + m] - f[m])) * inv_roH;
                 H.tp = (roA * Af.tp - cf * Sp[4]) * inv_roH;
             }
@@ -368,9 +386,9 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
             oz[0] = Pzb.tp; oz[1] = Pzb.u[0]; oz[2] = Pzb.u[1];
         }
         if (owned && p >= k0 && p < kend) {
-            const float d1 = A.cfl_dx[0] / (A.c + fabsf(H.u[0]));
-            const float d2 = A.cfl_dx[1] / (A.c + fabsf(H.u[1]));
-            const float d3 = A.cfl_dx[2] / (A.c + fabsf(H.u[2]));
+            const float d1 = A.cfl_dx[0] * rcpf(A.c + fabsf(H.u[0]));
+            const float d2 = A.cfl_dx[1] * rcpf(A.c + fabsf(H.u[1]));
+            const float d3 = A.cfl_dx[2] * rcpf(A.c + fabsf(H.u[2]));
             const double dmin = (double)fminf(fminf(d1, d2), d3);
             const unsigned long long b = (unsigned long long)__double_as_longlong(dmin);
             dtbits = b < dtbits ? b : dtbits;
@@ -386,18 +404,15 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
                 if (p < kend || kend == g.N3 + 1) store_face<2>(A, at(g, i, j, p), o);
                 if (p > k0) {
                     double Sc[5];
-                    for (int v = 0; v < 5; ++v)
-                        Sc[v] = (double)Sp_prev[v] + (double)(Sxy_prev[v] + (mz.m[v] - mz_prev.m[v]) * A.ds[2]);
-                    const double ro1 = Aprev.ro - cd * Sc[0];
-                    const double inv = 1.0 / ro1;
+                    for (int v = 0; v < 5; ++v) Sc[v] = cons_prev[v] - cd * (double)(mz.m[v] * A.ds[2]);
+                    const double ro1 = Sc[0];
+                    const double inv = rcpd(ro1);
                     const int co = at(g, i, j, p - 1);
                     std_(A.Ao, A.fb, 0, co, ro1);
                     for (int m = 0; m < 3; ++m) {
-                        const double mom = Aprev.ro * Aprev.u[m] - cd * (Sc[1 + m] - 2.0 * (double)f_prev[m]);
-                        std_(A.Ao, A.fb, 1 + m, co, mom * inv);
+                        std_(A.Ao, A.fb, 1 + m, co, Sc[1 + m] * inv);
                     }
-                    const double rt = Aprev.ro * (Aprev.t - A.t0d) - cd * Sc[4];
-                    std_(A.Ao, A.fb, 4, co, A.t0d + rt * inv);
+                    std_(A.Ao, A.fb, 4, co, A.t0d + Sc[4] * inv);
                 }
             }
         }
@@ -440,15 +455,14 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
             __syncthreads();
         }
 
-        Aprev = Acur;
         Hprev = H;
         fz_prev = fz;
-        for (int m = 0; m < 3; ++m) f_prev[m] = f[m];
         for (int v = 0; v < 5; ++v) {
-            Sp_prev[v] = Sp[v];
-            Sxy_prev[v] = Sxy[v];
+            double inc = (double)Sp[v] + (double)(Sxy[v] - mz.m[v] * A.ds[2]);
+            if (v >= 1 && v <= 3) inc -= 2.0 * (double)f[v - 1];
+            const double q = v == 0 ? Acur.ro : Acur.ro * (v == 4 ? Acur.t - A.t0d : Acur.u[v - 1]);
+            cons_prev[v] = q - cd * inc;
         }
-        mz_prev = mz;
         Acur = Anext;
     }
     block_min_atomic(dtbits, A.dt_next);
@@ -517,8 +531,8 @@ hipError_t run_mixed(void* p) {
     a.S = c->S;
     a.dt_next = &c->S->dt_bits;
     a.LZ = ml->LZ;
-    dim3 grid((c->geo.N1 + mixed::TX - 1) / mixed::TX, (c->geo.N2 + mixed::TY - 1) / mixed::TY,
-              (c->geo.N3 + ml->LZ - 1) / ml->LZ);
+    dim3 grid(((c->geo.N1 + mixed::TX - 1) / mixed::TX) * ((c->geo.N2 + mixed::TY - 1) / mixed::TY) *
+              ((c->geo.N3 + ml->LZ - 1) / ml->LZ));
     if (c->prm.scheme == ARK_SCHEME_REFERENCE)
         hipLaunchKernelGGL(mixed::k_fused_mixed<true>, grid, dim3(mixed::NTHR), 0, c->stream, a);
     else

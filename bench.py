@@ -38,6 +38,20 @@ HBM_PEAK_GBS = 8000.0
 # fields x 8 B x 2 + 15 face fields x 4 B x 2.
 BYTES_PER_CELL = {"fp64": 320, "mixed": 200}
 FUSED_KERNEL = {"fp64": "fused_step", "mixed": "fused_mixed"}
+# Algorithmic HBM bytes per interior cell of the phase-by-phase FP64 kernels
+# (every array element each kernel reads or writes, once): cfl reads u1..3;
+# phase1 reads 5 level-n cell fields and 6 face fields per axis (u1,u2,u3,
+# p, rho, t) and writes Con and nCon; stress_faces reads u1..3 and the
+# condition and writes 9 stresses; forces reads condition, rho and 9 stresses
+# and writes f1..3; use_forces reads condition, rho, u1..3, f1..3 and writes
+# u1..3; phase2_d reads 5 faces and 10 cell fields and writes 6 faces;
+# rho_faces reads and writes 3 face fields.
+PHASE_BYTES_PER_CELL = {"cfl": 24, "phase1": 264, "stress_faces": 104, "forces": 112, "use_forces": 88,
+                        "phase2_x": 168, "phase2_y": 168, "phase2_z": 168, "rho_faces": 48}
+# Ghost fills move planes: 3 cell fields x 3 axes x 2 sides before the stress;
+# 7 cell fields and 5 face fields x 3 axes x 2 sides before Phase2 (read and
+# write, 16 B per element).
+GHOST_PLANES = {"ghost_stress": 3 * 3 * 2, "ghost_phase2": (7 + 5) * 3 * 2}
 
 
 def time_steps(N, steps, warmup, precision, scheme, profile_steps):
@@ -93,6 +107,35 @@ def traffic_of(kernel):
         return None
 
 
+def phase_kernels(N, steps):
+    """Achieved GB/s of every kernel of the phase-by-phase FP64 step."""
+    import arkamd
+    with arkamd.Solver(N, N, N) as s:
+        s.init_tgv(1.0)
+        s.step_phases(1)
+        s.synchronize()
+        s.profile(True)
+        s.profile_reset()
+        s.step_phases(steps)
+        prof = s.profile_read()
+    out = {}
+    cells = N ** 3
+    for name, (launches, total_ms) in sorted(prof.items()):
+        avg_s = total_ms / launches / 1e3
+        if name in PHASE_BYTES_PER_CELL:
+            algo = PHASE_BYTES_PER_CELL[name] * cells
+        elif name in GHOST_PLANES:
+            algo = GHOST_PLANES[name] * N * N * 16
+        else:
+            algo = None
+        rec = {"launches": launches, "avg_ms": round(avg_s * 1e3, 4)}
+        if algo:
+            rec.update({"bytes": algo, "GB/s": round(algo / avg_s / 1e9, 1),
+                        "frac": round(algo / avg_s / 1e9 / HBM_PEAK_GBS, 4)})
+        out[name] = rec
+    return out
+
+
 def accuracy(N, steps):
     import numpy as np
     import arkamd
@@ -103,10 +146,16 @@ def accuracy(N, steps):
             s.step(steps)
             out[prec] = {n: s.download(n)[1:-1, 1:-1, 1:-1] for n in ("ronCon", "u1nCon", "u2nCon", "u3nCon")}
     res = {"scheme": "corrected", "steps": steps}
-    for n, a in out[arkamd.PREC_MIXED].items():
-        b = out[arkamd.PREC_FP64][n]
+    mix, ref = out[arkamd.PREC_MIXED], out[arkamd.PREC_FP64]
+    for n, a in mix.items():
+        b = ref[n]
         res[n] = {"rel_l2": float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum())),
                   "rel_linf": float(np.abs(a - b).max() / np.abs(b).max())}
+    # the velocity vector as one field: |u_mixed - u_fp64|_2 / |u_fp64|_2
+    vel = ("u1nCon", "u2nCon", "u3nCon")
+    num = sum(((mix[n] - ref[n]) ** 2).sum() for n in vel)
+    den = sum((ref[n] ** 2).sum() for n in vel)
+    res["velocity"] = {"rel_l2": float(np.sqrt(num / den))}
     return res
 
 
@@ -150,6 +199,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--accuracy-steps", type=int, default=500)
+    ap.add_argument("--phase-steps", type=int, default=3,
+                    help="steps of the phase-by-phase FP64 path timed per kernel (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-ranks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -204,6 +255,8 @@ def main():
     line["final"] = {p: {"dt": r["diag"]["dt"], "nstep": r["diag"]["nstep"], "nonfinite": r["diag"]["nonfinite"]}
                      for p, r in runs.items()}
     line["device_bytes"] = {p: r["device_bytes"] for p, r in runs.items()}
+    if args.phase_steps > 0 and not args.only:
+        line["phase_kernels_fp64"] = phase_kernels(N, args.phase_steps)
     if args.accuracy_steps > 0 and not args.only:
         line["accuracy"] = accuracy(N, args.accuracy_steps)
     base = None
