@@ -103,15 +103,16 @@ __device__ __forceinline__ CellF to_f(const CellD& c, const Args& a) {
     f.tp = (float)(c.t - a.t0d);
     return f;
 }
-__device__ __forceinline__ FaceF load_face(const Args& a, int d, int idx) {
+__device__ __forceinline__ FaceF load_face_g(const char* grp, unsigned long long fb, int idx) {
     FaceF f;
-    f.u[0] = ldf(a.F[d], a.fb, 0, idx);
-    f.u[1] = ldf(a.F[d], a.fb, 1, idx);
-    f.u[2] = ldf(a.F[d], a.fb, 2, idx);
-    f.p = ldf(a.F[d], a.fb, 3, idx);
-    f.tp = ldf(a.F[d], a.fb, 4, idx);
+    f.u[0] = ldf(grp, fb, 0, idx);
+    f.u[1] = ldf(grp, fb, 1, idx);
+    f.u[2] = ldf(grp, fb, 2, idx);
+    f.p = ldf(grp, fb, 3, idx);
+    f.tp = ldf(grp, fb, 4, idx);
     return f;
 }
+__device__ __forceinline__ FaceF load_face(const Args& a, int d, int idx) { return load_face_g(a.F[d], a.fb, idx); }
 
 struct Flux { float m[5]; };
 
@@ -246,14 +247,23 @@ __device__ __forceinline__ Side make_side(const CellF& ac, const CellF& h, const
     return s;
 }
 
-struct LdsCell { float h[4]; float fx[4]; float fy[4]; };   // h: u1, u2, u3, t'
-struct LdsFlux { float mx[5], my[5]; };
-union LdsSlot { LdsCell c; LdsFlux f; };
+// One lane's inputs for one plane: its own x-, y- and top z-face (the
+// neighbours' faces come through LDS), the face beyond the region for lanes
+// on its last column or row, and the velocity of the stress neighbour outside
+// the region for lanes on its edge.
+struct PlaneIn {
+    FaceF xw, ys, zt, ex;
+    double ou[3];
+};
 
-// At most 168 VGPRs: three waves per SIMD.
+// At most 256 VGPRs: two waves per SIMD, two four-wave workgroups per CU.
 template <bool Quirk>
-__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) k_fused_mixed(Args a) {
-    __shared__ LdsSlot lds[NTHR];
+__global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
+    __shared__ float4 s_h[NTHR];             // half level: u1, u2, u3, t'
+    __shared__ float4 s_fx[NTHR], s_fy[NTHR];  // forward extrapolations along x, y
+    __shared__ float s_m[10][NTHR];          // new x- and y-face fluxes
+    __shared__ double s_u[2][3][NTHR];       // level-n velocities of a plane
+    __shared__ float s_f[2][10][NTHR];       // x- and y-face of each lane's cell
     const Geo& g = a.g;
     const double dtd = a.S->dt;
     const float dt = (float)dtd, inv_dt2 = (float)(2.0 / dtd);
@@ -268,27 +278,73 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
     const bool in_region = (i0 - 1 + tx <= iend) && (j0 - 1 + ty <= jend);
     const bool own_x = i >= i0 && i < iend, own_y = j >= j0 && j < jend;
     const bool owned = in_region && own_x && own_y;
-    const bool xface = in_region && own_y && i >= i0 && i <= iend;
-    const bool yface = in_region && own_x && j >= j0 && j <= jend;
-    const bool xface_store = xface && (i < iend || iend == g.N1 + 1);
-    const bool yface_store = yface && (j < jend || jend == g.N2 + 1);
+    const bool xface = in_region && own_y && i >= i0;
+    const bool yface = in_region && own_x && j >= j0;
+    const bool xfirst = tx == 0, yfirst = ty == 0;
+    const bool xlast = i == iend, ylast = j == jend;
     const int iw = wrapc(i, g.N1), jw = wrapc(j, g.N2);
-    const int iwm = wrapc(i - 1, g.N1), iwp = wrapc(i + 1, g.N1);
-    const int jwm = wrapc(j - 1, g.N2), jwp = wrapc(j + 1, g.N2);
     const int px = g.px, sxy = g.sxy;
+    // LDS neighbours, clamped so every lane reads inside the arrays
+    const int tW = xfirst ? tid : tid - 1, tE = xlast ? tid : tid + 1;
+    const int tS = yfirst ? tid : tid - RX, tN = ylast ? tid : tid + RX;
+    // cell and face offsets of this lane within a plane
+    const int base = iw + jw * px;
+    const int ex_off = xlast ? wrapc(i + 1, g.N1) + jw * px : iw + wrapc(j + 1, g.N2) * px;
+    const bool has_ex = xlast || ylast;
+    int o_off = base;
+    if (xfirst) o_off = wrapc(i - 1, g.N1) + jw * px;
+    else if (xlast) o_off = wrapc(i + 1, g.N1) + jw * px;
+    else if (yfirst) o_off = iw + wrapc(j - 1, g.N2) * px;
+    else if (ylast) o_off = iw + wrapc(j + 1, g.N2) * px;
+    const bool has_o = xfirst || xlast || yfirst || ylast;
     Args A = a;
 
-    CellD Acur = load_cell(A, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
+    auto load_plane = [&](int p) {
+        PlaneIn r;
+        const int zo = wrapc(p, g.N3) * sxy;
+        r.xw = load_face(A, 0, base + zo);
+        r.ys = load_face(A, 1, base + zo);
+        r.zt = load_face(A, 2, base + wrapc(p + 1, g.N3) * sxy);
+        r.ex = FaceF{{0.f, 0.f, 0.f}, 0.f, 0.f};
+        if (has_ex) r.ex = load_face_g(xlast ? A.F[0] : A.F[1], A.fb, ex_off + zo);
+        r.ou[0] = r.ou[1] = r.ou[2] = 0.0;
+        if (has_o)
+            for (int m = 0; m < 3; ++m) r.ou[m] = ldd(A.A, A.fb, 1 + m, o_off + zo);
+        return r;
+    };
+    auto stage = [&](int p, const PlaneIn& r) {
+        float* f = &s_f[p & 1][0][0];
+        f[0 * NTHR + tid] = r.xw.u[0]; f[1 * NTHR + tid] = r.xw.u[1]; f[2 * NTHR + tid] = r.xw.u[2];
+        f[3 * NTHR + tid] = r.xw.p; f[4 * NTHR + tid] = r.xw.tp;
+        f[5 * NTHR + tid] = r.ys.u[0]; f[6 * NTHR + tid] = r.ys.u[1]; f[7 * NTHR + tid] = r.ys.u[2];
+        f[8 * NTHR + tid] = r.ys.p; f[9 * NTHR + tid] = r.ys.tp;
+    };
+    auto staged_face = [&](int p, int q, int off) {
+        const float* f = &s_f[p & 1][off][0];
+        FaceF r;
+        r.u[0] = f[0 * NTHR + q]; r.u[1] = f[1 * NTHR + q]; r.u[2] = f[2 * NTHR + q];
+        r.p = f[3 * NTHR + q]; r.tp = f[4 * NTHR + q];
+        return r;
+    };
+
+    // prologue: plane k0-1 (the ring plane below the chunk)
+    CellD Acur = load_cell(A, base + wrapc(k0 - 1, g.N3) * sxy);
+    CellD Anext = load_cell(A, base + wrapc(k0, g.N3) * sxy);
     float sz_lo[3];
     {
-        const CellD Ab = load_cell(A, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
+        const CellD Ab = load_cell(A, base + wrapc(k0 - 2, g.N3) * sxy);
         for (int m = 0; m < 3; ++m) sz_lo[m] = A.VIS * (float)(Acur.u[m] - Ab.u[m]) * A.inv_dx[2];
     }
+    PlaneIn cur = load_plane(k0 - 1);
+    FaceF zb = load_face(A, 2, base + wrapc(k0 - 1, g.N3) * sxy);
+    for (int m = 0; m < 3; ++m) s_u[(k0 - 1) & 1][m][tid] = Acur.u[m];
+    stage(k0 - 1, cur);
+    // Retire every prologue load here, so that the loop's waits only ever
+    // cover the plane being consumed, never the plane being prefetched.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
     // carried from plane p-1 to plane p
-    // Conservative variables of plane p-1 (rho, rho u, rho t') after every
-    // increment except the flux through its top z-face: level n minus c times
-    // predictor sums, corrector in-plane sums, bottom z-face flux and
-    // (momentum) twice the force, in FP64.
     double cons_prev[5] = {0, 0, 0, 0, 0};
     CellF Hprev{};
     FwdF fz_prev{};
@@ -296,23 +352,32 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
 
     for (int p = k0 - 1; p <= kend; ++p) {
         asm volatile("" : "+s"(A.fb));
-        const int pw = wrapc(p, g.N3);
-        const int c = at(g, iw, jw, pw);
-        const CellD Anext = load_cell(A, at(g, iw, jw, wrapc(p + 1, g.N3)));
+        const bool inner = p >= k0 && p < kend;
+        // prefetch plane p+1 (and the cell two planes up, for its z-stress)
+        PlaneIn nxt = cur;
+        CellD Anext2 = Anext;
+        if (p < kend) {
+            for (int m = 0; m < 3; ++m) s_u[(p + 1) & 1][m][tid] = Anext.u[m];
+            nxt = load_plane(p + 1);
+            Anext2 = load_cell(A, base + wrapc(p + 2, g.N3) * sxy);
+        }
         const CellF Af = to_f(Acur, A);
 
         // stress force (FP32 differences of FP64 velocities)
         float f[3];
         {
-            const int cw = at(g, iwm, jw, pw), ce = at(g, iwp, jw, pw);
-            const int cs = at(g, iw, jwm, pw), cn = at(g, iw, jwp, pw);
             const float roc = (float)Acur.ro;
             for (int m = 0; m < 3; ++m) {
+                const double* u = &s_u[p & 1][m][0];
                 const double uc = Acur.u[m];
-                const float sxl = A.VIS * (float)(uc - ldd(A.A, A.fb, 1 + m, cw)) * A.inv_dx[0];
-                const float sxh = A.VIS * (float)(ldd(A.A, A.fb, 1 + m, ce) - uc) * A.inv_dx[0];
-                const float syl = A.VIS * (float)(uc - ldd(A.A, A.fb, 1 + m, cs)) * A.inv_dx[1];
-                const float syh = A.VIS * (float)(ldd(A.A, A.fb, 1 + m, cn) - uc) * A.inv_dx[1];
+                const double uW = xfirst ? cur.ou[m] : u[tW];
+                const double uE = xlast ? cur.ou[m] : u[tE];
+                const double uS = yfirst ? cur.ou[m] : u[tS];
+                const double uN = ylast ? cur.ou[m] : u[tN];
+                const float sxl = A.VIS * (float)(uc - uW) * A.inv_dx[0];
+                const float sxh = A.VIS * (float)(uE - uc) * A.inv_dx[0];
+                const float syl = A.VIS * (float)(uc - uS) * A.inv_dx[1];
+                const float syh = A.VIS * (float)(uN - uc) * A.inv_dx[1];
                 const float szh = A.VIS * (float)(Anext.u[m] - uc) * A.inv_dx[2];
                 f[m] = ((sxh - sxl) * A.ds[0] + (syh - syl) * A.ds[1] + (szh - sz_lo[m]) * A.ds[2]) * roc;
                 sz_lo[m] = szh;
@@ -326,13 +391,15 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
         BwdF bx, by, bz;
         float ox[3], oy[3], oz[3];
         {
-            const FaceF Xw = load_face(A, 0, c), Xe = load_face(A, 0, c + 1);
-            const FaceF Ys = load_face(A, 1, c), Yn = load_face(A, 1, c + px);
-            const FaceF Zb = load_face(A, 2, c), Zt = load_face(A, 2, c + sxy);
+            const FaceF& Xw = cur.xw;
+            const FaceF Xe = xlast ? cur.ex : staged_face(p, tE, 0);
+            const FaceF& Ys = cur.ys;
+            const FaceF Yn = ylast ? cur.ex : staged_face(p, tN, 5);
+            const FaceF& Zt = cur.zt;
             {
                 const Flux a0 = flux<0>(Xw.u, Xw.p, Xw.tp, A), a1 = flux<0>(Xe.u, Xe.p, Xe.tp, A);
                 const Flux b0 = flux<1>(Ys.u, Ys.p, Ys.tp, A), b1 = flux<1>(Yn.u, Yn.p, Yn.tp, A);
-                const Flux c0 = flux<2>(Zb.u, Zb.p, Zb.tp, A), c1 = flux<2>(Zt.u, Zt.p, Zt.tp, A);
+                const Flux c0 = flux<2>(zb.u, zb.p, zb.tp, A), c1 = flux<2>(Zt.u, Zt.p, Zt.tp, A);
                 for (int v = 0; v < 5; ++v)
                     Sp[v] = (a1.m[v] - a0.m[v]) * A.ds[0] + (b1.m[v] - b0.m[v]) * A.ds[1] +
                             (c1.m[v] - c0.m[v]) * A.ds[2];
@@ -344,48 +411,26 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
                 for (int m = 0; m < 3; ++m) H.u[m] = (roA * Af.u[m] - cf * (Sp[1 + m] - f[m])) * inv_roH;
                 H.tp = (roA * Af.tp - cf * Sp[4]) * inv_roH;
             }
-            FaceF Pxb = Xw, Pxf = Xe, Pyb = Ys, Pyf = Yn, Pzb = Zb, Pzf = Zt;
-            if (i != iw) {
-                const int cl = at(g, 0, jw, pw);
-                Pxb = load_face(A, 0, cl + mapf(i, g.N1));
-                Pxf = load_face(A, 0, cl + mapf(i + 1, g.N1));
-                Pxb.tp = ldf(A.F[0], A.fb, 4, cl + i);
-                Pxf.tp = ldf(A.F[0], A.fb, 4, cl + i + 1);
-            }
-            if (j != jw) {
-                const int cl = at(g, iw, 0, pw);
-                Pyb = load_face(A, 1, cl + mapf(j, g.N2) * px);
-                Pyf = load_face(A, 1, cl + mapf(j + 1, g.N2) * px);
-                Pyb.tp = ldf(A.F[1], A.fb, 4, cl + j * px);
-                Pyf.tp = ldf(A.F[1], A.fb, 4, cl + (j + 1) * px);
-            }
-            if (p != pw) {
-                const int cl = at(g, iw, jw, 0);
-                Pzb = load_face(A, 2, cl + mapf(p, g.N3) * sxy);
-                Pzf = load_face(A, 2, cl + mapf(p + 1, g.N3) * sxy);
-                Pzb.tp = ldf(A.F[2], A.fb, 4, cl + p * sxy);
-                Pzf.tp = ldf(A.F[2], A.fb, 4, cl + (p + 1) * sxy);
-            }
             {
-                const Side s = make_side<0>(Af, H, Pxb, Pxf, A);
+                const Side s = make_side<0>(Af, H, Xw, Xe, A);
                 fx = forward(s, dt, inv_dt2, A.inv_dx[0], A);
                 bx = backward<Quirk>(s, dt, inv_dt2, A.inv_dx[0], A);
             }
             {
-                const Side s = make_side<1>(Af, H, Pyb, Pyf, A);
+                const Side s = make_side<1>(Af, H, Ys, Yn, A);
                 fy = forward(s, dt, inv_dt2, A.inv_dx[1], A);
                 by = backward<false>(s, dt, inv_dt2, A.inv_dx[1], A);
             }
             {
-                const Side s = make_side<2>(Af, H, Pzb, Pzf, A);
+                const Side s = make_side<2>(Af, H, zb, Zt, A);
                 fz = forward(s, dt, inv_dt2, A.inv_dx[2], A);
                 bz = backward<Quirk>(s, dt, inv_dt2, A.inv_dx[2], A);
             }
-            ox[0] = Pxb.tp; ox[1] = Pxb.u[1]; ox[2] = Pxb.u[2];
-            oy[0] = Pyb.tp; oy[1] = Pyb.u[0]; oy[2] = Pyb.u[2];
-            oz[0] = Pzb.tp; oz[1] = Pzb.u[0]; oz[2] = Pzb.u[1];
+            ox[0] = Xw.tp; ox[1] = Xw.u[1]; ox[2] = Xw.u[2];
+            oy[0] = Ys.tp; oy[1] = Ys.u[0]; oy[2] = Ys.u[2];
+            oz[0] = zb.tp; oz[1] = zb.u[0]; oz[2] = zb.u[1];
         }
-        if (owned && p >= k0 && p < kend) {
+        if (owned && inner) {
             const float d1 = A.cfl_dx[0] * rcpf(A.c + fabsf(H.u[0]));
             const float d2 = A.cfl_dx[1] * rcpf(A.c + fabsf(H.u[1]));
             const float d3 = A.cfl_dx[2] * rcpf(A.c + fabsf(H.u[2]));
@@ -401,7 +446,10 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
                                        Hprev.u[1], H.u[1], oz[0], oz[1], oz[2], A);
             mz = out_flux<2>(o, A);
             if (owned) {
-                if (p < kend || kend == g.N3 + 1) store_face<2>(A, at(g, i, j, p), o);
+                if (p < kend) {
+                    store_face<2>(A, at(g, i, j, p), o);
+                    if (p == 1) store_face<2>(A, at(g, i, j, g.N3 + 1), o);
+                }
                 if (p > k0) {
                     double Sc[5];
                     for (int v = 0; v < 5; ++v) Sc[v] = cons_prev[v] - cd * (double)(mz.m[v] * A.ds[2]);
@@ -409,9 +457,7 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
                     const double inv = rcpd(ro1);
                     const int co = at(g, i, j, p - 1);
                     std_(A.Ao, A.fb, 0, co, ro1);
-                    for (int m = 0; m < 3; ++m) {
-                        std_(A.Ao, A.fb, 1 + m, co, Sc[1 + m] * inv);
-                    }
+                    for (int m = 0; m < 3; ++m) std_(A.Ao, A.fb, 1 + m, co, Sc[1 + m] * inv);
                     std_(A.Ao, A.fb, 4, co, A.t0d + Sc[4] * inv);
                 }
             }
@@ -419,40 +465,44 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
 
         // x/y faces and the in-plane corrector flux sums
         float Sxy[5] = {0, 0, 0, 0, 0};
-        if (p >= k0 && p < kend) {
-            LdsCell& me = lds[tid].c;
-            me.h[0] = H.u[0]; me.h[1] = H.u[1]; me.h[2] = H.u[2]; me.h[3] = H.tp;
-            me.fx[0] = fx.r; me.fx[1] = fx.t; me.fx[2] = fx.a; me.fx[3] = fx.b;
-            me.fy[0] = fy.r; me.fy[1] = fy.t; me.fy[2] = fy.a; me.fy[3] = fy.b;
+        if (p < kend) {
+            if (inner) {
+                s_h[tid] = make_float4(H.u[0], H.u[1], H.u[2], H.tp);
+                s_fx[tid] = make_float4(fx.r, fx.t, fx.a, fx.b);
+                s_fy[tid] = make_float4(fy.r, fy.t, fy.a, fy.b);
+            }
             __syncthreads();
             Flux mx{}, my{};
-            if (xface) {
-                const LdsCell& nb = lds[tid - 1].c;
-                const FwdF fwb{nb.fx[0], nb.fx[1], nb.fx[2], nb.fx[3]};
-                const FaceOutF o = combine(fwb, bx, nb.h[0], H.u[0], nb.h[3], H.tp, nb.h[1], H.u[1], nb.h[2], H.u[2],
-                                           ox[0], ox[1], ox[2], A);
-                if (xface_store) store_face<0>(A, at(g, i, j, p), o);
-                mx = out_flux<0>(o, A);
+            if (inner) {
+                if (xface) {
+                    const float4 nb = s_h[tW], fw = s_fx[tW];
+                    const FwdF fwb{fw.x, fw.y, fw.z, fw.w};
+                    const FaceOutF o = combine(fwb, bx, nb.x, H.u[0], nb.w, H.tp, nb.y, H.u[1], nb.z, H.u[2], ox[0],
+                                               ox[1], ox[2], A);
+                    if (own_x) {
+                        store_face<0>(A, at(g, i, j, p), o);
+                        if (i == 1) store_face<0>(A, at(g, g.N1 + 1, j, p), o);
+                    }
+                    mx = out_flux<0>(o, A);
+                }
+                if (yface) {
+                    const float4 nb = s_h[tS], fw = s_fy[tS];
+                    const FwdF fwb{fw.x, fw.y, fw.z, fw.w};
+                    const FaceOutF o = combine(fwb, by, nb.y, H.u[1], nb.w, H.tp, nb.x, H.u[0], nb.z, H.u[2], oy[0],
+                                               oy[1], oy[2], A);
+                    if (own_y) {
+                        store_face<1>(A, at(g, i, j, p), o);
+                        if (j == 1) store_face<1>(A, at(g, i, g.N2 + 1, p), o);
+                    }
+                    my = out_flux<1>(o, A);
+                }
+                for (int v = 0; v < 5; ++v) { s_m[v][tid] = mx.m[v]; s_m[5 + v][tid] = my.m[v]; }
             }
-            if (yface) {
-                const LdsCell& nb = lds[tid - RX].c;
-                const FwdF fwb{nb.fy[0], nb.fy[1], nb.fy[2], nb.fy[3]};
-                const FaceOutF o = combine(fwb, by, nb.h[1], H.u[1], nb.h[3], H.tp, nb.h[0], H.u[0], nb.h[2], H.u[2],
-                                           oy[0], oy[1], oy[2], A);
-                if (yface_store) store_face<1>(A, at(g, i, j, p), o);
-                my = out_flux<1>(o, A);
-            }
+            stage(p + 1, nxt);
             __syncthreads();
-            LdsFlux& mf = lds[tid].f;
-            for (int v = 0; v < 5; ++v) { mf.mx[v] = mx.m[v]; mf.my[v] = my.m[v]; }
-            __syncthreads();
-            if (owned) {
-                const LdsFlux& e = lds[tid + 1].f;
-                const LdsFlux& n = lds[tid + RX].f;
+            if (owned && inner)
                 for (int v = 0; v < 5; ++v)
-                    Sxy[v] = (e.mx[v] - mx.m[v]) * A.ds[0] + (n.my[v] - my.m[v]) * A.ds[1];
-            }
-            __syncthreads();
+                    Sxy[v] = (s_m[v][tE] - mx.m[v]) * A.ds[0] + (s_m[5 + v][tN] - my.m[v]) * A.ds[1];
         }
 
         Hprev = H;
@@ -463,7 +513,10 @@ __global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(3))) 
             const double q = v == 0 ? Acur.ro : Acur.ro * (v == 4 ? Acur.t - A.t0d : Acur.u[v - 1]);
             cons_prev[v] = q - cd * inc;
         }
+        zb = cur.zt;
+        cur = nxt;
         Acur = Anext;
+        Anext = Anext2;
     }
     block_min_atomic(dtbits, A.dt_next);
 }
