@@ -43,9 +43,9 @@ __device__ __forceinline__ double rcpd(double x) {
     return fma(y, fma(-x, y, 1.0), y);
 }
 
-// 30x6 owned columns plus the ring: 32x8 = 256 threads, four full
-// wavefronts, so three workgroups fill a CU evenly at three waves per SIMD.
-constexpr int TX = 30, TY = 6;
+// 30x14 owned columns plus the ring: 32x16 = 512 threads, eight full
+// wavefronts, two per SIMD.
+constexpr int TX = 30, TY = 14;
 constexpr int RX = TX + 2, RY = TY + 2;
 constexpr int NTHR = RX * RY;
 
@@ -256,14 +256,19 @@ struct PlaneIn {
     double ou[3];
 };
 
-// At most 256 VGPRs: two waves per SIMD, two four-wave workgroups per CU.
+// At most 256 VGPRs: two waves per SIMD.
 template <bool Quirk>
 __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     __shared__ float4 s_h[NTHR];             // half level: u1, u2, u3, t'
     __shared__ float4 s_fx[NTHR], s_fy[NTHR];  // forward extrapolations along x, y
     __shared__ float s_m[10][NTHR];          // new x- and y-face fluxes
-    __shared__ double s_u[2][3][NTHR];       // level-n velocities of a plane
-    __shared__ float s_f[2][10][NTHR];       // x- and y-face of each lane's cell
+    // Level-n velocities of a plane and the x- and y-faces of each lane's
+    // cell, with a halo that the edge lanes fill with the value beyond the
+    // region, so every neighbour read is one unconditional LDS load.
+    constexpr int UX = RX + 2, UN = (RX + 2) * (RY + 2);   // velocities: one-cell halo all round
+    constexpr int FX = RX + 1, FN = (RX + 1) * (RY + 1);   // faces: one more column and row
+    __shared__ double s_u[2][3][UN];
+    __shared__ float s_f[2][10][FN];
     const Geo& g = a.g;
     const double dtd = a.S->dt;
     const float dt = (float)dtd, inv_dt2 = (float)(2.0 / dtd);
@@ -287,6 +292,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     // LDS neighbours, clamped so every lane reads inside the arrays
     const int tW = xfirst ? tid : tid - 1, tE = xlast ? tid : tid + 1;
     const int tS = yfirst ? tid : tid - RX, tN = ylast ? tid : tid + RX;
+    const int upos = (ty + 1) * UX + tx + 1, fpos = ty * FX + tx;
+    const int opos = xfirst ? upos - 1 : xlast ? upos + 1 : yfirst ? upos - UX : upos + UX;
     // cell and face offsets of this lane within a plane
     const int base = iw + jw * px;
     const int ex_off = xlast ? wrapc(i + 1, g.N1) + jw * px : iw + wrapc(j + 1, g.N2) * px;
@@ -297,6 +304,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     else if (yfirst) o_off = iw + wrapc(j - 1, g.N2) * px;
     else if (ylast) o_off = iw + wrapc(j + 1, g.N2) * px;
     const bool has_o = xfirst || xlast || yfirst || ylast;
+    const bool put_o = in_region && has_o, put_ex = in_region && has_ex;
     Args A = a;
 
     auto load_plane = [&](int p) {
@@ -312,18 +320,33 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
             for (int m = 0; m < 3; ++m) r.ou[m] = ldd(A.A, A.fb, 1 + m, o_off + zo);
         return r;
     };
+    auto put_face = [&](float* f, int q, const FaceF& r) {
+        f[0 * FN + q] = r.u[0]; f[1 * FN + q] = r.u[1]; f[2 * FN + q] = r.u[2];
+        f[3 * FN + q] = r.p; f[4 * FN + q] = r.tp;
+    };
+    // Plane p's faces and outward velocity into LDS (the lane's own velocity
+    // is written separately, from registers that are already there).
     auto stage = [&](int p, const PlaneIn& r) {
+        if (!in_region) return;
         float* f = &s_f[p & 1][0][0];
-        f[0 * NTHR + tid] = r.xw.u[0]; f[1 * NTHR + tid] = r.xw.u[1]; f[2 * NTHR + tid] = r.xw.u[2];
-        f[3 * NTHR + tid] = r.xw.p; f[4 * NTHR + tid] = r.xw.tp;
-        f[5 * NTHR + tid] = r.ys.u[0]; f[6 * NTHR + tid] = r.ys.u[1]; f[7 * NTHR + tid] = r.ys.u[2];
-        f[8 * NTHR + tid] = r.ys.p; f[9 * NTHR + tid] = r.ys.tp;
+        put_face(f, fpos, r.xw);
+        put_face(f + 5 * FN, fpos, r.ys);
+        if (put_ex) {
+            if (xlast) put_face(f, fpos + 1, r.ex);
+            else put_face(f + 5 * FN, fpos + FX, r.ex);
+        }
+        if (put_o)
+            for (int m = 0; m < 3; ++m) s_u[p & 1][m][opos] = r.ou[m];
+    };
+    auto put_u = [&](int p, const CellD& c) {
+        if (in_region)
+            for (int m = 0; m < 3; ++m) s_u[p & 1][m][upos] = c.u[m];
     };
     auto staged_face = [&](int p, int q, int off) {
         const float* f = &s_f[p & 1][off][0];
         FaceF r;
-        r.u[0] = f[0 * NTHR + q]; r.u[1] = f[1 * NTHR + q]; r.u[2] = f[2 * NTHR + q];
-        r.p = f[3 * NTHR + q]; r.tp = f[4 * NTHR + q];
+        r.u[0] = f[0 * FN + q]; r.u[1] = f[1 * FN + q]; r.u[2] = f[2 * FN + q];
+        r.p = f[3 * FN + q]; r.tp = f[4 * FN + q];
         return r;
     };
 
@@ -337,7 +360,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     }
     PlaneIn cur = load_plane(k0 - 1);
     FaceF zb = load_face(A, 2, base + wrapc(k0 - 1, g.N3) * sxy);
-    for (int m = 0; m < 3; ++m) s_u[(k0 - 1) & 1][m][tid] = Acur.u[m];
+    put_u(k0 - 1, Acur);
     stage(k0 - 1, cur);
     // Retire every prologue load here, so that the loop's waits only ever
     // cover the plane being consumed, never the plane being prefetched.
@@ -357,7 +380,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
         PlaneIn nxt = cur;
         CellD Anext2 = Anext;
         if (p < kend) {
-            for (int m = 0; m < 3; ++m) s_u[(p + 1) & 1][m][tid] = Anext.u[m];
+            put_u(p + 1, Anext);
             nxt = load_plane(p + 1);
             Anext2 = load_cell(A, base + wrapc(p + 2, g.N3) * sxy);
         }
@@ -370,10 +393,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
             for (int m = 0; m < 3; ++m) {
                 const double* u = &s_u[p & 1][m][0];
                 const double uc = Acur.u[m];
-                const double uW = xfirst ? cur.ou[m] : u[tW];
-                const double uE = xlast ? cur.ou[m] : u[tE];
-                const double uS = yfirst ? cur.ou[m] : u[tS];
-                const double uN = ylast ? cur.ou[m] : u[tN];
+                const double uW = u[upos - 1], uE = u[upos + 1];
+                const double uS = u[upos - UX], uN = u[upos + UX];
                 const float sxl = A.VIS * (float)(uc - uW) * A.inv_dx[0];
                 const float sxh = A.VIS * (float)(uE - uc) * A.inv_dx[0];
                 const float syl = A.VIS * (float)(uc - uS) * A.inv_dx[1];
@@ -392,9 +413,9 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
         float ox[3], oy[3], oz[3];
         {
             const FaceF& Xw = cur.xw;
-            const FaceF Xe = xlast ? cur.ex : staged_face(p, tE, 0);
+            const FaceF Xe = staged_face(p, fpos + 1, 0);
             const FaceF& Ys = cur.ys;
-            const FaceF Yn = ylast ? cur.ex : staged_face(p, tN, 5);
+            const FaceF Yn = staged_face(p, fpos + FX, 5);
             const FaceF& Zt = cur.zt;
             {
                 const Flux a0 = flux<0>(Xw.u, Xw.p, Xw.tp, A), a1 = flux<0>(Xe.u, Xe.p, Xe.tp, A);
