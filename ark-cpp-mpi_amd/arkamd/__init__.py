@@ -47,7 +47,7 @@ EXPORTED_SYMBOLS = [
     "ark_get_dt", "ark_time_step_size", "ark_phase1", "ark_stress_tensor", "ark_use_forces",
     "ark_phase2", "ark_step", "ark_advance", "ark_step_phases", "ark_diagnostics", "ark_init_tgv",
     "ark_synchronize", "ark_get_stream", "ark_profile_enable", "ark_profile_read",
-    "ark_profile_reset", "ark_device_bytes",
+    "ark_profile_reset", "ark_get_plane", "ark_set_plane", "ark_boundary_conditions", "ark_device_bytes",
 ]
 
 
@@ -121,6 +121,9 @@ def lib() -> ctypes.CDLL:
     L.ark_profile_read.argtypes = [vp, ip, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), dp,
                                    ctypes.POINTER(ctypes.c_int)]
     L.ark_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.ark_get_plane.argtypes = [vp, ip, ip, ip, dp, ip, ip]
+    L.ark_set_plane.argtypes = [vp, ip, ip, ip, dp, ip, ip]
+    L.ark_boundary_conditions.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int), ip, ctypes.c_double]
     for fn in EXPORTED_SYMBOLS:
         if fn not in ("ark_destroy", "ark_last_error"):
             getattr(L, fn).restype = ctypes.c_int
@@ -274,6 +277,34 @@ class Solver:
         s = ctypes.c_void_p()
         self._check(self._L.ark_get_stream(self._h, ctypes.byref(s)))
         return s.value or 0
+
+    # ---- halo planes and BoundaryConditions -----------------------------------
+    def plane_shape(self, axis: int) -> tuple:
+        """(nb, na) of a plane normal to `axis`: numpy shape of the Arr2d buffer."""
+        N1, N2, N3 = self.N
+        na = N2 if axis == 0 else N1
+        nb = N2 if axis == 2 else N3
+        return (nb, na)
+
+    def get_plane(self, name: str, axis: int, q: int) -> np.ndarray:
+        nb, na = self.plane_shape(axis)
+        out = np.empty((nb, na), dtype=np.float64)
+        self._check(self._L.ark_get_plane(self._h, FIELD_ID[name], axis, q,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), na, nb))
+        return out
+
+    def set_plane(self, name: str, axis: int, q: int, plane: np.ndarray) -> None:
+        nb, na = self.plane_shape(axis)
+        a = np.ascontiguousarray(plane, dtype=np.float64)
+        if a.shape != (nb, na):
+            raise ValueError(f"plane shape {a.shape} != {(nb, na)}")
+        self._check(self._L.ark_set_plane(self._h, FIELD_ID[name], axis, q,
+                                          a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), na, nb))
+
+    def boundary_conditions(self, axis: int, fields, periodic: bool, t0: float) -> None:
+        """BoundaryConditions::alongX{axis+1} on the fields (u1, u2, u3, ro, t)."""
+        ids = (ctypes.c_int * 5)(*[FIELD_ID[f] for f in fields])
+        self._check(self._L.ark_boundary_conditions(self._h, axis, ids, int(bool(periodic)), float(t0)))
 
     def device_bytes(self) -> int:
         v = ctypes.c_int64()

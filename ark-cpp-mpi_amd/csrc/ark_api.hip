@@ -202,6 +202,15 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
         if (s == ARK_OK) c->allocs.push_back(c->diag_part);
     }
     if (s == ARK_OK) {
+        const long long N1 = p->N1, N2 = p->N2, N3 = p->N3;
+        const long long plane = std::max(N2 * N3, std::max(N1 * N3, N1 * N2));
+        s = c->hip(hipMalloc(&c->plane_buf, plane * sizeof(double)), "hipMalloc plane buffer");
+        if (s == ARK_OK) {
+            c->allocs.push_back(c->plane_buf);
+            c->bytes += plane * sizeof(double);
+        }
+    }
+    if (s == ARK_OK) {
         // dt starts from min(1e8, dtv), the constant part of the reference's
         // running min3d(dt, dtu, dtv) (source/main.cpp:377, 393-405).
         double dtv;
@@ -719,6 +728,66 @@ ark_status ark_profile_read(ark_ctx* c, int max_entries, char (*names)[64], int6
     }
     *n_entries = (int)keys.size();
     return ARK_OK;
+}
+
+// Field planes and boundary conditions (the host-side halo and BC interface).
+static bool is_cell_field(int f) { return f >= F_roCon && f <= F_condition; }
+
+static ark_status plane_args(ark_ctx* c, int field, int axis, int q, int na, int nb) {
+    if (field < 0 || field >= F_NUM || axis < 0 || axis > 2) return c->fail(ARK_ERR_INVALID, "plane: bad field or axis");
+    const int Nd = axis == 0 ? c->geo.N1 : axis == 1 ? c->geo.N2 : c->geo.N3;
+    const int Na = axis == 0 ? c->geo.N2 : c->geo.N1, Nb = axis == 2 ? c->geo.N2 : c->geo.N3;
+    const int qmax = Nd + (is_cell_field(field) ? 1 : 2);
+    if (q < 0 || q > qmax || na != Na || nb != Nb) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "plane: index %d (0..%d) or extents %dx%d (expected %dx%d) out of range", q, qmax,
+                 na, nb, Na, Nb);
+        return c->fail(ARK_ERR_INVALID, buf);
+    }
+    if (c->prm.precision == ARK_PREC_MIXED && !is_cell_field(field))
+        return c->fail(ARK_ERR_UNSUPPORTED, "plane: face fields are FP32 in mixed precision");
+    return ARK_OK;
+}
+
+ark_status ark_get_plane(ark_ctx* c, int field, int axis, int q, double* host, int na, int nb) {
+    if (!c || !host) return ARK_ERR_INVALID;
+    ark_status s = plane_args(c, field, axis, q, na, nb);
+    if (s == ARK_OK && (field == F_ro1 || field == F_ro2 || field == F_ro3)) s = ark_refresh_rho(c);
+    if (s == ARK_OK) s = c->hip(launch_plane(c->stream, c->geo, c->cur[field], axis, q, c->plane_buf, 1), "plane");
+    if (s == ARK_OK)
+        s = c->hip(hipMemcpyAsync(host, c->plane_buf, (size_t)na * nb * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream), "plane to host");
+    return s == ARK_OK ? c->hip(hipStreamSynchronize(c->stream), "sync") : s;
+}
+
+ark_status ark_set_plane(ark_ctx* c, int field, int axis, int q, const double* host, int na, int nb) {
+    if (!c || !host) return ARK_ERR_INVALID;
+    ark_status s = plane_args(c, field, axis, q, na, nb);
+    if (s != ARK_OK) return s;
+    if (field >= F_sigm11) c->sigma_dirty = true;
+    if (field == F_ro1 || field == F_ro2 || field == F_ro3 || field == F_p1 || field == F_p2 || field == F_p3) {
+        c->rho_consistent = false;
+        c->rho_stored = true;
+    }
+    if (field == F_condition) {
+        for (long long k = 0; k < (long long)na * nb; ++k)
+            if (host[k] != 0.0) { c->has_solid = true; break; }
+    }
+    s = c->hip(hipMemcpyAsync(c->plane_buf, host, (size_t)na * nb * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream), "plane to device");
+    if (s == ARK_OK) s = c->hip(launch_plane(c->stream, c->geo, c->cur[field], axis, q, c->plane_buf, 0), "plane");
+    return s == ARK_OK ? c->hip(hipStreamSynchronize(c->stream), "sync") : s;
+}
+
+ark_status ark_boundary_conditions(ark_ctx* c, int axis, const int fields[5], int periodic, double t0) {
+    if (!c || !fields || axis < 0 || axis > 2) return ARK_ERR_INVALID;
+    BcFields F;
+    for (int q = 0; q < 5; ++q) {
+        if (fields[q] < 0 || !is_cell_field(fields[q]))
+            return c->fail(ARK_ERR_INVALID, "boundary conditions: fields must be cell fields");
+        F.f[q] = c->cur[fields[q]];
+    }
+    return c->hip(launch_boundary(c->stream, c->geo, F, axis, periodic, t0), "boundary conditions");
 }
 
 ark_status ark_device_bytes(const ark_ctx* c, int64_t* bytes) {

@@ -32,6 +32,7 @@ struct ark_ctx {
     arkd::Scalars* S = nullptr;        // device scalars
     double* diag_part = nullptr;
     double* staging = nullptr;         // one FP64 field (mixed precision conversions)
+    double* plane_buf = nullptr;       // one interior plane (ark_get_plane / ark_set_plane)
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;

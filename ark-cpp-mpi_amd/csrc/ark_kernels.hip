@@ -558,4 +558,72 @@ hipError_t launch_init_tgv(hipStream_t st, const Geo& g, const Fields& F, const 
     return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// BoundaryConditions::alongX1/2/3 (source/BoundaryConditions.cpp:22-85) on
+// five cell fields u1, u2, u3, ro, t.  With n = N_d + 1 (the reference's node
+// count) the ghost planes are 0 and n and the loops run over the interior
+// 1..N of the two other axes.  alongX1 copies the neighbouring interior plane
+// whatever the period flag (22-42); alongX2/alongX3 wrap when periodic and
+// otherwise set walls (u = 0, ro zero-gradient, t = t0); the periodic top
+// plane of alongX3 takes t from ro(…, 1) as the reference does (73).
+__global__ void __launch_bounds__(256) k_boundary(Geo g, BcFields F, int axis, int periodic, double t0) {
+    const int Na = axis == 0 ? g.N2 : g.N1, Nb = axis == 2 ? g.N2 : g.N3;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int b = blockIdx.y + 1;
+    if (a > Na || b > Nb) return;
+    const int n = (axis == 0 ? g.N1 : axis == 1 ? g.N2 : g.N3) + 1;
+    const int stride = axis == 0 ? 1 : axis == 1 ? g.px : g.sxy;
+    const int c0 = axis == 0 ? at(g, 0, a, b) : axis == 1 ? at(g, a, 0, b) : at(g, a, b, 0);
+    const int top = c0 + n * stride, bot = c0;
+    if (axis == 0) {
+        for (int q = 0; q < 5; ++q) {
+            F.f[q][top] = F.f[q][top - stride];
+            F.f[q][bot] = F.f[q][bot + stride];
+        }
+        return;
+    }
+    const int one = c0 + stride, last = c0 + (n - 1) * stride;
+    if (periodic) {
+        for (int q = 0; q < 3; ++q) F.f[q][top] = F.f[q][one];
+        F.f[3][top] = F.f[3][one];
+        F.f[4][top] = axis == 2 ? F.f[3][one] : F.f[4][one];
+        for (int q = 0; q < 5; ++q) F.f[q][bot] = F.f[q][last];
+    } else {
+        for (int q = 0; q < 3; ++q) {
+            F.f[q][top] = 0.;
+            F.f[q][bot] = 0.;
+        }
+        F.f[3][top] = F.f[3][last];
+        F.f[3][bot] = F.f[3][one];
+        F.f[4][top] = t0;
+        F.f[4][bot] = t0;
+    }
+}
+
+// One plane normal to `axis` at index q, interior 1..N of the two other axes,
+// to or from a buffer in the Arr2d layout buf[b*Na + a] (source/Arr2d.cpp:14-16),
+// as the reference's halo packing loops fill it (e.g. source/main.cpp:619-630).
+__global__ void __launch_bounds__(256) k_plane(Geo g, double* field, int axis, int q, double* buf, int to_buf) {
+    const int Na = axis == 0 ? g.N2 : g.N1, Nb = axis == 2 ? g.N2 : g.N3;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (a >= Na || b >= Nb) return;
+    const int c = axis == 0 ? at(g, q, a + 1, b + 1) : axis == 1 ? at(g, a + 1, q, b + 1) : at(g, a + 1, b + 1, q);
+    if (to_buf) buf[(size_t)b * Na + a] = field[c];
+    else field[c] = buf[(size_t)b * Na + a];
+}
+
+hipError_t launch_boundary(hipStream_t st, const Geo& g, const BcFields& F, int axis, int periodic, double t0) {
+    const int Na = axis == 0 ? g.N2 : g.N1, Nb = axis == 2 ? g.N2 : g.N3;
+    hipLaunchKernelGGL(k_boundary, dim3((Na + 255) / 256, Nb), dim3(256), 0, st, g, F, axis, periodic, t0);
+    return hipGetLastError();
+}
+
+hipError_t launch_plane(hipStream_t st, const Geo& g, double* field, int axis, int q, double* buf, int to_buf) {
+    const int Na = axis == 0 ? g.N2 : g.N1, Nb = axis == 2 ? g.N2 : g.N3;
+    hipLaunchKernelGGL(k_plane, dim3((Na + 255) / 256, Nb), dim3(256), 0, st, g, field, axis, q, buf, to_buf);
+    return hipGetLastError();
+}
+
 }  // namespace arkd

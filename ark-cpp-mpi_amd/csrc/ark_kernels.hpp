@@ -33,6 +33,9 @@ hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, 
 hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
 hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan);
+struct BcFields { double* f[5]; };   // u1, u2, u3, ro, t
+hipError_t launch_boundary(hipStream_t st, const Geo& g, const BcFields& F, int axis, int periodic, double t0);
+hipError_t launch_plane(hipStream_t st, const Geo& g, double* field, int axis, int q, double* buf, int to_buf);
 hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 // Zero narr arrays of n elements each; dev_ptrs is a device array of pointers.
 hipError_t launch_zero(hipStream_t st, double* const* dev_ptrs, int narr, long long n);

@@ -174,6 +174,26 @@ ark_status ark_profile_read(ark_ctx* ctx, int max_entries, char (*names)[64],
                             int64_t* launches, double* total_ms, int* n_entries);
 ark_status ark_profile_reset(ark_ctx* ctx);
 
+/* Halo planes, as the reference packs them for MPI_Sendrecv
+ * (source/main.cpp:619-653, 1101-1189 and the X2/X3 analogues): plane q
+ * normal to `axis` (0..N_d+1 for cell fields, 0..N_d+2 for face fields),
+ * the interior 1..N of the two other axes, in the Arr2d layout
+ * buf[b*na + a] (source/Arr2d.cpp:14-16) with a the lower of the two other
+ * axes; na, nb must equal their interior extents.  A multi-rank host packs
+ * its send buffers with ark_get_plane and unpacks the received ones with
+ * ark_set_plane.  Both synchronise.  Face fields are FP64-only. */
+ark_status ark_get_plane(ark_ctx* ctx, int field, int axis, int q, double* host, int na, int nb);
+ark_status ark_set_plane(ark_ctx* ctx, int field, int axis, int q, const double* host, int na, int nb);
+
+/* BoundaryConditions::alongX1/2/3() (source/BoundaryConditions.cpp:22-85) on
+ * five cell fields given as field ids {u1, u2, u3, ro, t}: ghost planes 0
+ * and N_d+1 of axis `axis` over the interior of the other two axes.  Along
+ * x1 the neighbouring interior plane is copied whatever `periodic` says;
+ * along x2 and x3 the planes wrap when periodic, otherwise u = 0, ro is
+ * zero-gradient and t = t0; the periodic top plane along x3 takes t from
+ * ro(i, j, 1), as the reference does.  Enqueued on the context's stream. */
+ark_status ark_boundary_conditions(ark_ctx* ctx, int axis, const int fields[5], int periodic, double t0);
+
 /* Bytes of device memory held by the context. */
 ark_status ark_device_bytes(const ark_ctx* ctx, int64_t* bytes);
 

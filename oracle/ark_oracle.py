@@ -251,3 +251,51 @@ def state_from_npz(z, prefix: str) -> OracleState:
         if name in sc:
             st.arrays[name] = np.ascontiguousarray(sc[name], dtype=np.float64)
     return st
+
+
+# ---- BoundaryConditions and halo planes -------------------------------------
+def boundary_conditions(arrays, axis: int, periodic: bool, t0: float) -> None:
+    """BoundaryConditions::alongX{axis+1}() (source/BoundaryConditions.cpp:22-85)
+    in place on five cell arrays [u1, u2, u3, ro, t], numpy shape
+    (N3+2, N2+2, N1+2).  n = N+1 is the reference's node count; the loops run
+    over the interior 1..N of the two other axes."""
+    u1, u2, u3, ro, t = arrays
+    N3, N2, N1 = (s - 2 for s in u1.shape)
+    if axis == 0:                                   # alongX1 (22-42): period flag unused
+        n = N1 + 1
+        for a in arrays:
+            a[1:N3 + 1, 1:N2 + 1, n] = a[1:N3 + 1, 1:N2 + 1, n - 1]
+            a[1:N3 + 1, 1:N2 + 1, 0] = a[1:N3 + 1, 1:N2 + 1, 1]
+        return
+
+    def sl(q):                                      # plane q of the axis, interior of the others
+        return (slice(1, N3 + 1), q, slice(1, N1 + 1)) if axis == 1 else (q, slice(1, N2 + 1), slice(1, N1 + 1))
+
+    n = (N2 if axis == 1 else N3) + 1
+    if periodic:                                    # 44-63 (x2), 65-85 (x3)
+        for a in (u1, u2, u3, ro):
+            a[sl(n)] = a[sl(1)]
+        t[sl(n)] = ro[sl(1)] if axis == 2 else t[sl(1)]   # x3: t from ro (73)
+        for a in arrays:
+            a[sl(0)] = a[sl(n - 1)]
+    else:
+        for a in (u1, u2, u3):
+            a[sl(n)] = 0.0
+            a[sl(0)] = 0.0
+        ro[sl(n)] = ro[sl(n - 1)]
+        ro[sl(0)] = ro[sl(1)]
+        t[sl(n)] = t0
+        t[sl(0)] = t0
+
+
+def pack_plane(arr: np.ndarray, N, axis: int, q: int) -> np.ndarray:
+    """The reference's halo packing loop (e.g. source/main.cpp:619-630): plane
+    q normal to `axis` over the interior 1..N of the two other axes, as an
+    Arr2d whose elem(a, b) = buf[b*na + a] (source/Arr2d.cpp:14-16), a being
+    the lower of the two other axes; returned with numpy shape (nb, na)."""
+    N1, N2, N3 = N
+    if axis == 0:
+        return arr[1:N3 + 1, 1:N2 + 1, q].copy()
+    if axis == 1:
+        return arr[1:N3 + 1, q, 1:N1 + 1].copy()
+    return arr[q, 1:N2 + 1, 1:N1 + 1].copy()

@@ -15,6 +15,11 @@ Fixtures written:
   tgv8.npz
       the full reference state (all 41 arrays with ghosts) of the 8^3
       Taylor-Green run after 1, 2 and 10 steps, with its dt/KE/mass series.
+  bc.npz
+      five seeded random cell arrays on a 5x4x3 box and the reference
+      BoundaryConditions class's output (oracle/_ref/ark_ref_bc, built from
+      source/BoundaryConditions.cpp and source/Arr3d.cpp) for alongX1/2/3,
+      periodic and not.  ``python make_fixtures.py bc`` writes only this file.
   tgv_digests.json
       per-array SHA-256 and norms of the 16^3 x 10-step and 64^3 x 200-step
       Taylor-Green runs (1 rank, full arrays), their per-step dt, KE and mass
@@ -39,6 +44,7 @@ sys.path.insert(0, os.path.join(REPO, "oracle"))
 from ark_oracle import FIELDS, OracleState, read_meta  # noqa: E402
 
 HARNESS = os.path.join(REPO, "oracle", "_ref", "ark_ref_harness")
+BC_HARNESS = os.path.join(REPO, "oracle", "_ref", "ark_ref_bc")
 MPIRUN = "/opt/conda/bin/mpirun"
 PHASES = ["TimeStepSize", "Phase1", "StressTensor", "UseForces", "Phase2", "Step"]
 
@@ -140,11 +146,40 @@ def tgv_run(N: int, steps: int, work: str, dumps, ranks=1) -> str:
     return d
 
 
+def bc_fixture(path: str, work: str) -> None:
+    """Reference BoundaryConditions on seeded random arrays, every axis and flag."""
+    N1, N2, N3 = 5, 4, 3
+    t0 = 1.25
+    rng = np.random.default_rng(777)
+    inp = rng.uniform(-1.0, 2.0, size=(5, N3 + 2, N2 + 2, N1 + 2))
+    fin = os.path.join(work, "bc_in.f64")
+    inp.astype("<f8").tofile(fin)
+    data = {"input": inp, "N": np.array([N1, N2, N3]), "t0": np.array(t0)}
+    for axis in range(3):
+        for periodic in (0, 1):
+            fout = os.path.join(work, f"bc_{axis}_{periodic}.f64")
+            subprocess.run([BC_HARNESS, str(N1), str(N2), str(N3), str(axis), str(periodic), repr(t0), fin, fout],
+                           check=True)
+            data[f"out_{axis}_{periodic}"] = np.fromfile(fout, dtype="<f8").reshape(inp.shape)
+    np.savez_compressed(path, **data)
+
+
 def main() -> None:
+    if len(sys.argv) > 1 and sys.argv[1] == "bc":
+        if not os.path.exists(BC_HARNESS):
+            sys.exit("reference BoundaryConditions harness not built: make -C oracle ref")
+        work = tempfile.mkdtemp(prefix="ark_golden_")
+        try:
+            bc_fixture(os.path.join(HERE, "bc.npz"), work)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        print("bc fixture written to", HERE)
+        return
     if not os.path.exists(HARNESS):
         sys.exit("reference harness not built: make -C oracle ref")
     work = tempfile.mkdtemp(prefix="ark_golden_")
     try:
+        bc_fixture(os.path.join(HERE, "bc.npz"), work)
         phase_fixture(os.path.join(HERE, "phase_fluid.npz"), 1234, False, work)
         phase_fixture(os.path.join(HERE, "phase_solid.npz"), 4321, True, work)
 

@@ -38,3 +38,18 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(arkamd, "LIB_PATH", "/nonexistent/libark_amd.so")
     with pytest.raises(RuntimeError, match="not built"):
         arkamd.lib()
+
+
+def test_cpp_host_layer_built():
+    """The C++ host interface is in the library and its programs link against
+    it (no GPU call: the programs are only asked for their usage)."""
+    import subprocess
+    lib = arkamd.LIB_PATH
+    syms = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    for name in ("ark::Solver::Solver", "ark::Arr3d::bind", "ark::Arr2d::pack",
+                 "ark::BoundaryConditions::alongX1", "ark::BoundaryConditions::alongX3"):
+        assert name in syms, name
+    bindir = os.path.join(os.path.dirname(lib), "..", "bin")
+    for prog in ("ark_driver", "ark_host_check"):
+        res = subprocess.run([os.path.join(bindir, prog)], capture_output=True, text=True)
+        assert res.returncode == 2 and "usage" in res.stderr
