@@ -48,6 +48,8 @@ EXPORTED_SYMBOLS = [
     "ark_phase2", "ark_step", "ark_advance", "ark_step_phases", "ark_diagnostics", "ark_init_tgv",
     "ark_synchronize", "ark_get_stream", "ark_profile_enable", "ark_profile_read",
     "ark_profile_reset", "ark_get_plane", "ark_set_plane", "ark_boundary_conditions", "ark_device_bytes",
+    "ark_batch_create", "ark_batch_destroy", "ark_batch_last_error", "ark_batch_step", "ark_batch_synchronize",
+    "ark_batch_autotune", "ark_batch_set_config", "ark_batch_get_config",
 ]
 
 
@@ -124,8 +126,18 @@ def lib() -> ctypes.CDLL:
     L.ark_get_plane.argtypes = [vp, ip, ip, ip, dp, ip, ip]
     L.ark_set_plane.argtypes = [vp, ip, ip, ip, dp, ip, ip]
     L.ark_boundary_conditions.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int), ip, ctypes.c_double]
+    L.ark_batch_create.argtypes = [ctypes.POINTER(vp), ip, ctypes.POINTER(vp)]
+    L.ark_batch_destroy.argtypes = [vp]
+    L.ark_batch_destroy.restype = None
+    L.ark_batch_last_error.argtypes = [vp]
+    L.ark_batch_last_error.restype = ctypes.c_char_p
+    L.ark_batch_step.argtypes = [vp, ip]
+    L.ark_batch_synchronize.argtypes = [vp]
+    L.ark_batch_autotune.argtypes = [vp, ip, dp, ip]
+    L.ark_batch_set_config.argtypes = [vp, ip, ip]
+    L.ark_batch_get_config.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 4
     for fn in EXPORTED_SYMBOLS:
-        if fn not in ("ark_destroy", "ark_last_error"):
+        if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error"):
             getattr(L, fn).restype = ctypes.c_int
     _lib = L
     return L
@@ -330,3 +342,63 @@ class Solver:
         for q in range(min(n.value, cap)):
             out[bytes(names[q]).split(b"\0", 1)[0].decode()] = (int(launches[q]), float(total[q]))
         return out
+
+
+BATCH_SHAPES = ["30x6", "30x14", "62x6"]
+BATCH_LZ = [16, 32, 64]
+
+
+class Batch:
+    """Mixed-precision boxes advanced together by one launch per step
+    (``ark_batch``).  The Solver objects stay usable and must outlive it."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        self._L = lib()
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        st = self._L.ark_batch_create(arr, len(self.solvers), ctypes.byref(h))
+        if st != ARK_OK:
+            raise ArkError(st, "ark_batch_create failed (boxes must be mixed precision, fluid, one device and "
+                               "scheme, N >= 2)")
+        self._h = h
+
+    def _check(self, st: int) -> None:
+        if st != ARK_OK:
+            raise ArkError(st, self._L.ark_batch_last_error(self._h).decode())
+
+    def step(self, nsteps: int = 1) -> None:
+        self._check(self._L.ark_batch_step(self._h, int(nsteps)))
+
+    def synchronize(self) -> None:
+        self._check(self._L.ark_batch_synchronize(self._h))
+
+    def autotune(self, reps: int = 3) -> Dict[tuple, float]:
+        ms = (ctypes.c_double * 9)()
+        self._check(self._L.ark_batch_autotune(self._h, int(reps), ms, 9))
+        return {(BATCH_SHAPES[i // 3], BATCH_LZ[i % 3]): ms[i] for i in range(9)}
+
+    def set_config(self, shape: int, lz: int) -> None:
+        self._check(self._L.ark_batch_set_config(self._h, int(shape), int(lz)))
+
+    def config(self) -> Dict[str, int]:
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self._L.ark_batch_get_config(self._h, *[ctypes.byref(x) for x in v]))
+        return {"shape": BATCH_SHAPES[v[0].value], "lz": v[1].value, "tiles": v[2].value, "threads": v[3].value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.ark_batch_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -446,6 +446,11 @@ ark_status ark_cfl_finalize(ark_ctx* c, int advance) {
     return c->launch("cfl_finalize", run_cfl_fin, &f);
 }
 
+ark_status ark_cfl_scan(ark_ctx* c) {
+    CflArgs a{c};
+    return c->launch("cfl", run_cfl, &a);
+}
+
 ark_status ark_cfl_step(ark_ctx* c, int advance) {
     CflArgs a{c};
     ark_status s = c->launch("cfl", run_cfl, &a);

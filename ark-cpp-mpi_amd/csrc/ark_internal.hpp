@@ -56,6 +56,7 @@ struct ark_ctx {
 
 // Step building blocks shared by the drivers (ark_api.hip; internal, not in ark.h).
 extern "C" {
+ark_status ark_cfl_scan(ark_ctx* c);                    // running MIN of the CFL bound over Con
 ark_status ark_cfl_step(ark_ctx* c, int advance);       // TimeStepSize on Con, then dt := MIN
 ark_status ark_cfl_finalize(ark_ctx* c, int mode);      // see k_cfl_finalize
 ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pending MIN or Con), TIME, nStep

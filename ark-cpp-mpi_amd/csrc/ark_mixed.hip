@@ -23,6 +23,10 @@
 //
 // Blocking and the order of work per plane are those of the FP64 fused
 // kernel (ark_fused.hip): owned columns plus a one-cell ring, z-march.
+#include <algorithm>
+#include <string>
+#include <vector>
+
 #include "ark_internal.hpp"
 #include "ark_kernels.hpp"
 
@@ -43,11 +47,17 @@ __device__ __forceinline__ double rcpd(double x) {
     return fma(y, fma(-x, y, 1.0), y);
 }
 
-// 30x14 owned columns plus the ring: 32x16 = 512 threads, eight full
-// wavefronts, two per SIMD.
-constexpr int TX = 30, TY = 14;
-constexpr int RX = TX + 2, RY = TY + 2;
-constexpr int NTHR = RX * RY;
+// Tile shapes: TX x TY owned columns plus a one-cell ring, RX x RY = (TX+2)
+// x (TY+2) threads, a multiple of four wavefronts so the four SIMDs of a CU
+// are equally loaded.  The single-box step uses 30x14 (512 threads, two
+// waves per SIMD); the batched step lets its autotuner choose.
+template <int TX_, int TY_>
+struct Shape {
+    static constexpr int TX = TX_, TY = TY_, RX = TX_ + 2, RY = TY_ + 2, NTHR = (TX_ + 2) * (TY_ + 2);
+};
+using Shape30x6 = Shape<30, 6>;
+using Shape30x14 = Shape<30, 14>;
+using Shape62x6 = Shape<62, 6>;
 
 struct Args {
     Geo g;
@@ -256,9 +266,11 @@ struct PlaneIn {
     double ou[3];
 };
 
-// At most 256 VGPRs: two waves per SIMD.
-template <bool Quirk>
-__global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
+// One tile (tile indices tbx, tby, tbz) of one box.  At most 256 VGPRs: two
+// waves per SIMD.
+template <bool Quirk, class S>
+__device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int tbz) {
+    constexpr int TX = S::TX, TY = S::TY, RX = S::RX, RY = S::RY, NTHR = S::NTHR;
     __shared__ float4 s_h[NTHR];             // half level: u1, u2, u3, t'
     __shared__ float4 s_fx[NTHR], s_fy[NTHR];  // forward extrapolations along x, y
     __shared__ float s_m[10][NTHR];          // new x- and y-face fluxes
@@ -276,8 +288,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     const float cf = (float)cd;
     const int tid = threadIdx.x;
     const int tx = tid % RX, ty = tid / RX;
-    const TileId tile = die_major_tile((g.N1 + TX - 1) / TX, (g.N2 + TY - 1) / TY, (g.N3 + a.LZ - 1) / a.LZ);
-    const int i0 = tile.x * TX + 1, j0 = tile.y * TY + 1, k0 = tile.z * a.LZ + 1;
+    const int i0 = tbx * TX + 1, j0 = tby * TY + 1, k0 = tbz * a.LZ + 1;
     const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = min(k0 + a.LZ, g.N3 + 1);
     const int i = min(i0 - 1 + tx, iend), j = min(j0 - 1 + ty, jend);
     const bool in_region = (i0 - 1 + tx <= iend) && (j0 - 1 + ty <= jend);
@@ -542,6 +553,50 @@ __global__ void __launch_bounds__(NTHR) k_fused_mixed(Args a) {
     block_min_atomic(dtbits, A.dt_next);
 }
 
+template <bool Quirk, class S>
+__global__ void __launch_bounds__(S::NTHR) k_fused_mixed(Args a) {
+    const TileId t = die_major_tile((a.g.N1 + S::TX - 1) / S::TX, (a.g.N2 + S::TY - 1) / S::TY,
+                                    (a.g.N3 + a.LZ - 1) / a.LZ);
+    mixed_tile<Quirk, S>(a, t.x, t.y, t.z);
+}
+
+// Many boxes in one launch.  start[b] is the first global tile of box b
+// (start[nboxes] the total); the global tile of a workgroup follows the same
+// die-major order as a single box, so each die works through one run of
+// tiles, box after box.  Every box is computed exactly as by k_fused_mixed:
+// the values of a cell do not depend on the tiling.
+template <bool Quirk, class S>
+__global__ void __launch_bounds__(S::NTHR) k_fused_mixed_batch(const Args* __restrict__ boxes,
+                                                               const int* __restrict__ start, int nboxes) {
+    const int T = start[nboxes];
+    const int die = blockIdx.x % kDies, local = blockIdx.x / kDies;
+    const int q = T / kDies, r = T % kDies;
+    const int t = die * q + min(die, r) + local;
+    int lo = 0, hi = nboxes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const Args& a = boxes[lo];
+    const int tl = t - start[lo];
+    const int nbx = (a.g.N1 + S::TX - 1) / S::TX, nby = (a.g.N2 + S::TY - 1) / S::TY;
+    mixed_tile<Quirk, S>(a, tl % nbx, (tl / nbx) % nby, tl / (nbx * nby));
+}
+
+// dt of a new step for every box of a batch (k_cfl_finalize, mode 1, per box).
+__global__ void k_batch_begin_step(Scalars* const* __restrict__ S, int nboxes) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nboxes) return;
+    Scalars* s = S[b];
+    const double dt = __longlong_as_double((long long)s->dt_bits);
+    s->dt_bits = s->dt_init;
+    s->dt = dt;
+    s->nstep += 1;
+    const double tm = s->time + 0.5 * dt;
+    s->time = tm + 0.5 * dt;
+}
+
 // ---------------------------------------------------------------------------
 // Conversions between the FP64 API arrays and the FP32 face storage.
 // mode 0: u or p (plain), mode 1: t (t' = t - t0).
@@ -566,14 +621,30 @@ using namespace arkd;
 
 namespace {
 
-struct MixedLaunch {
-    ark_ctx* c;
-    int LZ;
-};
+// Tile shapes a launch can use (mixed::Shape): owned 30x6, 30x14, 62x6.
+constexpr int kShapes = 3;
+constexpr int kShapeTX[kShapes] = {30, 30, 62}, kShapeTY[kShapes] = {6, 14, 6};
+constexpr int kShapeThreads[kShapes] = {256, 512, 512};
+constexpr int kDefaultShape = 1;
 
-hipError_t run_mixed(void* p) {
-    MixedLaunch* ml = (MixedLaunch*)p;
-    ark_ctx* c = ml->c;
+int tiles_of(const Geo& g, int shape, int LZ) {
+    return ((g.N1 + kShapeTX[shape] - 1) / kShapeTX[shape]) * ((g.N2 + kShapeTY[shape] - 1) / kShapeTY[shape]) *
+           ((g.N3 + LZ - 1) / LZ);
+}
+
+// Planes per chunk: enough tiles to fill the chip several times over, long
+// enough chunks that the two ring planes per chunk stay cheap.
+int choose_lz_mixed(const Geo& g, int shape) {
+    long long cols = (long long)((g.N1 + kShapeTX[shape] - 1) / kShapeTX[shape]) *
+                     ((g.N2 + kShapeTY[shape] - 1) / kShapeTY[shape]);
+    int lz = 64;
+    while (lz > 8 && cols * ((g.N3 + lz - 1) / lz) < 2048) lz /= 2;
+    return lz < g.N3 ? lz : g.N3;
+}
+
+// Kernel arguments of one mixed step of context c: read the current buffers,
+// write the other ones; dt_next is where the next step's dt MIN goes.
+mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, int flip = 0) {
     const ark_params& P = c->prm;
     mixed::Args a;
     a.g = c->geo;
@@ -595,30 +666,57 @@ hipError_t run_mixed(void* p) {
     a.ro0d = P.ro0;
     a.t0d = P.t0;
     a.dvc = c->K.dvc;
-    a.A = c->gbase[0][c->gpar[0]];
-    a.Ao = c->gbase[0][c->gpar[0] ^ 1];
+    a.A = c->gbase[0][c->gpar[0] ^ flip];
+    a.Ao = c->gbase[0][c->gpar[0] ^ flip ^ 1];
     for (int d = 0; d < 3; ++d) {
-        a.F[d] = c->gbase[1 + d][c->gpar[1 + d]];
-        a.Fo[d] = c->gbase[1 + d][c->gpar[1 + d] ^ 1];
+        a.F[d] = c->gbase[1 + d][c->gpar[1 + d] ^ flip];
+        a.Fo[d] = c->gbase[1 + d][c->gpar[1 + d] ^ flip ^ 1];
     }
     a.fb = c->field_bytes;
     a.S = c->S;
-    a.dt_next = &c->S->dt_bits;
-    a.LZ = ml->LZ;
-    dim3 grid(((c->geo.N1 + mixed::TX - 1) / mixed::TX) * ((c->geo.N2 + mixed::TY - 1) / mixed::TY) *
-              ((c->geo.N3 + ml->LZ - 1) / ml->LZ));
-    if (c->prm.scheme == ARK_SCHEME_REFERENCE)
-        hipLaunchKernelGGL(mixed::k_fused_mixed<true>, grid, dim3(mixed::NTHR), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL(mixed::k_fused_mixed<false>, grid, dim3(mixed::NTHR), 0, c->stream, a);
+    a.dt_next = dt_next;
+    a.LZ = LZ;
+    return a;
+}
+
+template <class S>
+void launch_single(bool quirk, dim3 grid, hipStream_t st, const mixed::Args& a) {
+    if (quirk) hipLaunchKernelGGL((mixed::k_fused_mixed<true, S>), grid, dim3(S::NTHR), 0, st, a);
+    else hipLaunchKernelGGL((mixed::k_fused_mixed<false, S>), grid, dim3(S::NTHR), 0, st, a);
+}
+
+template <class S>
+void launch_batch(bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxes, const int* start, int n) {
+    if (quirk) hipLaunchKernelGGL((mixed::k_fused_mixed_batch<true, S>), grid, dim3(S::NTHR), 0, st, boxes, start, n);
+    else hipLaunchKernelGGL((mixed::k_fused_mixed_batch<false, S>), grid, dim3(S::NTHR), 0, st, boxes, start, n);
+}
+
+hipError_t launch_shape_single(int shape, bool quirk, dim3 grid, hipStream_t st, const mixed::Args& a) {
+    if (shape == 0) launch_single<mixed::Shape30x6>(quirk, grid, st, a);
+    else if (shape == 1) launch_single<mixed::Shape30x14>(quirk, grid, st, a);
+    else launch_single<mixed::Shape62x6>(quirk, grid, st, a);
     return hipGetLastError();
 }
 
-int choose_lz_mixed(const Geo& g) {
-    long long cols = (long long)((g.N1 + mixed::TX - 1) / mixed::TX) * ((g.N2 + mixed::TY - 1) / mixed::TY);
-    int lz = 64;
-    while (lz > 8 && cols * ((g.N3 + lz - 1) / lz) < 2048) lz /= 2;
-    return lz < g.N3 ? lz : g.N3;
+hipError_t launch_shape_batch(int shape, bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxes,
+                              const int* start, int n) {
+    if (shape == 0) launch_batch<mixed::Shape30x6>(quirk, grid, st, boxes, start, n);
+    else if (shape == 1) launch_batch<mixed::Shape30x14>(quirk, grid, st, boxes, start, n);
+    else launch_batch<mixed::Shape62x6>(quirk, grid, st, boxes, start, n);
+    return hipGetLastError();
+}
+
+struct MixedLaunch {
+    ark_ctx* c;
+    int LZ;
+};
+
+hipError_t run_mixed(void* p) {
+    MixedLaunch* ml = (MixedLaunch*)p;
+    ark_ctx* c = ml->c;
+    const mixed::Args a = make_args(c, ml->LZ, &c->S->dt_bits);
+    dim3 grid(tiles_of(c->geo, kDefaultShape, ml->LZ));
+    return launch_shape_single(kDefaultShape, c->prm.scheme == ARK_SCHEME_REFERENCE, grid, c->stream, a);
 }
 
 int face_mode(int f) { return (f == F_t1 || f == F_t2 || f == F_t3) ? 1 : 0; }
@@ -629,7 +727,7 @@ ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)
         return c->fail(ARK_ERR_UNSUPPORTED, "mixed precision needs N1, N2, N3 >= 2");
     ark_status s = ARK_OK;
-    MixedLaunch ml{c, choose_lz_mixed(c->geo)};
+    MixedLaunch ml{c, choose_lz_mixed(c->geo, kDefaultShape)};
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
         if (s == ARK_OK) s = c->launch("fused_mixed", run_mixed, &ml);
@@ -671,3 +769,292 @@ ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* stag
     }
     return ARK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batched mixed-precision steps: many independent boxes in one launch per
+// step, the launches replayed from two captured HIP graphs (one per buffer
+// parity), tile shape and chunk length chosen by timing the candidates.
+struct ark_batch {
+    std::vector<ark_ctx*> box;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool quirk = true;
+    int shape = kDefaultShape, lz_target = 32;
+    int tiles = 0;
+    mixed::Args* args[2] = {nullptr, nullptr};   // device: parity 0 (current -> other), parity 1
+    int* start = nullptr;                         // device: first tile of each box, total last
+    Scalars** sptr = nullptr;                     // device: each box's scalars
+    unsigned long long* scratch_dt = nullptr;     // device: dt MIN slots of the tuning launches
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool built = false;
+    std::vector<int> gpar_at_build;               // box buffer parities the tables were built for
+    int parity = 0;                               // which table the next step uses
+    std::vector<double> tune_ms;                  // ms per step of each candidate (shape-major)
+    std::string err;
+
+    ark_status fail(ark_status s, const std::string& m) {
+        err = m;
+        return s;
+    }
+    ark_status hip(hipError_t e, const char* what) {
+        if (e == hipSuccess) return ARK_OK;
+        return fail(ARK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+};
+
+namespace {
+
+int box_lz(const ark_batch* b, const ark_ctx* c) { return std::min(b->lz_target, c->geo.N3); }
+
+void drop_graphs(ark_batch* b) {
+    for (int p = 0; p < 2; ++p) {
+        if (b->exec[p]) (void)hipGraphExecDestroy(b->exec[p]);
+        if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+        b->exec[p] = nullptr;
+        b->graph[p] = nullptr;
+    }
+    b->built = false;
+}
+
+// Argument tables for the boxes' current buffers (and, for the tuning
+// launches, dt MINs redirected to scratch slots), the tile table, and the two
+// one-step graphs.
+ark_status build_tables(ark_batch* b, bool tuning) {
+    const int n = (int)b->box.size();
+    std::vector<int> start(n + 1, 0);
+    std::vector<mixed::Args> h[2];
+    for (int p = 0; p < 2; ++p) h[p].resize(n);
+    for (int i = 0; i < n; ++i) {
+        ark_ctx* c = b->box[i];
+        const int lz = box_lz(b, c);
+        start[i + 1] = start[i] + tiles_of(c->geo, b->shape, lz);
+        for (int p = 0; p < 2; ++p)
+            h[p][i] = make_args(c, lz, tuning ? b->scratch_dt + i : &c->S->dt_bits, p);
+    }
+    b->tiles = start[n];
+    ark_status s = ARK_OK;
+    for (int p = 0; p < 2 && s == ARK_OK; ++p)
+        s = b->hip(hipMemcpyAsync(b->args[p], h[p].data(), n * sizeof(mixed::Args), hipMemcpyHostToDevice,
+                                  b->stream), "batch args");
+    if (s == ARK_OK)
+        s = b->hip(hipMemcpyAsync(b->start, start.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream),
+                   "batch tiles");
+    if (s == ARK_OK) s = b->hip(hipStreamSynchronize(b->stream), "batch tables");
+    return s;
+}
+
+ark_status enqueue_step(ark_batch* b, int parity, bool begin) {
+    const int n = (int)b->box.size();
+    if (begin) {
+        hipLaunchKernelGGL(mixed::k_batch_begin_step, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->sptr, n);
+        ark_status s = b->hip(hipGetLastError(), "batch begin step");
+        if (s != ARK_OK) return s;
+    }
+    return b->hip(launch_shape_batch(b->shape, b->quirk, dim3(b->tiles), b->stream, b->args[parity], b->start, n),
+                  "batch step");
+}
+
+ark_status build_graphs(ark_batch* b) {
+    drop_graphs(b);
+    ark_status s = build_tables(b, false);
+    for (int p = 0; p < 2 && s == ARK_OK; ++p) {
+        s = b->hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (s != ARK_OK) break;
+        ark_status se = enqueue_step(b, p, true);
+        s = b->hip(hipStreamEndCapture(b->stream, &b->graph[p]), "end capture");
+        if (se != ARK_OK) s = se;
+        if (s == ARK_OK) s = b->hip(hipGraphInstantiate(&b->exec[p], b->graph[p], nullptr, nullptr, 0), "instantiate");
+    }
+    if (s != ARK_OK) return s;
+    b->gpar_at_build.clear();
+    for (ark_ctx* c : b->box) b->gpar_at_build.push_back(c->gpar[0]);
+    b->parity = 0;
+    b->built = true;
+    return ARK_OK;
+}
+
+// Every box's work so far is done, and every box has the next step's dt as a
+// pending MIN (a box that has not stepped yet gets it from a CFL scan).
+ark_status prepare_boxes(ark_batch* b) {
+    for (ark_ctx* c : b->box) {
+        if (!c->dt_pending) {
+            ark_status s = ark_cfl_scan(c);
+            if (s != ARK_OK) return b->fail(s, std::string("batch: CFL scan of a box: ") + c->err);
+            c->dt_pending = true;
+        }
+        ark_status s = b->hip(hipStreamSynchronize(c->stream), "box stream");
+        if (s != ARK_OK) return s;
+    }
+    return ARK_OK;
+}
+
+// The boxes' streams wait for the batch's work.
+ark_status release_boxes(ark_batch* b) {
+    hipEvent_t ev;
+    ark_status s = b->hip(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+    if (s != ARK_OK) return s;
+    s = b->hip(hipEventRecord(ev, b->stream), "event record");
+    for (ark_ctx* c : b->box)
+        if (s == ARK_OK) s = b->hip(hipStreamWaitEvent(c->stream, ev, 0), "box wait");
+    (void)hipEventDestroy(ev);
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+ark_status ark_batch_create(ark_ctx* const* boxes, int nboxes, ark_batch** out) {
+    if (!boxes || nboxes < 1 || !out) return ARK_ERR_INVALID;
+    *out = nullptr;
+    ark_batch* b = new ark_batch();
+    ark_status s = ARK_OK;
+    for (int i = 0; i < nboxes && s == ARK_OK; ++i) {
+        ark_ctx* c = boxes[i];
+        if (!c) s = b->fail(ARK_ERR_INVALID, "batch: null box");
+        else if (c->prm.precision != ARK_PREC_MIXED)
+            s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes must use mixed precision");
+        else if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)
+            s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes need N1, N2, N3 >= 2");
+        else if (c->has_solid) s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes must be all fluid");
+        else if (i > 0 && (c->prm.device != boxes[0]->prm.device || c->prm.scheme != boxes[0]->prm.scheme))
+            s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes must share the device and the scheme");
+        for (int j = 0; j < i && s == ARK_OK; ++j)
+            if (boxes[j] == c) s = b->fail(ARK_ERR_INVALID, "batch: a box appears twice");
+        if (s == ARK_OK) b->box.push_back(c);
+    }
+    if (s == ARK_OK) {
+        b->device = boxes[0]->prm.device;
+        b->quirk = boxes[0]->prm.scheme == ARK_SCHEME_REFERENCE;
+        s = b->hip(hipSetDevice(b->device), "hipSetDevice");
+    }
+    if (s == ARK_OK) s = b->hip(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+    const size_t n = b->box.size();
+    for (int p = 0; p < 2 && s == ARK_OK; ++p)
+        s = b->hip(hipMalloc(&b->args[p], n * sizeof(mixed::Args)), "hipMalloc batch args");
+    if (s == ARK_OK) s = b->hip(hipMalloc(&b->start, (n + 1) * sizeof(int)), "hipMalloc batch tiles");
+    if (s == ARK_OK) s = b->hip(hipMalloc(&b->sptr, n * sizeof(Scalars*)), "hipMalloc batch scalars");
+    if (s == ARK_OK) s = b->hip(hipMalloc(&b->scratch_dt, n * sizeof(unsigned long long)), "hipMalloc batch dt");
+    if (s == ARK_OK) {
+        std::vector<Scalars*> sp;
+        for (ark_ctx* c : b->box) sp.push_back(c->S);
+        s = b->hip(hipMemcpy(b->sptr, sp.data(), n * sizeof(Scalars*), hipMemcpyHostToDevice), "batch scalars");
+    }
+    if (s != ARK_OK) {
+        ark_batch_destroy(b);
+        return s;
+    }
+    *out = b;
+    return ARK_OK;
+}
+
+void ark_batch_destroy(ark_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    for (int p = 0; p < 2; ++p)
+        if (b->args[p]) (void)hipFree(b->args[p]);
+    if (b->start) (void)hipFree(b->start);
+    if (b->sptr) (void)hipFree(b->sptr);
+    if (b->scratch_dt) (void)hipFree(b->scratch_dt);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* ark_batch_last_error(const ark_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+ark_status ark_batch_set_config(ark_batch* b, int shape, int lz) {
+    if (!b || shape < 0 || shape >= kShapes || lz < 1) return ARK_ERR_INVALID;
+    b->shape = shape;
+    b->lz_target = lz;
+    drop_graphs(b);
+    return ARK_OK;
+}
+
+ark_status ark_batch_get_config(const ark_batch* b, int* shape, int* lz, int* tiles, int* threads) {
+    if (!b) return ARK_ERR_INVALID;
+    if (shape) *shape = b->shape;
+    if (lz) *lz = b->lz_target;
+    if (tiles) *tiles = b->tiles;
+    if (threads) *threads = kShapeThreads[b->shape];
+    return ARK_OK;
+}
+
+ark_status ark_batch_autotune(ark_batch* b, int reps, double* ms_out, int max_out) {
+    if (!b || reps < 1) return ARK_ERR_INVALID;
+    ark_status s = b->hip(hipSetDevice(b->device), "hipSetDevice");
+    if (s == ARK_OK) s = prepare_boxes(b);
+    if (s != ARK_OK) return s;
+    const int lzs[3] = {16, 32, 64};
+    b->tune_ms.assign(kShapes * 3, 0.0);
+    hipEvent_t e0, e1;
+    s = b->hip(hipEventCreate(&e0), "event");
+    if (s == ARK_OK) s = b->hip(hipEventCreate(&e1), "event");
+    if (s != ARK_OK) return s;
+    double best = 1e300;
+    int bs = kDefaultShape, bl = 32;
+    drop_graphs(b);
+    for (int sh = 0; sh < kShapes && s == ARK_OK; ++sh) {
+        for (int l = 0; l < 3 && s == ARK_OK; ++l) {
+            b->shape = sh;
+            b->lz_target = lzs[l];
+            // Tuning launches write only the boxes' other buffers (never
+            // swapped in) and their dt MINs go to scratch slots: the boxes'
+            // state is left as it was.
+            s = build_tables(b, true);
+            if (s == ARK_OK) s = enqueue_step(b, 0, false);
+            if (s == ARK_OK) s = b->hip(hipEventRecord(e0, b->stream), "event");
+            for (int r = 0; r < reps && s == ARK_OK; ++r) s = enqueue_step(b, 0, false);
+            if (s == ARK_OK) s = b->hip(hipEventRecord(e1, b->stream), "event");
+            if (s == ARK_OK) s = b->hip(hipEventSynchronize(e1), "event sync");
+            float ms = 0.f;
+            if (s == ARK_OK) s = b->hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            const double per = (double)ms / reps;
+            b->tune_ms[sh * 3 + l] = per;
+            if (s == ARK_OK && per < best) {
+                best = per;
+                bs = sh;
+                bl = lzs[l];
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->shape = bs;
+    b->lz_target = bl;
+    if (ms_out)
+        for (int q = 0; q < max_out && q < (int)b->tune_ms.size(); ++q) ms_out[q] = b->tune_ms[q];
+    return s;
+}
+
+ark_status ark_batch_step(ark_batch* b, int nsteps) {
+    if (!b || nsteps < 0) return ARK_ERR_INVALID;
+    if (nsteps == 0) return ARK_OK;
+    ark_status s = b->hip(hipSetDevice(b->device), "hipSetDevice");
+    if (s == ARK_OK) s = prepare_boxes(b);
+    if (s != ARK_OK) return s;
+    // The tables hold buffer addresses: rebuild them when a box has stepped
+    // on its own since (its buffers swapped), or when the batch is new.
+    bool same = b->built;
+    for (size_t i = 0; same && i < b->box.size(); ++i) {
+        const int expect = b->gpar_at_build[i] ^ b->parity;
+        same = b->box[i]->gpar[0] == expect;
+    }
+    if (!same) s = build_graphs(b);
+    for (int n = 0; n < nsteps && s == ARK_OK; ++n) {
+        s = b->hip(hipGraphLaunch(b->exec[b->parity], b->stream), "graph launch");
+        b->parity ^= 1;
+        for (ark_ctx* c : b->box)
+            for (int g = 0; g < 4; ++g) ark_swap_group(c, g);
+    }
+    if (s == ARK_OK) s = release_boxes(b);
+    return s;
+}
+
+ark_status ark_batch_synchronize(ark_batch* b) {
+    if (!b) return ARK_ERR_INVALID;
+    return b->hip(hipStreamSynchronize(b->stream), "batch sync");
+}
+
+}  // extern "C"

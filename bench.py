@@ -15,6 +15,8 @@ Printed: ONE JSON line.  Beyond the common fields it carries
                 timed with HIP events on the library's stream
   accuracy      mixed vs FP64 after the same steps, per field (corrected
                 scheme: the reference scheme goes unstable on this vortex)
+  batched       64 independent boxes of uneven size and Reynolds number in one
+                launch per step (graph-replayed, tile shape autotuned)
   cpu_baseline  the unmodified reference (oracle/_ref, built from its own
                 sources) under mpirun on this host's cores, bounded sample
   --calibrate-copy prints the HBM bytes a plain device copy of known size
@@ -136,6 +138,41 @@ def phase_kernels(N, steps):
     return out
 
 
+def batched(nboxes, steps, seed=4):
+    """Many independent boxes in one launch per step: cubes with edges drawn
+    from {32, 48, 64, 96, 128}, Re uniform in [400, 3200] (nu = U0 L / Re with
+    the vortex scale L = 1), Taylor-Green initial state, mixed precision,
+    tile shape autotuned.  Aggregate Mcell-steps/s over all boxes."""
+    import numpy as np
+    import arkamd
+    rng = np.random.default_rng(seed)
+    edges = rng.choice([32, 48, 64, 96, 128], size=nboxes)
+    res = rng.uniform(400.0, 3200.0, size=nboxes)
+    boxes = [arkamd.Solver(int(e), int(e), int(e), VIS=1.0 / float(r), precision=arkamd.PREC_MIXED)
+             for e, r in zip(edges, res)]
+    for s in boxes:
+        s.init_tgv(1.0)
+    cells = int(sum(int(e) ** 3 for e in edges))
+    with arkamd.Batch(boxes) as b:
+        tune = b.autotune(reps=3)
+        cfg = b.config()
+        b.step(2)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.synchronize()
+        el = time.perf_counter() - t0
+    nonfinite = sum(s.diagnostics(allow_nonfinite=True)["nonfinite"] for s in boxes)
+    for s in boxes:
+        s.close()
+    return {"boxes": nboxes, "cells": cells, "steps": steps, "edges_hist": {str(e): int((edges == e).sum())
+                                                                             for e in (32, 48, 64, 96, 128)},
+            "re_range": [round(float(res.min()), 1), round(float(res.max()), 1)],
+            "value": round(cells * steps / el / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el * 1e3 / steps, 4),
+            "config": cfg, "autotune_ms": {f"{k[0]}/lz{k[1]}": round(v, 4) for k, v in tune.items()},
+            "nonfinite": nonfinite}
+
+
 def accuracy(N, steps):
     import numpy as np
     import arkamd
@@ -199,6 +236,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--accuracy-steps", type=int, default=500)
+    ap.add_argument("--batch-boxes", type=int, default=64, help="boxes of the batched run (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--phase-steps", type=int, default=3,
                     help="steps of the phase-by-phase FP64 path timed per kernel (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=10)
@@ -257,6 +296,8 @@ def main():
     line["device_bytes"] = {p: r["device_bytes"] for p, r in runs.items()}
     if args.phase_steps > 0 and not args.only:
         line["phase_kernels_fp64"] = phase_kernels(N, args.phase_steps)
+    if args.batch_boxes > 0 and not args.only:
+        line["batched"] = batched(args.batch_boxes, args.batch_steps)
     if args.accuracy_steps > 0 and not args.only:
         line["accuracy"] = accuracy(N, args.accuracy_steps)
     base = None

@@ -194,6 +194,30 @@ ark_status ark_set_plane(ark_ctx* ctx, int field, int axis, int q, const double*
  * ro(i, j, 1), as the reference does.  Enqueued on the context's stream. */
 ark_status ark_boundary_conditions(ark_ctx* ctx, int axis, const int fields[5], int periodic, double t0);
 
+/* Batched steps (no counterpart in the reference): many independent boxes,
+ * each an ordinary mixed-precision context of its own size and parameters
+ * (same device and scheme), advanced together by ONE kernel launch per step
+ * whose workgroups cover the tiles of every box.  The steps are replayed from
+ * HIP graphs captured at the first ark_batch_step (and again whenever a box
+ * has stepped on its own since).  Every box evolves exactly as ark_step on
+ * its own context would evolve it.  The boxes stay usable through their own
+ * contexts between batch calls; the batch orders its work after theirs and
+ * theirs after its own.  A context must outlive the batches that hold it. */
+typedef struct ark_batch ark_batch;
+ark_status ark_batch_create(ark_ctx* const* boxes, int nboxes, ark_batch** out);
+void ark_batch_destroy(ark_batch* batch);
+const char* ark_batch_last_error(const ark_batch* batch);
+ark_status ark_batch_step(ark_batch* batch, int nsteps);
+ark_status ark_batch_synchronize(ark_batch* batch);
+/* Times every (tile shape, chunk length) candidate on the batch's own boxes,
+ * reps launches each, and keeps the fastest.  Tile shapes: 0 = 30x6 owned
+ * (256 threads), 1 = 30x14 (512), 2 = 62x6 (512); chunk lengths 16, 32, 64
+ * planes.  ms_out[shape*3 + lz_index] receives each candidate's ms per step
+ * (max_out entries).  The boxes' state is not changed. */
+ark_status ark_batch_autotune(ark_batch* batch, int reps, double* ms_out, int max_out);
+ark_status ark_batch_set_config(ark_batch* batch, int shape, int lz);
+ark_status ark_batch_get_config(const ark_batch* batch, int* shape, int* lz, int* tiles, int* threads);
+
 /* Bytes of device memory held by the context. */
 ark_status ark_device_bytes(const ark_ctx* ctx, int64_t* bytes);
 

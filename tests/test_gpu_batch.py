@@ -1,0 +1,98 @@
+"""GPU: batched steps (one launch per step over many boxes, replayed from HIP
+graphs, tile shape chosen by the autotuner).  Every box must evolve bit for
+bit as its own standalone mixed-precision context does."""
+import numpy as np
+import pytest
+
+import arkamd
+
+pytestmark = pytest.mark.gpu
+
+STATE = ["ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon",
+         "u11", "u21", "u31", "p1", "t1", "u12", "u22", "u32", "p2", "t2", "u13", "u23", "u33", "p3", "t3"]
+# uneven boxes and Reynolds numbers
+BOXES = [((20, 16, 12), 1 / 400), ((32, 32, 32), 1 / 1600), ((24, 40, 16), 1 / 3200), ((48, 36, 30), 1 / 900)]
+
+
+def make(boxes=BOXES, U0=1.0):
+    out = []
+    for N, vis in boxes:
+        s = arkamd.Solver(*N, VIS=vis, precision=arkamd.PREC_MIXED)
+        s.init_tgv(U0)
+        out.append(s)
+    return out
+
+
+def assert_same(a, b, what):
+    for sa, sb in zip(a, b):
+        for n in STATE:
+            x, y = sa.download(n), sb.download(n)
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64)), (what, sa.N, n)
+        da, db = sa.diagnostics(), sb.diagnostics()
+        assert da["dt"] == db["dt"] and da["nstep"] == db["nstep"] and da["time"] == db["time"], (what, sa.N)
+
+
+def test_batch_equals_standalone():
+    ref, bat = make(), make()
+    for s in ref:
+        s.step(7)
+    with arkamd.Batch(bat) as b:
+        b.step(7)
+        cfg = b.config()
+    assert cfg["tiles"] > 0
+    assert_same(ref, bat, "batch vs standalone")
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("lz", [16, 64])
+def test_batch_result_independent_of_tiling(shape, lz):
+    ref, bat = make(BOXES[:2]), make(BOXES[:2])
+    for s in ref:
+        s.step(5)
+    with arkamd.Batch(bat) as b:
+        b.set_config(shape, lz)
+        b.step(5)
+    assert_same(ref, bat, f"shape {shape} lz {lz}")
+
+
+def test_batch_graph_replays_with_eager_work_between():
+    """Replays of the captured graphs, with the boxes' own eager work (a
+    standalone step, which swaps its buffers and forces new tables, and
+    downloads) between them, each checked against the standalone result."""
+    ref, bat = make(BOXES[:3]), make(BOXES[:3])
+    with arkamd.Batch(bat) as b:
+        for k, (nb, solo) in enumerate([(3, False), (2, True), (1, False), (4, True), (2, False)]):
+            b.step(nb)
+            for s in ref:
+                s.step(nb)
+            assert_same(ref, bat, f"replay {k}")
+            if solo:               # one box steps alone between batch calls
+                bat[1].step(1)
+                ref[1].step(1)
+                bat[0].diagnostics()
+
+
+def test_batch_autotune_leaves_state_unchanged():
+    ref, bat = make(BOXES[:3]), make(BOXES[:3])
+    for s in ref:
+        s.step(2)
+    with arkamd.Batch(bat) as b:
+        b.step(1)
+        ms = b.autotune(reps=2)
+        assert len(ms) == 9 and all(v > 0 for v in ms.values())
+        best = min(ms, key=ms.get)
+        cfg = b.config()
+        assert (cfg["shape"], cfg["lz"]) == best
+        b.step(1)
+    assert_same(ref, bat, "after autotune")
+
+
+def test_batch_rejects_bad_boxes():
+    f64 = arkamd.Solver(16, 16, 16)
+    m = arkamd.Solver(16, 16, 16, precision=arkamd.PREC_MIXED)
+    with pytest.raises(arkamd.ArkError):
+        arkamd.Batch([m, f64])
+    with pytest.raises(arkamd.ArkError):
+        arkamd.Batch([m, m])
+    f64.close()
+    m.close()
