@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = [
     "ark_profile_reset", "ark_get_plane", "ark_set_plane", "ark_boundary_conditions", "ark_device_bytes",
     "ark_batch_create", "ark_batch_destroy", "ark_batch_last_error", "ark_batch_step", "ark_batch_synchronize",
     "ark_batch_autotune", "ark_batch_set_config", "ark_batch_get_config",
+    "ark_init_turbulence", "ark_spectrum", "ark_dissipation", "ark_set_forcing", "ark_filter_box",
 ]
 
 
@@ -136,6 +137,12 @@ def lib() -> ctypes.CDLL:
     L.ark_batch_autotune.argtypes = [vp, ip, dp, ip]
     L.ark_batch_set_config.argtypes = [vp, ip, ip]
     L.ark_batch_get_config.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 4
+    u64 = ctypes.c_ulonglong
+    L.ark_init_turbulence.argtypes = [vp, u64, ctypes.c_double, ctypes.c_double]
+    L.ark_spectrum.argtypes = [vp, ip, dp]
+    L.ark_dissipation.argtypes = [vp, dp]
+    L.ark_set_forcing.argtypes = [vp, u64, ctypes.c_double]
+    L.ark_filter_box.argtypes = [vp, ip, dp, dp, dp]
     for fn in EXPORTED_SYMBOLS:
         if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error"):
             getattr(L, fn).restype = ctypes.c_int
@@ -289,6 +296,34 @@ class Solver:
         s = ctypes.c_void_p()
         self._check(self._L.ark_get_stream(self._h, ctypes.byref(s)))
         return s.value or 0
+
+    # ---- homogeneous turbulence ------------------------------------------------
+    def init_turbulence(self, seed: int, k0: float = 4.0, urms: float = 1.0) -> None:
+        self._check(self._L.ark_init_turbulence(self._h, int(seed), float(k0), float(urms)))
+
+    def spectrum(self, nbins: Optional[int] = None) -> np.ndarray:
+        nb = nbins or (int(np.ceil(np.sqrt(3) * max(self.N) / 2)) + 2)
+        out = np.zeros(nb)
+        self._check(self._L.ark_spectrum(self._h, nb, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def dissipation(self) -> float:
+        v = ctypes.c_double()
+        self._check(self._L.ark_dissipation(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_forcing(self, seed: int, eps: float) -> None:
+        self._check(self._L.ark_set_forcing(self._h, int(seed), float(eps)))
+
+    def filter_box(self, width: int):
+        """(ubar[3], tau[6], grad[9]) on the coarse grid, numpy (Nc3, Nc2, Nc1)."""
+        N1, N2, N3 = self.N
+        shp = (N3 // width, N2 // width, N1 // width)
+        ub, tau, gr = np.zeros((3,) + shp), np.zeros((6,) + shp), np.zeros((9,) + shp)
+        P = ctypes.POINTER(ctypes.c_double)
+        self._check(self._L.ark_filter_box(self._h, int(width), ub.ctypes.data_as(P), tau.ctypes.data_as(P),
+                                           gr.ctypes.data_as(P)))
+        return ub, tau, gr
 
     # ---- halo planes and BoundaryConditions -----------------------------------
     def plane_shape(self, axis: int) -> tuple:

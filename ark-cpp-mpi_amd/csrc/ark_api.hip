@@ -642,8 +642,15 @@ ark_status ark_init_tgv(ark_ctx* c, double U0) {
     double amp = c->prm.ro0 * U0 * U0 / 16.0;
     if (s == ARK_OK) s = c->hip(launch_init_tgv(c->stream, c->geo, c->fields(), F2, T, U0, amp, c->K), "init_tgv");
     if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
-    hipFree(dtab);
-    if (s == ARK_OK && c->prm.precision == ARK_PREC_MIXED) {
+    (void)hipFree(dtab);
+    return s == ARK_OK ? ark_finish_init(c) : s;
+}
+
+// After a new initial state in the FP64 layout: FP32 face storage in mixed
+// precision, state flags, TIME = 0, nStep = 0, running MIN reset.
+ark_status ark_finish_init(ark_ctx* c) {
+    ark_status s = ARK_OK;
+    if (c->prm.precision == ARK_PREC_MIXED) {
         // FP32 face storage: convert each face field (both buffers hold it)
         for (int g = 1; g < 4 && s == ARK_OK; ++g)
             for (int v = 0; v < 5 && s == ARK_OK; ++v) {

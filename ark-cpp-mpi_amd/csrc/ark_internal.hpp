@@ -33,6 +33,13 @@ struct ark_ctx {
     double* diag_part = nullptr;
     double* staging = nullptr;         // one FP64 field (mixed precision conversions)
     double* plane_buf = nullptr;       // one interior plane (ark_get_plane / ark_set_plane)
+    // body force of the mixed-precision step (ark_set_forcing): acceleration
+    // components at force + m*force_stride in the field index map, FP32
+    float* force = nullptr;
+    float* force_table = nullptr;      // this step's 16 forced modes
+    long long force_stride = 0;
+    unsigned long long force_seed = 0;
+    double force_eps = 0.0;
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;
@@ -57,6 +64,8 @@ struct ark_ctx {
 // Step building blocks shared by the drivers (ark_api.hip; internal, not in ark.h).
 extern "C" {
 ark_status ark_cfl_scan(ark_ctx* c);                    // running MIN of the CFL bound over Con
+ark_status ark_finish_init(ark_ctx* c);                 // after an initial state: FP32 faces, flags, TIME = 0
+ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st);   // this step's forcing field (mixed precision)
 ark_status ark_cfl_step(ark_ctx* c, int advance);       // TimeStepSize on Con, then dt := MIN
 ark_status ark_cfl_finalize(ark_ctx* c, int mode);      // see k_cfl_finalize
 ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pending MIN or Con), TIME, nStep

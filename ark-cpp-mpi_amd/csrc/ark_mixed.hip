@@ -75,6 +75,11 @@ struct Args {
     const Scalars* S;
     unsigned long long* dt_next;
     int LZ;
+    // body acceleration (forcing, FP32, components force + m*fstride in the
+    // field index map) or null; the cell force is rho dvc F
+    const float* force;
+    long long fstride;
+    float dvcf;
 };
 
 __device__ __forceinline__ double ldd(const char* grp, unsigned long long fb, int v, int idx) {
@@ -264,6 +269,7 @@ __device__ __forceinline__ Side make_side(const CellF& ac, const CellF& h, const
 struct PlaneIn {
     FaceF xw, ys, zt, ex;
     double ou[3];
+    float fo[3];    // body acceleration at the lane's cell (forcing)
 };
 
 // One tile (tile indices tbx, tby, tbz) of one box.  At most 256 VGPRs: two
@@ -329,6 +335,9 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         r.ou[0] = r.ou[1] = r.ou[2] = 0.0;
         if (has_o)
             for (int m = 0; m < 3; ++m) r.ou[m] = ldd(A.A, A.fb, 1 + m, o_off + zo);
+        r.fo[0] = r.fo[1] = r.fo[2] = 0.f;
+        if (A.force)
+            for (int m = 0; m < 3; ++m) r.fo[m] = A.force[m * A.fstride + base + zo];
         return r;
     };
     auto put_face = [&](float* f, int q, const FaceF& r) {
@@ -411,7 +420,8 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
                 const float syl = A.VIS * (float)(uc - uS) * A.inv_dx[1];
                 const float syh = A.VIS * (float)(uN - uc) * A.inv_dx[1];
                 const float szh = A.VIS * (float)(Anext.u[m] - uc) * A.inv_dx[2];
-                f[m] = ((sxh - sxl) * A.ds[0] + (syh - syl) * A.ds[1] + (szh - sz_lo[m]) * A.ds[2]) * roc;
+                f[m] = ((sxh - sxl) * A.ds[0] + (syh - syl) * A.ds[1] + (szh - sz_lo[m]) * A.ds[2]) * roc +
+                       roc * A.dvcf * cur.fo[m];
                 sz_lo[m] = szh;
             }
         }
@@ -676,6 +686,9 @@ mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, int
     a.S = c->S;
     a.dt_next = dt_next;
     a.LZ = LZ;
+    a.force = c->force_eps > 0.0 ? c->force : nullptr;
+    a.fstride = c->force_stride;
+    a.dvcf = (float)c->K.dvc;
     return a;
 }
 
@@ -730,6 +743,7 @@ ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     MixedLaunch ml{c, choose_lz_mixed(c->geo, kDefaultShape)};
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
+        if (s == ARK_OK) s = ark_turb_force_step(c, c->stream);
         if (s == ARK_OK) s = c->launch("fused_mixed", run_mixed, &ml);
         if (s != ARK_OK) break;
         for (int g = 0; g < 4; ++g) ark_swap_group(c, g);
@@ -849,6 +863,8 @@ ark_status enqueue_step(ark_batch* b, int parity, bool begin) {
     if (begin) {
         hipLaunchKernelGGL(mixed::k_batch_begin_step, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->sptr, n);
         ark_status s = b->hip(hipGetLastError(), "batch begin step");
+        for (ark_ctx* c : b->box)
+            if (s == ARK_OK) s = ark_turb_force_step(c, b->stream);
         if (s != ARK_OK) return s;
     }
     return b->hip(launch_shape_batch(b->shape, b->quirk, dim3(b->tiles), b->stream, b->args[parity], b->start, n),

@@ -1,0 +1,588 @@
+// Homogeneous turbulence on the periodic box (no counterpart in the
+// reference, which solves a channel; these build and measure the forced
+// turbulence runs the learned closure is trained on):
+//   * a seeded solenoidal initial velocity with a prescribed spectrum,
+//   * stochastic random-phase forcing on the lowest wavenumber shells,
+//   * the shell-summed kinetic-energy spectrum and the dissipation rate,
+//   * a w-cell box filter with the subgrid stresses and the filtered
+//     velocity gradients (closure training pairs).
+// FFTs are hipFFT double-precision real-to-complex transforms of the
+// interior N3 x N2 x N1 cells (x fastest).  Random numbers are Philox4x32-10,
+// a pure function of (seed, counter): results do not depend on the launch.
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ark/ark.h"
+#include "ark_internal.hpp"
+#include "ark_kernels.hpp"
+#include "ark_rng.hpp"
+
+using namespace arkd;
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+struct Spec {
+    int N1, N2, N3;     // interior cells
+    int H1;             // N1/2 + 1: complex extent of the fastest axis
+    double L[3];        // box lengths
+};
+
+__device__ __forceinline__ int wave_index(int q, int N) { return q <= N / 2 ? q : q - N; }
+
+// Gaussian white noise, three components, contiguous N3*N2*N1 each.
+__global__ void k_noise(Spec s, double* out, unsigned long long seed) {
+    const long long n = (long long)s.N1 * s.N2 * s.N3;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        double g[4];
+        gauss4(seed, (uint32_t)q, (uint32_t)(q >> 32), 0x7a3c51u, 0u, g);
+        out[q] = g[0];
+        out[n + q] = g[1];
+        out[2 * n + q] = g[2];
+    }
+}
+
+// Spectral shaping of the three transformed components in place: amplitude
+// |k| exp(-(|k|/k0)^2) per mode, so that the shell energy of white noise
+// (proportional to the 4 pi k^2 modes of a shell) becomes
+// E(k) ~ k^4 exp(-2 (k/k0)^2); projection on the plane normal to k (a
+// solenoidal field); the mean and the Nyquist modes are zeroed.
+__global__ void k_shape(Spec s, hipfftDoubleComplex* u, double k0) {
+    const long long nc = (long long)s.H1 * s.N2 * s.N3;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % s.H1), j = (int)((q / s.H1) % s.N2), k = (int)(q / ((long long)s.H1 * s.N2));
+        const int m1 = i, m2 = wave_index(j, s.N2), m3 = wave_index(k, s.N3);
+        const double kv[3] = {kTwoPi / s.L[0] * m1, kTwoPi / s.L[1] * m2, kTwoPi / s.L[2] * m3};
+        const double kk = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+        const bool nyq = (2 * m1 == s.N1) || (2 * j == s.N2) || (2 * k == s.N3);
+        hipfftDoubleComplex* c[3] = {u + q, u + nc + q, u + 2 * nc + q};
+        if (kk == 0.0 || nyq) {
+            for (int d = 0; d < 3; ++d) c[d]->x = c[d]->y = 0.0;
+            continue;
+        }
+        const double km = sqrt(kk);
+        const double amp = km * exp(-(km / k0) * (km / k0));
+        double re[3], im[3];
+        for (int d = 0; d < 3; ++d) {
+            re[d] = c[d]->x;
+            im[d] = c[d]->y;
+        }
+        const double pr = (kv[0] * re[0] + kv[1] * re[1] + kv[2] * re[2]) / kk;
+        const double pi = (kv[0] * im[0] + kv[1] * im[1] + kv[2] * im[2]) / kk;
+        for (int d = 0; d < 3; ++d) {
+            c[d]->x = amp * (re[d] - pr * kv[d]);
+            c[d]->y = amp * (im[d] - pi * kv[d]);
+        }
+    }
+}
+
+// sum over the full spectrum of |u_hat|^2 (three components), counting the
+// modes the half spectrum stands for twice: N^3 * sum_x |u|^2 (Parseval).
+__global__ void k_spec_energy(Spec s, const hipfftDoubleComplex* u, double* out) {
+    const long long nc = (long long)s.H1 * s.N2 * s.N3;
+    double acc = 0.0;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % s.H1);
+        const double w = (i == 0 || 2 * i == s.N1) ? 1.0 : 2.0;
+        for (int d = 0; d < 3; ++d) {
+            const hipfftDoubleComplex v = u[d * nc + q];
+            acc += w * (v.x * v.x + v.y * v.y);
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+// dst = src * exp(-i k_d dx_d / 2): the field sampled half a cell back along
+// axis d, i.e. on the faces normal to d (face q lies dx_d/2 below cell q).
+__global__ void k_shift(Spec s, const hipfftDoubleComplex* src, hipfftDoubleComplex* dst, int d) {
+    const long long nc = (long long)s.H1 * s.N2 * s.N3;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % s.H1), j = (int)((q / s.H1) % s.N2), k = (int)(q / ((long long)s.H1 * s.N2));
+        const int m = d == 0 ? i : d == 1 ? wave_index(j, s.N2) : wave_index(k, s.N3);
+        const int Nd = d == 0 ? s.N1 : d == 1 ? s.N2 : s.N3;
+        const double ph = -0.5 * kTwoPi * m / Nd;
+        const double cr = cos(ph), ci = sin(ph);
+        const hipfftDoubleComplex v = src[q];
+        dst[q].x = v.x * cr - v.y * ci;
+        dst[q].y = v.x * ci + v.y * cr;
+    }
+}
+
+// Field (cell or face extents, periodic ghosts) from an interior array
+// src[(k-1)*N1*N2 + (j-1)*N1 + (i-1)] times scale plus offset; src may be
+// null (constant).  Every index wraps to 1..N: face N+1 is face 1.
+__global__ void k_scatter(Geo g, double* dst, const double* src, double scale, double offset, int ext) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= g.N1 + ext || j >= g.N2 + ext || k >= g.N3 + ext) return;
+    const int iw = ((i - 1) % g.N1 + g.N1) % g.N1, jw = ((j - 1) % g.N2 + g.N2) % g.N2,
+              kw = ((k - 1) % g.N3 + g.N3) % g.N3;
+    const double v = src ? src[((size_t)kw * g.N2 + jw) * g.N1 + iw] : 0.0;
+    dst[at(g, i, j, k)] = v * scale + offset;
+}
+
+// Interior of a cell field into a contiguous array.
+__global__ void k_extract(Geo g, const double* f, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= g.N1) return;
+    out[((size_t)k * g.N2 + j) * g.N1 + i] = f[at(g, i + 1, j + 1, k + 1)];
+}
+
+// Shell sums of 1/2 |u_hat|^2 / (N^3)^2 over integer-index shells
+// round(|m|): E[b] summed over b equals 1/2 <u.u> (mean over cells).
+__global__ void k_shell_sum(Spec s, const hipfftDoubleComplex* u, int nb, double* E) {
+    extern __shared__ double bins[];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0.0;
+    __syncthreads();
+    const long long nc = (long long)s.H1 * s.N2 * s.N3;
+    const double n3 = (double)s.N1 * s.N2 * s.N3;
+    const double norm = 0.5 / (n3 * n3);
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % s.H1), j = (int)((q / s.H1) % s.N2), k = (int)(q / ((long long)s.H1 * s.N2));
+        const int m1 = i, m2 = wave_index(j, s.N2), m3 = wave_index(k, s.N3);
+        const int b = (int)(sqrt((double)(m1 * m1 + m2 * m2 + m3 * m3)) + 0.5);
+        if (b >= nb) continue;
+        const double w = (i == 0 || 2 * i == s.N1) ? 1.0 : 2.0;
+        double e = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const hipfftDoubleComplex v = u[d * nc + q];
+            e += v.x * v.x + v.y * v.y;
+        }
+        atomicAdd(&bins[b], w * norm * e);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        if (bins[b] != 0.0) atomicAdd(&E[b], bins[b]);
+}
+
+// nu * <sum_ij (d_j u_i)^2> with central differences of the level-n cells.
+__global__ void k_dissipation(Geo g, const double* u1, const double* u2, const double* u3, double inv2dx0,
+                              double inv2dx1, double inv2dx2, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    double acc = 0.0;
+    if (i <= g.N1) {
+        const int ip = i == g.N1 ? 1 : i + 1, im = i == 1 ? g.N1 : i - 1;
+        const int jp = j == g.N2 ? 1 : j + 1, jm = j == 1 ? g.N2 : j - 1;
+        const int kp = k == g.N3 ? 1 : k + 1, km = k == 1 ? g.N3 : k - 1;
+        const double* u[3] = {u1, u2, u3};
+        for (int m = 0; m < 3; ++m) {
+            const double dx = (u[m][at(g, ip, j, k)] - u[m][at(g, im, j, k)]) * inv2dx0;
+            const double dy = (u[m][at(g, i, jp, k)] - u[m][at(g, i, jm, k)]) * inv2dx1;
+            const double dz = (u[m][at(g, i, j, kp)] - u[m][at(g, i, j, km)]) * inv2dx2;
+            acc += dx * dx + dy * dy + dz * dz;
+        }
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Forcing.  The forced wavevectors are the integer m with 1 <= |m|^2 <= 4
+// whose first non-zero component is positive (16 of the 32; the other half
+// are their conjugates).  Each step, mode n gets a random complex amplitude
+// vector a_n normal to its k, of norm A = sqrt(eps / (dt * 16)), drawn from
+// (seed, step, n); the acceleration field F(x) = sum_n 2 Re(a_n e^{ik.x}) then
+// has <|F|^2> = 32 A^2, and a white-in-time force injects on average
+// <|F|^2> dt / 2 = eps per unit mass (the force is independent of the
+// velocity, so <u.F> averages to zero).
+struct ForceMode { int m[3]; };
+
+__constant__ ForceMode c_modes[16];
+
+// 16 modes x (re, im) x 3 components.
+__global__ void k_force_modes(const Scalars* S, unsigned long long seed, double eps, double L0, double L1, double L2,
+                              float* table) {
+    const int n = threadIdx.x;
+    if (n >= 16) return;
+    const double dt = S->dt;
+    const uint32_t step = (uint32_t)S->nstep;
+    double g[8];
+    gauss4(seed, (uint32_t)n, step, 0xf0c3u, 0u, g);
+    gauss4(seed, (uint32_t)n, step, 0xf0c3u, 1u, g + 4);
+    const double kv[3] = {kTwoPi / L0 * c_modes[n].m[0], kTwoPi / L1 * c_modes[n].m[1], kTwoPi / L2 * c_modes[n].m[2]};
+    const double kk = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+    double re[3] = {g[0], g[1], g[2]}, im[3] = {g[3], g[4], g[5]};
+    const double pr = (kv[0] * re[0] + kv[1] * re[1] + kv[2] * re[2]) / kk;
+    const double pi = (kv[0] * im[0] + kv[1] * im[1] + kv[2] * im[2]) / kk;
+    double nrm = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        re[d] -= pr * kv[d];
+        im[d] -= pi * kv[d];
+        nrm += re[d] * re[d] + im[d] * im[d];
+    }
+    const double A = (dt > 0.0 && nrm > 0.0) ? sqrt(eps / (dt * 16.0)) / sqrt(nrm) : 0.0;
+    for (int d = 0; d < 3; ++d) {
+        table[n * 6 + 2 * d] = (float)(A * re[d]);
+        table[n * 6 + 2 * d + 1] = (float)(A * im[d]);
+    }
+}
+
+// F at the cell centres x = (i - 1/2) dx, three FP32 components at force +
+// m*stride in the field index map.
+__global__ void k_force_field(Geo g, const float* table, double dx0, double dx1, double dx2, double L0, double L1,
+                              double L2, float* force, long long stride) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    __shared__ float tab[96];
+    for (int q = threadIdx.x; q < 96; q += blockDim.x) tab[q] = table[q];
+    __syncthreads();
+    if (i > g.N1) return;
+    const double x[3] = {(i - 0.5) * dx0, (j - 0.5) * dx1, (k - 0.5) * dx2};
+    const double Ls[3] = {L0, L1, L2};
+    // e^{i m x_d} for m = 0, 1, 2 along each axis
+    float cr[3][3], ci[3][3];
+    for (int d = 0; d < 3; ++d)
+        for (int m = 0; m < 3; ++m) {
+            const double ph = kTwoPi / Ls[d] * m * x[d];
+            cr[d][m] = (float)cos(ph);
+            ci[d][m] = (float)sin(ph);
+        }
+    float F[3] = {0.f, 0.f, 0.f};
+    for (int n = 0; n < 16; ++n) {
+        float er = 1.f, ei = 0.f;
+        for (int d = 0; d < 3; ++d) {
+            const int m = c_modes[n].m[d];
+            const float r = cr[d][m < 0 ? -m : m], s = m < 0 ? -ci[d][-m] : ci[d][m];
+            const float nr = er * r - ei * s, ni = er * s + ei * r;
+            er = nr;
+            ei = ni;
+        }
+        for (int d = 0; d < 3; ++d) F[d] += 2.f * (tab[n * 6 + 2 * d] * er - tab[n * 6 + 2 * d + 1] * ei);
+    }
+    const int c = at(g, i, j, k);
+    force[c] = F[0];
+    force[stride + c] = F[1];
+    force[2 * stride + c] = F[2];
+}
+
+// ---------------------------------------------------------------------------
+// Box filter of width w cells (w divides N): per coarse cell the mean of u_i
+// and of u_i u_j over its w^3 fine cells (FP64 sums), giving the filtered
+// velocity and the subgrid stress tau_ij = <u_i u_j> - <u_i><u_j>
+// (ij = 11, 22, 33, 12, 13, 23), coarse arrays contiguous (x fastest).
+__global__ void k_box_filter(Geo g, const double* u1, const double* u2, const double* u3, int w, double* ubar,
+                             double* tau) {
+    const int Nc1 = g.N1 / w, Nc2 = g.N2 / w, Nc3 = g.N3 / w;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y, K = blockIdx.z;
+    if (I >= Nc1) return;
+    double s[3] = {0, 0, 0}, p[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < w; ++c)
+        for (int b = 0; b < w; ++b)
+            for (int a = 0; a < w; ++a) {
+                const int q = at(g, I * w + a + 1, J * w + b + 1, K * w + c + 1);
+                const double v[3] = {u1[q], u2[q], u3[q]};
+                for (int m = 0; m < 3; ++m) s[m] += v[m];
+                p[0] += v[0] * v[0];
+                p[1] += v[1] * v[1];
+                p[2] += v[2] * v[2];
+                p[3] += v[0] * v[1];
+                p[4] += v[0] * v[2];
+                p[5] += v[1] * v[2];
+            }
+    const double inv = 1.0 / ((double)w * w * w);
+    const long long nc = (long long)Nc1 * Nc2 * Nc3, o = ((long long)K * Nc2 + J) * Nc1 + I;
+    double m[3];
+    for (int d = 0; d < 3; ++d) {
+        m[d] = s[d] * inv;
+        ubar[d * nc + o] = m[d];
+    }
+    const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; ++q) tau[q * nc + o] = p[q] * inv - m[pi[q]] * m[pj[q]];
+}
+
+// Gradient d_j ubar_i of the filtered velocity on the coarse grid (central
+// differences, periodic), component 3*i + j.
+__global__ void k_coarse_gradient(int Nc1, int Nc2, int Nc3, const double* ubar, double h0, double h1, double h2,
+                                  double* grad) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y, K = blockIdx.z;
+    if (I >= Nc1) return;
+    const long long nc = (long long)Nc1 * Nc2 * Nc3;
+    auto id = [&](int a, int b, int c) {
+        a = (a + Nc1) % Nc1;
+        b = (b + Nc2) % Nc2;
+        c = (c + Nc3) % Nc3;
+        return ((long long)c * Nc2 + b) * Nc1 + a;
+    };
+    const long long o = id(I, J, K);
+    for (int m = 0; m < 3; ++m) {
+        const double* u = ubar + m * nc;
+        grad[(3 * m + 0) * nc + o] = (u[id(I + 1, J, K)] - u[id(I - 1, J, K)]) / (2.0 * h0);
+        grad[(3 * m + 1) * nc + o] = (u[id(I, J + 1, K)] - u[id(I, J - 1, K)]) / (2.0 * h1);
+        grad[(3 * m + 2) * nc + o] = (u[id(I, J, K + 1)] - u[id(I, J, K - 1)]) / (2.0 * h2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+struct Fft {
+    hipfftHandle plan_f = 0, plan_b = 0;
+    Spec s{};
+    ~Fft() {
+        if (plan_f) hipfftDestroy(plan_f);
+        if (plan_b) hipfftDestroy(plan_b);
+    }
+};
+
+ark_status make_fft(ark_ctx* c, Fft& f, bool backward) {
+    f.s.N1 = c->geo.N1;
+    f.s.N2 = c->geo.N2;
+    f.s.N3 = c->geo.N3;
+    f.s.H1 = c->geo.N1 / 2 + 1;
+    f.s.L[0] = c->prm.dx1 * c->geo.N1;
+    f.s.L[1] = c->prm.dx2 * c->geo.N2;
+    f.s.L[2] = c->prm.dx3 * c->geo.N3;
+    if (hipfftPlan3d(&f.plan_f, f.s.N3, f.s.N2, f.s.N1, HIPFFT_D2Z) != HIPFFT_SUCCESS)
+        return c->fail(ARK_ERR_HIP, "hipfftPlan3d D2Z failed");
+    hipfftSetStream(f.plan_f, c->stream);
+    if (backward) {
+        if (hipfftPlan3d(&f.plan_b, f.s.N3, f.s.N2, f.s.N1, HIPFFT_Z2D) != HIPFFT_SUCCESS)
+            return c->fail(ARK_ERR_HIP, "hipfftPlan3d Z2D failed");
+        hipfftSetStream(f.plan_b, c->stream);
+    }
+    return ARK_OK;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+dim3 interior_grid(const Geo& g, int ext) {
+    return dim3((g.N1 + ext + 255) / 256, g.N2 + ext, g.N3 + ext);
+}
+
+}  // namespace
+
+// Mixed precision: the forcing field of the current step (null when off).
+ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st) {
+    if (!c->force || c->force_eps <= 0.0) return ARK_OK;
+    const double L0 = c->prm.dx1 * c->geo.N1, L1 = c->prm.dx2 * c->geo.N2, L2 = c->prm.dx3 * c->geo.N3;
+    hipLaunchKernelGGL(k_force_modes, dim3(1), dim3(64), 0, st, c->S, (unsigned long long)c->force_seed,
+                       c->force_eps, L0, L1, L2, c->force_table);
+    ark_status s = c->hip(hipGetLastError(), "force modes");
+    if (s != ARK_OK) return s;
+    hipLaunchKernelGGL(k_force_field, interior_grid(c->geo, 0), dim3(256), 0, st, c->geo, c->force_table,
+                       c->prm.dx1, c->prm.dx2, c->prm.dx3, L0, L1, L2, c->force, (long long)c->force_stride);
+    return c->hip(hipGetLastError(), "force field");
+}
+
+extern "C" {
+
+ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, double urms) {
+    if (!c || !(k0 > 0.0) || !(urms >= 0.0)) return ARK_ERR_INVALID;
+    const Geo& g = c->geo;
+    if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "turbulence: N must be even");
+    Fft f;
+    ark_status s = make_fft(c, f, true);
+    if (s != ARK_OK) return s;
+    const long long n = (long long)g.N1 * g.N2 * g.N3, nc = (long long)f.s.H1 * g.N2 * g.N3;
+    DevBuf real, spec, work, acc;
+    s = c->hip(hipMalloc(&real.p, 3 * n * sizeof(double)), "hipMalloc turbulence");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&work.p, nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&acc.p, sizeof(double)), "hipMalloc turbulence");
+    if (s != ARK_OK) return s;
+    double* R = (double*)real.p;
+    hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
+    hipfftDoubleComplex* W = (hipfftDoubleComplex*)work.p;
+    hipLaunchKernelGGL(k_noise, dim3(4096), dim3(256), 0, c->stream, f.s, R, seed);
+    s = c->hip(hipGetLastError(), "noise");
+    for (int d = 0; d < 3 && s == ARK_OK; ++d)
+        if (hipfftExecD2Z(f.plan_f, R + d * n, U + d * nc) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "D2Z");
+    if (s != ARK_OK) return s;
+    hipLaunchKernelGGL(k_shape, dim3(4096), dim3(256), 0, c->stream, f.s, U, k0);
+    s = c->hip(hipMemsetAsync(acc.p, 0, sizeof(double), c->stream), "memset");
+    hipLaunchKernelGGL(k_spec_energy, dim3(1024), dim3(256), 0, c->stream, f.s, U, (double*)acc.p);
+    double e = 0.0;
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(&e, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s != ARK_OK) return s;
+    // The unnormalised inverse R(x) = sum_k U_k e^{ikx} has (Parseval)
+    // sum_x |R|^2 = N^3 sum_k |U_k|^2 = N^3 e, so its per-component rms is
+    // sqrt(e / 3); R is scaled to the requested u_rms.
+    const double rms_raw = sqrt(e / 3.0);
+    const double scale = rms_raw > 0.0 ? urms / rms_raw : 0.0;
+
+    // every array zero, then velocities on cells and the three face families
+    for (int q = 0; q < F_NUM && s == ARK_OK; ++q) {
+        s = c->hip(hipMemsetAsync(c->cur[q] - 15, 0, c->elems * sizeof(double), c->stream), "zero");
+        if (s == ARK_OK && c->alt[q]) s = c->hip(hipMemsetAsync(c->alt[q] - 15, 0, c->elems * sizeof(double), c->stream), "zero");
+    }
+    const int cellu[3] = {F_u1nCon, F_u2nCon, F_u3nCon}, conu[3] = {F_u1Con, F_u2Con, F_u3Con};
+    for (int m = 0; m < 3 && s == ARK_OK; ++m) {
+        s = c->hip(hipMemcpyAsync(W, U + m * nc, nc * sizeof(hipfftDoubleComplex), hipMemcpyDeviceToDevice, c->stream),
+                   "copy spectrum");
+        if (s == ARK_OK && hipfftExecZ2D(f.plan_b, W, R) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "Z2D");
+        for (double* dst : {c->cur[cellu[m]], c->alt[cellu[m]], c->cur[conu[m]]}) {
+            if (s != ARK_OK || !dst) continue;
+            hipLaunchKernelGGL(k_scatter, interior_grid(g, 2), dim3(256), 0, c->stream, g, dst, R, scale, 0.0, 2);
+            s = c->hip(hipGetLastError(), "scatter");
+        }
+        for (int d = 0; d < 3 && s == ARK_OK; ++d) {
+            hipLaunchKernelGGL(k_shift, dim3(4096), dim3(256), 0, c->stream, f.s, U + m * nc, W, d);
+            s = c->hip(hipGetLastError(), "shift");
+            if (s == ARK_OK && hipfftExecZ2D(f.plan_b, W, R) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "Z2D");
+            const int fu = face_u(d, m);
+            for (double* dst : {c->cur[fu], c->alt[fu]}) {
+                if (s != ARK_OK || !dst) continue;
+                hipLaunchKernelGGL(k_scatter, interior_grid(g, 3), dim3(256), 0, c->stream, g, dst, R, scale, 0.0, 3);
+                s = c->hip(hipGetLastError(), "scatter");
+            }
+        }
+    }
+    // uniform density rho0 and t0, zero pressure
+    const int cellc[4] = {F_ronCon, F_roCon, F_tnCon, F_tCon};
+    const double cellv[4] = {c->prm.ro0, c->prm.ro0, c->prm.t0, c->prm.t0};
+    for (int q = 0; q < 4 && s == ARK_OK; ++q)
+        for (double* dst : {c->cur[cellc[q]], c->alt[cellc[q]]}) {
+            if (s != ARK_OK || !dst) continue;
+            hipLaunchKernelGGL(k_scatter, interior_grid(g, 2), dim3(256), 0, c->stream, g, dst, (const double*)nullptr,
+                               0.0, cellv[q], 2);
+            s = c->hip(hipGetLastError(), "fill");
+        }
+    for (int d = 0; d < 3 && s == ARK_OK; ++d) {
+        const int fq[2] = {face_ro(d), face_t(d)};
+        const double fv[2] = {c->prm.ro0, c->prm.t0};
+        for (int q = 0; q < 2; ++q)
+            for (double* dst : {c->cur[fq[q]], c->alt[fq[q]]}) {
+                if (s != ARK_OK || !dst) continue;
+                hipLaunchKernelGGL(k_scatter, interior_grid(g, 3), dim3(256), 0, c->stream, g, dst,
+                                   (const double*)nullptr, 0.0, fv[q], 3);
+                s = c->hip(hipGetLastError(), "fill");
+            }
+    }
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s == ARK_OK) s = ark_finish_init(c);
+    return s;
+}
+
+ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
+    if (!c || nbins < 1 || !E) return ARK_ERR_INVALID;
+    const Geo& g = c->geo;
+    Fft f;
+    ark_status s = make_fft(c, f, false);
+    if (s != ARK_OK) return s;
+    const long long n = (long long)g.N1 * g.N2 * g.N3, nc = (long long)f.s.H1 * g.N2 * g.N3;
+    DevBuf real, spec, bins;
+    s = c->hip(hipMalloc(&real.p, n * sizeof(double)), "hipMalloc spectrum");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc spectrum");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&bins.p, nbins * sizeof(double)), "hipMalloc spectrum");
+    if (s == ARK_OK) s = c->hip(hipMemsetAsync(bins.p, 0, nbins * sizeof(double), c->stream), "memset");
+    const int cellu[3] = {F_u1nCon, F_u2nCon, F_u3nCon};
+    hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
+    for (int m = 0; m < 3 && s == ARK_OK; ++m) {
+        hipLaunchKernelGGL(k_extract, interior_grid(g, 0), dim3(256), 0, c->stream, g, c->cur[cellu[m]], (double*)real.p);
+        s = c->hip(hipGetLastError(), "extract");
+        if (s == ARK_OK && hipfftExecD2Z(f.plan_f, (double*)real.p, U + m * nc) != HIPFFT_SUCCESS)
+            s = c->fail(ARK_ERR_HIP, "D2Z");
+    }
+    if (s == ARK_OK) {
+        hipLaunchKernelGGL(k_shell_sum, dim3(1024), dim3(256), nbins * sizeof(double), c->stream, f.s, U, nbins,
+                           (double*)bins.p);
+        s = c->hip(hipGetLastError(), "shell sum");
+    }
+    if (s == ARK_OK)
+        s = c->hip(hipMemcpyAsync(E, bins.p, nbins * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    return s;
+}
+
+ark_status ark_dissipation(ark_ctx* c, double* eps) {
+    if (!c || !eps) return ARK_ERR_INVALID;
+    DevBuf acc;
+    ark_status s = c->hip(hipMalloc(&acc.p, sizeof(double)), "hipMalloc");
+    if (s == ARK_OK) s = c->hip(hipMemsetAsync(acc.p, 0, sizeof(double), c->stream), "memset");
+    if (s == ARK_OK) {
+        hipLaunchKernelGGL(k_dissipation, interior_grid(c->geo, 0), dim3(256), 0, c->stream, c->geo,
+                           c->cur[F_u1nCon], c->cur[F_u2nCon], c->cur[F_u3nCon], 0.5 / c->prm.dx1, 0.5 / c->prm.dx2,
+                           0.5 / c->prm.dx3, (double*)acc.p);
+        s = c->hip(hipGetLastError(), "dissipation");
+    }
+    double v = 0.0;
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(&v, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    *eps = c->prm.VIS * v / ((double)c->geo.N1 * c->geo.N2 * c->geo.N3);
+    return s;
+}
+
+ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps) {
+    if (!c || !(eps >= 0.0)) return ARK_ERR_INVALID;
+    if (c->prm.precision != ARK_PREC_MIXED)
+        return c->fail(ARK_ERR_UNSUPPORTED, "forcing: mixed-precision contexts only");
+    ark_status s = ARK_OK;
+    if (eps > 0.0) {
+        ForceMode h[16];
+        int q = 0;
+        for (int a = -2; a <= 2; ++a)
+            for (int b = -2; b <= 2; ++b)
+                for (int d = -2; d <= 2; ++d) {
+                    const int k2 = a * a + b * b + d * d;
+                    const int first = a != 0 ? a : b != 0 ? b : d;
+                    if (k2 >= 1 && k2 <= 4 && first > 0) h[q++] = ForceMode{{a, b, d}};
+                }
+        s = c->hip(hipMemcpyToSymbol(HIP_SYMBOL(c_modes), h, sizeof h), "force modes");
+    }
+    if (s == ARK_OK && !c->force && eps > 0.0) {
+        float* p = nullptr;
+        if (s == ARK_OK) s = c->hip(hipMalloc(&p, (3 * c->elems + 96) * sizeof(float)), "hipMalloc force");
+        if (s == ARK_OK) {
+            c->allocs.push_back(p);
+            c->bytes += (3 * c->elems + 96) * sizeof(float);
+            s = c->hip(hipMemsetAsync(p, 0, (3 * c->elems + 96) * sizeof(float), c->stream), "memset force");
+            c->force_table = p;
+            c->force = p + 96 + 15;   // element (0,0,0) of component 0
+            c->force_stride = c->elems;
+        }
+    }
+    c->force_seed = seed;
+    c->force_eps = eps;
+    return s;
+}
+
+ark_status ark_filter_box(ark_ctx* c, int width, double* ubar, double* tau, double* grad) {
+    if (!c || width < 1 || !ubar || !tau || !grad) return ARK_ERR_INVALID;
+    const Geo& g = c->geo;
+    if (g.N1 % width || g.N2 % width || g.N3 % width)
+        return c->fail(ARK_ERR_INVALID, "filter: the width must divide N1, N2, N3");
+    const int Nc1 = g.N1 / width, Nc2 = g.N2 / width, Nc3 = g.N3 / width;
+    const long long nc = (long long)Nc1 * Nc2 * Nc3;
+    DevBuf db;
+    ark_status s = c->hip(hipMalloc(&db.p, 18 * nc * sizeof(double)), "hipMalloc filter");
+    if (s != ARK_OK) return s;
+    double* u = (double*)db.p;
+    double* t = u + 3 * nc;
+    double* gr = u + 9 * nc;
+    dim3 grid((Nc1 + 255) / 256, Nc2, Nc3);
+    hipLaunchKernelGGL(k_box_filter, grid, dim3(256), 0, c->stream, g, c->cur[F_u1nCon], c->cur[F_u2nCon],
+                       c->cur[F_u3nCon], width, u, t);
+    s = c->hip(hipGetLastError(), "box filter");
+    if (s == ARK_OK) {
+        hipLaunchKernelGGL(k_coarse_gradient, grid, dim3(256), 0, c->stream, Nc1, Nc2, Nc3, u, width * c->prm.dx1,
+                           width * c->prm.dx2, width * c->prm.dx3, gr);
+        s = c->hip(hipGetLastError(), "coarse gradient");
+    }
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(ubar, u, 3 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(tau, t, 6 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(grad, gr, 9 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    return s;
+}
+
+}  // extern "C"

@@ -218,6 +218,39 @@ ark_status ark_batch_autotune(ark_batch* batch, int reps, double* ms_out, int ma
 ark_status ark_batch_set_config(ark_batch* batch, int shape, int lz);
 ark_status ark_batch_get_config(const ark_batch* batch, int* shape, int* lz, int* tiles, int* threads);
 
+/* Homogeneous turbulence (no counterpart in the reference).
+ *
+ * ark_init_turbulence: a seeded solenoidal velocity field with the shell
+ * spectrum E(k) ~ k^4 exp(-2 (k/k0)^2) and per-component rms `urms`, uniform
+ * density rho0, zero pressure, t = t0; face values are the same Fourier
+ * series evaluated on the faces (exactly periodic: face N+1 is face 1).  The
+ * field is Gaussian white noise from Philox4x32-10 (key = seed, counter =
+ * (cell index, 0x7a3c51, 0)), transformed, shaped by |k| exp(-(|k|/k0)^2)
+ * per mode, projected normal to k, zero mean and Nyquist modes, scaled.
+ * Needs even N.  TIME and nStep restart at 0.
+ *
+ * ark_spectrum: E[b] = sum over integer-index shells b = round(|m|) of
+ * 1/2 |u_hat|^2 / N^6, b < nbins; sum_b E[b] = 1/2 <u.u> over the cells.
+ *
+ * ark_dissipation: nu <sum_ij (d_j u_i)^2> with central differences.
+ *
+ * ark_set_forcing (mixed precision): random-phase forcing of the 16 pairs of
+ * wavevectors with 1 <= |m|^2 <= 4; each step every mode gets a fresh
+ * random amplitude vector normal to k (Philox, key = seed, counter = (mode,
+ * nStep, 0xf0c3, 0|1)) of norm sqrt(eps/(16 dt)), so the mean injection
+ * rate is eps per unit mass.  eps = 0 switches it off.
+ *
+ * ark_filter_box: the box filter of width w cells (w divides N) of the
+ * level-n velocity: ubar (3 x Nc), the subgrid stresses tau = <u_i u_j> -
+ * ubar_i ubar_j (6 x Nc, ij = 11 22 33 12 13 23) and the central-difference
+ * gradients of ubar on the coarse grid (9 x Nc, component 3i + j = d_j ubar_i),
+ * Nc = (N1/w)(N2/w)(N3/w) values each, x fastest.  All synchronise. */
+ark_status ark_init_turbulence(ark_ctx* ctx, unsigned long long seed, double k0, double urms);
+ark_status ark_spectrum(ark_ctx* ctx, int nbins, double* E);
+ark_status ark_dissipation(ark_ctx* ctx, double* eps);
+ark_status ark_set_forcing(ark_ctx* ctx, unsigned long long seed, double eps);
+ark_status ark_filter_box(ark_ctx* ctx, int width, double* ubar, double* tau, double* grad);
+
 /* Bytes of device memory held by the context. */
 ark_status ark_device_bytes(const ark_ctx* ctx, int64_t* bytes);
 

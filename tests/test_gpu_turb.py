@@ -1,0 +1,116 @@
+"""GPU: homogeneous turbulence (seeded initial field, forcing, spectrum, box
+filter, dissipation) against the numpy oracle (oracle/turb_oracle.py) and
+closed forms."""
+import numpy as np
+import pytest
+
+import arkamd
+import turb_oracle as T
+
+pytestmark = pytest.mark.gpu
+
+
+
+def cells_of(s):
+    return [s.download(n)[1:-1, 1:-1, 1:-1] for n in ("u1nCon", "u2nCon", "u3nCon")]
+
+
+@pytest.mark.parametrize("N,prec", [((32, 32, 32), arkamd.PREC_FP64), ((24, 32, 16), arkamd.PREC_MIXED)])
+def test_init_turbulence_vs_oracle(N, prec):
+    with arkamd.Solver(*N, precision=prec) as s:
+        s.init_turbulence(seed=11, k0=4.0, urms=1.0)
+        L = tuple(d * n for d, n in zip(s.dx, N))
+        cells, faces = T.init_turbulence(N, L, 11, 4.0, 1.0)
+        for a, b in zip(cells_of(s), cells):
+            assert np.abs(a - b).max() < 1e-12
+        tol = 1e-12 if prec == arkamd.PREC_FP64 else 1e-6     # FP32 face storage
+        for d in range(3):
+            for m in range(3):
+                f = s.download(["u11", "u21", "u31", "u12", "u22", "u32", "u13", "u23", "u33"][3 * d + m])
+                got = f[1:N[2] + 1, 1:N[1] + 1, 1:N[0] + 1]
+                assert np.abs(got - faces[d][m]).max() < tol, (d, m)
+                # exactly periodic: face N+1 is face 1
+                if d == 0:
+                    assert np.array_equal(f[1:N[2] + 1, 1:N[1] + 1, N[0] + 1], f[1:N[2] + 1, 1:N[1] + 1, 1])
+        assert np.all(s.download("ronCon") == 1.0)
+        d = s.diagnostics()
+        assert d["nstep"] == 0 and d["time"] == 0.0
+
+
+def test_spectrum_dissipation_filter_vs_oracle():
+    N = (32, 24, 16)
+    with arkamd.Solver(*N, VIS=0.02) as s:
+        s.init_turbulence(seed=5, k0=3.0, urms=0.8)
+        u = cells_of(s)
+        nb = 30
+        assert np.abs(s.spectrum(nb) - T.spectrum(u, nb)).max() < 1e-14
+        assert abs(s.spectrum(nb).sum() - 0.5 * sum((a * a).mean() for a in u)) < 1e-13
+        assert abs(s.dissipation() - T.dissipation(u, s.dx, 0.02)) < 1e-13
+        ub, tau, gr = s.filter_box(4)
+        ub0, tau0 = T.box_filter(u, 4)
+        gr0 = T.coarse_gradient(ub0, tuple(4 * d for d in s.dx))
+        assert np.abs(ub - ub0).max() < 1e-14 and np.abs(tau - tau0).max() < 1e-14
+        assert np.abs(gr - gr0).max() < 1e-12
+
+
+def test_single_mode_spectrum_on_device():
+    N = 32
+    with arkamd.Solver(N, N, N) as s:
+        a = np.zeros(s.shape("u1nCon"))
+        x = (np.arange(N + 2) - 0.5) * s.dx[0]
+        a[:] = 0.7 * np.sin(5 * x)[None, None, :]
+        s.upload("u1nCon", a)
+        for n in ("u2nCon", "u3nCon"):
+            s.upload(n, np.zeros(s.shape(n)))
+        E = s.spectrum(30)
+        assert abs(E[5] - 0.49 / 4) < 1e-14 and abs(E.sum() - E[5]) < 1e-14
+
+
+def rest(N, vis=1e-6):
+    s = arkamd.Solver(*N, VIS=vis, precision=arkamd.PREC_MIXED)
+    s.init_turbulence(seed=1, k0=4.0, urms=0.0)      # fluid at rest, uniform density
+    return s
+
+
+def test_forcing_injects_eps_dt_in_one_step():
+    """From rest, one step of white forcing gives u = F dt, so the kinetic
+    energy per unit mass is <|F|^2> dt^2 / 2 = eps dt (the grid mean of the
+    forced modes' squares is exact).  Tolerance 1%: within the step the
+    scheme's pressure coupling (Phase2 faces between the two half-step force
+    applications) takes a few tenths of a percent."""
+    eps = 0.1
+    with rest((32, 32, 32)) as s:
+        s.set_forcing(seed=3, eps=eps)
+        s.step(1)
+        d = s.diagnostics()
+        ke = 0.5 * sum((a * a).mean() for a in cells_of(s))
+        assert abs(ke / (eps * d["dt"]) - 1.0) < 1e-2, ke / (eps * d["dt"])
+
+
+def test_forcing_is_seeded_and_solenoidal():
+    N = (32, 32, 32)
+    out = []
+    for seed in (3, 3, 4):
+        with rest(N) as s:
+            s.set_forcing(seed=seed, eps=0.1)
+            s.step(3)
+            out.append(cells_of(s))
+    assert all(np.array_equal(a, b) for a, b in zip(out[0], out[1]))
+    assert not np.array_equal(out[0][0], out[2][0])
+    # only the forced shells 1 <= |m| <= 2 carry energy; the field is solenoidal
+    E = T.spectrum(out[0], 17)
+    assert E[1:3].sum() > 0.98 * E.sum()
+    (k1, k2, k3), _ = T.wavevectors(N, (2 * np.pi,) * 3)
+    U = [np.fft.rfftn(a) for a in out[0]]
+    div = np.abs(k1 * U[0] + k2 * U[1] + k3 * U[2]).max()
+    assert div < 2e-2 * max(np.abs(x).max() for x in U)
+
+
+def test_forced_turbulence_runs():
+    with arkamd.Solver(32, 32, 32, VIS=1 / 400, precision=arkamd.PREC_MIXED) as s:
+        s.init_turbulence(seed=2, k0=4.0, urms=1.0)
+        s.set_forcing(seed=9, eps=0.1)
+        s.step(200)
+        d = s.diagnostics()
+        assert d["nonfinite"] == 0 and d["nstep"] == 200
+        assert s.dissipation() > 0
