@@ -51,6 +51,9 @@ EXPORTED_SYMBOLS = [
     "ark_batch_create", "ark_batch_destroy", "ark_batch_last_error", "ark_batch_step", "ark_batch_synchronize",
     "ark_batch_autotune", "ark_batch_set_config", "ark_batch_get_config",
     "ark_init_turbulence", "ark_spectrum", "ark_dissipation", "ark_set_forcing", "ark_filter_box",
+    "ark_mlp_param_count", "ark_mlp_create", "ark_mlp_destroy", "ark_mlp_last_error", "ark_mlp_set_params",
+    "ark_mlp_get_params", "ark_mlp_set_normalization", "ark_mlp_set_data", "ark_mlp_gradient", "ark_mlp_train",
+    "ark_mlp_predict",
 ]
 
 
@@ -143,8 +146,24 @@ def lib() -> ctypes.CDLL:
     L.ark_dissipation.argtypes = [vp, dp]
     L.ark_set_forcing.argtypes = [vp, u64, ctypes.c_double]
     L.ark_filter_box.argtypes = [vp, ip, dp, dp, dp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    ll = ctypes.c_longlong
+    L.ark_mlp_param_count.argtypes = []
+    L.ark_mlp_create.argtypes = [ip, u64, ctypes.POINTER(vp)]
+    L.ark_mlp_destroy.argtypes = [vp]
+    L.ark_mlp_destroy.restype = None
+    L.ark_mlp_last_error.argtypes = [vp]
+    L.ark_mlp_last_error.restype = ctypes.c_char_p
+    L.ark_mlp_set_params.argtypes = [vp, fp]
+    L.ark_mlp_get_params.argtypes = [vp, fp]
+    L.ark_mlp_set_normalization.argtypes = [vp, fp, fp, fp, fp]
+    L.ark_mlp_set_data.argtypes = [vp, fp, fp, ll]
+    L.ark_mlp_gradient.argtypes = [vp, fp, fp, ll, dp, dp]
+    L.ark_mlp_train.argtypes = [vp, ip, ll, ctypes.c_float, dp]
+    L.ark_mlp_predict.argtypes = [vp, fp, fp, ll]
     for fn in EXPORTED_SYMBOLS:
-        if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error"):
+        if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error",
+                      "ark_mlp_destroy", "ark_mlp_last_error"):
             getattr(L, fn).restype = ctypes.c_int
     _lib = L
     return L
@@ -424,6 +443,89 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.ark_batch_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"shape {a.shape} != {shape}")
+    return a
+
+
+class MLP:
+    """The closure network (``ark_mlp``): 10 -> 64 -> 64 -> 6, SiLU, BF16-MFMA."""
+
+    NI, NO = 10, 6
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        self._L = lib()
+        self.nparam = self._L.ark_mlp_param_count()
+        h = ctypes.c_void_p()
+        st = self._L.ark_mlp_create(int(device), int(seed), ctypes.byref(h))
+        if st != ARK_OK:
+            raise ArkError(st, "ark_mlp_create failed")
+        self._h = h
+
+    def _check(self, st: int) -> None:
+        if st != ARK_OK:
+            raise ArkError(st, self._L.ark_mlp_last_error(self._h).decode())
+
+    @staticmethod
+    def _p(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparam, dtype=np.float32)
+        self._check(self._L.ark_mlp_get_params(self._h, self._p(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = _f32(p, (self.nparam,))
+        self._check(self._L.ark_mlp_set_params(self._h, self._p(p)))
+
+    def set_normalization(self, in_mean, in_std, out_mean, out_std) -> None:
+        arrs = [_f32(in_mean, (10,)), _f32(in_std, (10,)), _f32(out_mean, (6,)), _f32(out_std, (6,))]
+        self._check(self._L.ark_mlp_set_normalization(self._h, *[self._p(a) for a in arrs]))
+
+    def set_data(self, X, T) -> None:
+        X, T = _f32(X), _f32(T)
+        self._check(self._L.ark_mlp_set_data(self._h, self._p(X), self._p(T), X.shape[0]))
+
+    def gradient(self, X, T):
+        X, T = _f32(X), _f32(T)
+        g = np.zeros(self.nparam)
+        loss = ctypes.c_double()
+        self._check(self._L.ark_mlp_gradient(self._h, self._p(X), self._p(T), X.shape[0],
+                                             g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, nsteps: int, batch: int = 65536, lr: float = 1e-3) -> float:
+        loss = ctypes.c_double()
+        self._check(self._L.ark_mlp_train(self._h, int(nsteps), int(batch), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def predict(self, X) -> np.ndarray:
+        X = _f32(X)
+        Y = np.zeros((X.shape[0], 6), dtype=np.float32)
+        self._check(self._L.ark_mlp_predict(self._h, self._p(X), self._p(Y), X.shape[0]))
+        return Y
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.ark_mlp_destroy(self._h)
         self._h = None
 
     def __enter__(self):

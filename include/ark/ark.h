@@ -251,6 +251,33 @@ ark_status ark_dissipation(ark_ctx* ctx, double* eps);
 ark_status ark_set_forcing(ark_ctx* ctx, unsigned long long seed, double eps);
 ark_status ark_filter_box(ark_ctx* ctx, int width, double* ubar, double* tau, double* grad);
 
+/* Learned subgrid closure (no counterpart in the reference): an MLP per cell,
+ * 10 inputs (the 9 filtered velocity-gradient components d_j u_i, index
+ * 3i + j, and the filter width) -> 64 -> 64 -> 6 subgrid stresses
+ * (11 22 33 12 13 23), SiLU, BF16 weights and activations with FP32
+ * accumulation on the matrix cores; FP32 master weights and Adam
+ * (beta 0.9, 0.999, eps 1e-8) for training.  Inputs and targets are in raw
+ * units; ark_mlp_set_normalization gives the per-feature shift and scale the
+ * network sees.  Parameters (ark_mlp_param_count() floats): W1[64][10] b1[64]
+ * W2[64][64] b2[64] W3[6][64] b3[6].  The loss is the mean squared error of
+ * the normalised outputs.  Training batches are drawn with replacement from
+ * the data set (Philox, key = seed, counter = (position, step)). */
+typedef struct ark_mlp ark_mlp;
+int ark_mlp_param_count(void);
+ark_status ark_mlp_create(int device, unsigned long long seed, ark_mlp** out);
+void ark_mlp_destroy(ark_mlp* mlp);
+const char* ark_mlp_last_error(const ark_mlp* mlp);
+ark_status ark_mlp_set_params(ark_mlp* mlp, const float* params);
+ark_status ark_mlp_get_params(ark_mlp* mlp, float* params);
+ark_status ark_mlp_set_normalization(ark_mlp* mlp, const float* in_mean, const float* in_std,
+                                     const float* out_mean, const float* out_std);
+ark_status ark_mlp_set_data(ark_mlp* mlp, const float* X, const float* T, long long n);
+/* Gradient of the loss over all n samples (X: n x 10, T: n x 6, host). */
+ark_status ark_mlp_gradient(ark_mlp* mlp, const float* X, const float* T, long long n, double* grad,
+                            double* loss);
+ark_status ark_mlp_train(ark_mlp* mlp, int nsteps, long long batch, float lr, double* last_loss);
+ark_status ark_mlp_predict(ark_mlp* mlp, const float* X, float* Y, long long n);
+
 /* Bytes of device memory held by the context. */
 ark_status ark_device_bytes(const ark_ctx* ctx, int64_t* bytes);
 
