@@ -53,7 +53,7 @@ EXPORTED_SYMBOLS = [
     "ark_init_turbulence", "ark_spectrum", "ark_dissipation", "ark_set_forcing", "ark_filter_box",
     "ark_mlp_param_count", "ark_mlp_create", "ark_mlp_destroy", "ark_mlp_last_error", "ark_mlp_set_params",
     "ark_mlp_get_params", "ark_mlp_set_normalization", "ark_mlp_set_data", "ark_mlp_gradient", "ark_mlp_train",
-    "ark_mlp_predict",
+    "ark_mlp_predict", "ark_set_closure", "ark_init_velocity",
 ]
 
 
@@ -161,6 +161,8 @@ def lib() -> ctypes.CDLL:
     L.ark_mlp_gradient.argtypes = [vp, fp, fp, ll, dp, dp]
     L.ark_mlp_train.argtypes = [vp, ip, ll, ctypes.c_float, dp]
     L.ark_mlp_predict.argtypes = [vp, fp, fp, ll]
+    L.ark_set_closure.argtypes = [vp, vp, ctypes.c_double]
+    L.ark_init_velocity.argtypes = [vp, dp]
     for fn in EXPORTED_SYMBOLS:
         if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error",
                       "ark_mlp_destroy", "ark_mlp_last_error"):
@@ -330,6 +332,19 @@ class Solver:
         v = ctypes.c_double()
         self._check(self._L.ark_dissipation(self._h, ctypes.byref(v)))
         return v.value
+
+    def set_closure(self, mlp, delta: float) -> None:
+        """Attach a trained MLP (or None) as the subgrid closure."""
+        self._closure = mlp
+        self._check(self._L.ark_set_closure(self._h, mlp._h if mlp is not None else None, float(delta)))
+
+    def init_velocity(self, u) -> None:
+        """Cells from u (3, N3, N2, N1); faces by the half-cell spectral shift."""
+        N1, N2, N3 = self.N
+        a = np.ascontiguousarray(u, dtype=np.float64)
+        if a.shape != (3, N3, N2, N1):
+            raise ValueError(f"velocity shape {a.shape} != {(3, N3, N2, N1)}")
+        self._check(self._L.ark_init_velocity(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
 
     def set_forcing(self, seed: int, eps: float) -> None:
         self._check(self._L.ark_set_forcing(self._h, int(seed), float(eps)))

@@ -40,6 +40,11 @@ struct ark_ctx {
     long long force_stride = 0;
     unsigned long long force_seed = 0;
     double force_eps = 0.0;
+    // learned closure (ark_set_closure): network, grid scale, its inputs and outputs
+    struct ark_mlp* closure = nullptr;
+    double closure_delta = 0.0;
+    float* les_X = nullptr;
+    float* les_Y = nullptr;
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;
@@ -66,6 +71,8 @@ extern "C" {
 ark_status ark_cfl_scan(ark_ctx* c);                    // running MIN of the CFL bound over Con
 ark_status ark_finish_init(ark_ctx* c);                 // after an initial state: FP32 faces, flags, TIME = 0
 ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st);   // this step's forcing field (mixed precision)
+ark_status ark_closure_step(ark_ctx* c, hipStream_t st);      // the closure's body force (after the forcing)
+ark_status ark_mlp_forward_device(struct ark_mlp* m, const float* X, float* Y, long long n, hipStream_t st);
 ark_status ark_cfl_step(ark_ctx* c, int advance);       // TimeStepSize on Con, then dt := MIN
 ark_status ark_cfl_finalize(ark_ctx* c, int mode);      // see k_cfl_finalize
 ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pending MIN or Con), TIME, nStep

@@ -686,7 +686,7 @@ mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, int
     a.S = c->S;
     a.dt_next = dt_next;
     a.LZ = LZ;
-    a.force = c->force_eps > 0.0 ? c->force : nullptr;
+    a.force = (c->force_eps > 0.0 || c->closure) ? c->force : nullptr;
     a.fstride = c->force_stride;
     a.dvcf = (float)c->K.dvc;
     return a;
@@ -744,6 +744,7 @@ ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
         if (s == ARK_OK) s = ark_turb_force_step(c, c->stream);
+        if (s == ARK_OK) s = ark_closure_step(c, c->stream);
         if (s == ARK_OK) s = c->launch("fused_mixed", run_mixed, &ml);
         if (s != ARK_OK) break;
         for (int g = 0; g < 4; ++g) ark_swap_group(c, g);
@@ -934,6 +935,7 @@ ark_status ark_batch_create(ark_ctx* const* boxes, int nboxes, ark_batch** out) 
         else if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)
             s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes need N1, N2, N3 >= 2");
         else if (c->has_solid) s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes must be all fluid");
+        else if (c->closure) s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes with a learned closure step alone");
         else if (i > 0 && (c->prm.device != boxes[0]->prm.device || c->prm.scheme != boxes[0]->prm.scheme))
             s = b->fail(ARK_ERR_UNSUPPORTED, "batch: boxes must share the device and the scheme");
         for (int j = 0; j < i && s == ARK_OK; ++j)

@@ -329,6 +329,48 @@ __global__ void k_coarse_gradient(int Nc1, int Nc2, int Nc3, const double* ubar,
 }
 
 // ---------------------------------------------------------------------------
+// Learned closure on the solver's grid: per cell the network's inputs (the
+// central-difference velocity gradient d_j u_i, component 3i + j, and the
+// grid scale), and from its output, the subgrid stresses tau_ij, the
+// acceleration -d_j tau_ij added to (or written into) the body force.
+__global__ void k_les_inputs(Geo g, const double* u1, const double* u2, const double* u3, double inv0, double inv1,
+                             double inv2, float delta, float* X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.N1) return;
+    const int ip = i == g.N1 ? 1 : i + 1, im = i == 1 ? g.N1 : i - 1;
+    const int jp = j == g.N2 ? 1 : j + 1, jm = j == 1 ? g.N2 : j - 1;
+    const int kp = k == g.N3 ? 1 : k + 1, km = k == 1 ? g.N3 : k - 1;
+    const double* u[3] = {u1, u2, u3};
+    float* x = X + (((size_t)(k - 1) * g.N2 + (j - 1)) * g.N1 + (i - 1)) * 10;
+    for (int m = 0; m < 3; ++m) {
+        x[3 * m + 0] = (float)((u[m][at(g, ip, j, k)] - u[m][at(g, im, j, k)]) * inv0);
+        x[3 * m + 1] = (float)((u[m][at(g, i, jp, k)] - u[m][at(g, i, jm, k)]) * inv1);
+        x[3 * m + 2] = (float)((u[m][at(g, i, j, kp)] - u[m][at(g, i, j, km)]) * inv2);
+    }
+    x[9] = delta;
+}
+
+__global__ void k_les_force(Geo g, const float* Y, float inv0, float inv1, float inv2, float* force, long long stride,
+                            int add) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.N1) return;
+    const int ip = i == g.N1 ? 1 : i + 1, im = i == 1 ? g.N1 : i - 1;
+    const int jp = j == g.N2 ? 1 : j + 1, jm = j == 1 ? g.N2 : j - 1;
+    const int kp = k == g.N3 ? 1 : k + 1, km = k == 1 ? g.N3 : k - 1;
+    auto tau = [&](int a, int b, int c, int q) { return Y[(((size_t)(c - 1) * g.N2 + (b - 1)) * g.N1 + (a - 1)) * 6 + q]; };
+    // tau components 11 22 33 12 13 23
+    const int cmp[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+    const int c = at(g, i, j, k);
+    for (int m = 0; m < 3; ++m) {
+        const float div = (tau(ip, j, k, cmp[m][0]) - tau(im, j, k, cmp[m][0])) * inv0 +
+                          (tau(i, jp, k, cmp[m][1]) - tau(i, jm, k, cmp[m][1])) * inv1 +
+                          (tau(i, j, kp, cmp[m][2]) - tau(i, j, km, cmp[m][2])) * inv2;
+        float* f = force + m * stride + c;
+        *f = (add ? *f : 0.f) - div;
+    }
+}
+
+// ---------------------------------------------------------------------------
 struct Fft {
     hipfftHandle plan_f = 0, plan_b = 0;
     Spec s{};
@@ -368,58 +410,16 @@ dim3 interior_grid(const Geo& g, int ext) {
     return dim3((g.N1 + ext + 255) / 256, g.N2 + ext, g.N3 + ext);
 }
 
-}  // namespace
-
-// Mixed precision: the forcing field of the current step (null when off).
-ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st) {
-    if (!c->force || c->force_eps <= 0.0) return ARK_OK;
-    const double L0 = c->prm.dx1 * c->geo.N1, L1 = c->prm.dx2 * c->geo.N2, L2 = c->prm.dx3 * c->geo.N3;
-    hipLaunchKernelGGL(k_force_modes, dim3(1), dim3(64), 0, st, c->S, (unsigned long long)c->force_seed,
-                       c->force_eps, L0, L1, L2, c->force_table);
-    ark_status s = c->hip(hipGetLastError(), "force modes");
-    if (s != ARK_OK) return s;
-    hipLaunchKernelGGL(k_force_field, interior_grid(c->geo, 0), dim3(256), 0, st, c->geo, c->force_table,
-                       c->prm.dx1, c->prm.dx2, c->prm.dx3, L0, L1, L2, c->force, (long long)c->force_stride);
-    return c->hip(hipGetLastError(), "force field");
-}
-
-extern "C" {
-
-ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, double urms) {
-    if (!c || !(k0 > 0.0) || !(urms >= 0.0)) return ARK_ERR_INVALID;
+// A velocity field given by its half spectrum U (three components, the
+// unnormalised forward transforms of the cell values) times `scale`: every
+// array zeroed, then the velocities on the cells (level n, both buffers, and
+// Con) and, by the half-cell shift, on the three face families; uniform rho0
+// and t0, zero pressure.  W (one component) and R (one real field) are work.
+ark_status install_velocity(ark_ctx* c, Fft& f, const hipfftDoubleComplex* U, hipfftDoubleComplex* W, double* R,
+                            double scale) {
     const Geo& g = c->geo;
-    if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "turbulence: N must be even");
-    Fft f;
-    ark_status s = make_fft(c, f, true);
-    if (s != ARK_OK) return s;
-    const long long n = (long long)g.N1 * g.N2 * g.N3, nc = (long long)f.s.H1 * g.N2 * g.N3;
-    DevBuf real, spec, work, acc;
-    s = c->hip(hipMalloc(&real.p, 3 * n * sizeof(double)), "hipMalloc turbulence");
-    if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
-    if (s == ARK_OK) s = c->hip(hipMalloc(&work.p, nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
-    if (s == ARK_OK) s = c->hip(hipMalloc(&acc.p, sizeof(double)), "hipMalloc turbulence");
-    if (s != ARK_OK) return s;
-    double* R = (double*)real.p;
-    hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
-    hipfftDoubleComplex* W = (hipfftDoubleComplex*)work.p;
-    hipLaunchKernelGGL(k_noise, dim3(4096), dim3(256), 0, c->stream, f.s, R, seed);
-    s = c->hip(hipGetLastError(), "noise");
-    for (int d = 0; d < 3 && s == ARK_OK; ++d)
-        if (hipfftExecD2Z(f.plan_f, R + d * n, U + d * nc) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "D2Z");
-    if (s != ARK_OK) return s;
-    hipLaunchKernelGGL(k_shape, dim3(4096), dim3(256), 0, c->stream, f.s, U, k0);
-    s = c->hip(hipMemsetAsync(acc.p, 0, sizeof(double), c->stream), "memset");
-    hipLaunchKernelGGL(k_spec_energy, dim3(1024), dim3(256), 0, c->stream, f.s, U, (double*)acc.p);
-    double e = 0.0;
-    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(&e, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
-    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
-    if (s != ARK_OK) return s;
-    // The unnormalised inverse R(x) = sum_k U_k e^{ikx} has (Parseval)
-    // sum_x |R|^2 = N^3 sum_k |U_k|^2 = N^3 e, so its per-component rms is
-    // sqrt(e / 3); R is scaled to the requested u_rms.
-    const double rms_raw = sqrt(e / 3.0);
-    const double scale = rms_raw > 0.0 ? urms / rms_raw : 0.0;
-
+    const long long nc = (long long)f.s.H1 * g.N2 * g.N3;
+    ark_status s = ARK_OK;
     // every array zero, then velocities on cells and the three face families
     for (int q = 0; q < F_NUM && s == ARK_OK; ++q) {
         s = c->hip(hipMemsetAsync(c->cur[q] - 15, 0, c->elems * sizeof(double), c->stream), "zero");
@@ -471,6 +471,138 @@ ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, d
     if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     if (s == ARK_OK) s = ark_finish_init(c);
     return s;
+}
+
+}  // namespace
+
+// Mixed precision: the forcing field of the current step (null when off).
+ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st) {
+    if (!c->force || c->force_eps <= 0.0) return ARK_OK;
+    const double L0 = c->prm.dx1 * c->geo.N1, L1 = c->prm.dx2 * c->geo.N2, L2 = c->prm.dx3 * c->geo.N3;
+    hipLaunchKernelGGL(k_force_modes, dim3(1), dim3(64), 0, st, c->S, (unsigned long long)c->force_seed,
+                       c->force_eps, L0, L1, L2, c->force_table);
+    ark_status s = c->hip(hipGetLastError(), "force modes");
+    if (s != ARK_OK) return s;
+    hipLaunchKernelGGL(k_force_field, interior_grid(c->geo, 0), dim3(256), 0, st, c->geo, c->force_table,
+                       c->prm.dx1, c->prm.dx2, c->prm.dx3, L0, L1, L2, c->force, (long long)c->force_stride);
+    return c->hip(hipGetLastError(), "force field");
+}
+
+// The force buffer (acceleration, 3 x FP32 fields, and the 16-mode table).
+static ark_status ensure_force(ark_ctx* c) {
+    if (c->force) return ARK_OK;
+    float* p = nullptr;
+    ark_status s = c->hip(hipMalloc(&p, (3 * c->elems + 96) * sizeof(float)), "hipMalloc force");
+    if (s != ARK_OK) return s;
+    c->allocs.push_back(p);
+    c->bytes += (3 * c->elems + 96) * sizeof(float);
+    s = c->hip(hipMemsetAsync(p, 0, (3 * c->elems + 96) * sizeof(float), c->stream), "memset force");
+    c->force_table = p;
+    c->force = p + 96 + 15;   // element (0,0,0) of component 0
+    c->force_stride = (long long)c->elems;
+    return s;
+}
+
+// The closure's contribution to this step's body force (after the forcing).
+ark_status ark_closure_step(ark_ctx* c, hipStream_t st) {
+    if (!c->closure) return ARK_OK;
+    const Geo& g = c->geo;
+    hipLaunchKernelGGL(k_les_inputs, interior_grid(g, 0), dim3(256), 0, st, g, c->cur[F_u1nCon], c->cur[F_u2nCon],
+                       c->cur[F_u3nCon], 0.5 / c->prm.dx1, 0.5 / c->prm.dx2, 0.5 / c->prm.dx3,
+                       (float)c->closure_delta, c->les_X);
+    ark_status s = c->hip(hipGetLastError(), "closure inputs");
+    if (s == ARK_OK) s = ark_mlp_forward_device(c->closure, c->les_X, c->les_Y, (long long)g.N1 * g.N2 * g.N3, st);
+    if (s != ARK_OK) return s;
+    hipLaunchKernelGGL(k_les_force, interior_grid(g, 0), dim3(256), 0, st, g, c->les_Y, (float)(0.5 / c->prm.dx1),
+                       (float)(0.5 / c->prm.dx2), (float)(0.5 / c->prm.dx3), c->force, c->force_stride,
+                       c->force_eps > 0.0 ? 1 : 0);
+    return c->hip(hipGetLastError(), "closure force");
+}
+
+extern "C" {
+
+ark_status ark_set_closure(ark_ctx* c, ark_mlp* mlp, double delta) {
+    if (!c) return ARK_ERR_INVALID;
+    if (c->prm.precision != ARK_PREC_MIXED)
+        return c->fail(ARK_ERR_UNSUPPORTED, "closure: mixed-precision contexts only");
+    if (!mlp) {
+        c->closure = nullptr;
+        return ARK_OK;
+    }
+    ark_status s = ensure_force(c);
+    const size_t n = (size_t)c->geo.N1 * c->geo.N2 * c->geo.N3;
+    if (s == ARK_OK && !c->les_X) {
+        s = c->hip(hipMalloc(&c->les_X, n * 16 * sizeof(float)), "hipMalloc closure");
+        if (s == ARK_OK) {
+            c->allocs.push_back(c->les_X);
+            c->bytes += n * 16 * sizeof(float);
+            c->les_Y = c->les_X + n * 10;
+        }
+    }
+    if (s == ARK_OK) {
+        c->closure = mlp;
+        c->closure_delta = delta;
+    }
+    return s;
+}
+
+ark_status ark_init_velocity(ark_ctx* c, const double* u) {
+    if (!c || !u) return ARK_ERR_INVALID;
+    const Geo& g = c->geo;
+    if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "init_velocity: N must be even");
+    Fft f;
+    ark_status s = make_fft(c, f, true);
+    if (s != ARK_OK) return s;
+    const long long n = (long long)g.N1 * g.N2 * g.N3, nc = (long long)f.s.H1 * g.N2 * g.N3;
+    DevBuf real, spec, work;
+    s = c->hip(hipMalloc(&real.p, 3 * n * sizeof(double)), "hipMalloc");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&work.p, nc * sizeof(hipfftDoubleComplex)), "hipMalloc");
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(real.p, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+    double* R = (double*)real.p;
+    hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
+    for (int d = 0; d < 3 && s == ARK_OK; ++d)
+        if (hipfftExecD2Z(f.plan_f, R + d * n, U + d * nc) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "D2Z");
+    if (s != ARK_OK) return s;
+    return install_velocity(c, f, U, (hipfftDoubleComplex*)work.p, R, 1.0 / (double)n);
+}
+
+ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, double urms) {
+    if (!c || !(k0 > 0.0) || !(urms >= 0.0)) return ARK_ERR_INVALID;
+    const Geo& g = c->geo;
+    if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "turbulence: N must be even");
+    Fft f;
+    ark_status s = make_fft(c, f, true);
+    if (s != ARK_OK) return s;
+    const long long n = (long long)g.N1 * g.N2 * g.N3, nc = (long long)f.s.H1 * g.N2 * g.N3;
+    DevBuf real, spec, work, acc;
+    s = c->hip(hipMalloc(&real.p, 3 * n * sizeof(double)), "hipMalloc turbulence");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&work.p, nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&acc.p, sizeof(double)), "hipMalloc turbulence");
+    if (s != ARK_OK) return s;
+    double* R = (double*)real.p;
+    hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
+    hipfftDoubleComplex* W = (hipfftDoubleComplex*)work.p;
+    hipLaunchKernelGGL(k_noise, dim3(4096), dim3(256), 0, c->stream, f.s, R, seed);
+    s = c->hip(hipGetLastError(), "noise");
+    for (int d = 0; d < 3 && s == ARK_OK; ++d)
+        if (hipfftExecD2Z(f.plan_f, R + d * n, U + d * nc) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "D2Z");
+    if (s != ARK_OK) return s;
+    hipLaunchKernelGGL(k_shape, dim3(4096), dim3(256), 0, c->stream, f.s, U, k0);
+    s = c->hip(hipMemsetAsync(acc.p, 0, sizeof(double), c->stream), "memset");
+    hipLaunchKernelGGL(k_spec_energy, dim3(1024), dim3(256), 0, c->stream, f.s, U, (double*)acc.p);
+    double e = 0.0;
+    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(&e, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s != ARK_OK) return s;
+    // The unnormalised inverse R(x) = sum_k U_k e^{ikx} has (Parseval)
+    // sum_x |R|^2 = N^3 sum_k |U_k|^2 = N^3 e, so its per-component rms is
+    // sqrt(e / 3); R is scaled to the requested u_rms.
+    const double rms_raw = sqrt(e / 3.0);
+    const double scale = rms_raw > 0.0 ? urms / rms_raw : 0.0;
+
+    return install_velocity(c, f, U, W, R, scale);
 }
 
 ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
@@ -539,18 +671,7 @@ ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps) {
                 }
         s = c->hip(hipMemcpyToSymbol(HIP_SYMBOL(c_modes), h, sizeof h), "force modes");
     }
-    if (s == ARK_OK && !c->force && eps > 0.0) {
-        float* p = nullptr;
-        if (s == ARK_OK) s = c->hip(hipMalloc(&p, (3 * c->elems + 96) * sizeof(float)), "hipMalloc force");
-        if (s == ARK_OK) {
-            c->allocs.push_back(p);
-            c->bytes += (3 * c->elems + 96) * sizeof(float);
-            s = c->hip(hipMemsetAsync(p, 0, (3 * c->elems + 96) * sizeof(float), c->stream), "memset force");
-            c->force_table = p;
-            c->force = p + 96 + 15;   // element (0,0,0) of component 0
-            c->force_stride = c->elems;
-        }
-    }
+    if (s == ARK_OK && eps > 0.0) s = ensure_force(c);
     c->force_seed = seed;
     c->force_eps = eps;
     return s;

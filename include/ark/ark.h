@@ -278,6 +278,20 @@ ark_status ark_mlp_gradient(ark_mlp* mlp, const float* X, const float* T, long l
 ark_status ark_mlp_train(ark_mlp* mlp, int nsteps, long long batch, float lr, double* last_loss);
 ark_status ark_mlp_predict(ark_mlp* mlp, const float* X, float* Y, long long n);
 
+/* Large-eddy simulation with the learned closure (mixed precision).  With a
+ * network attached, every step computes on the level-n cells the central-
+ * difference velocity gradient (inputs 0..8) and `delta` (input 9, the
+ * filter width the network was trained for), runs the network per cell and
+ * adds the acceleration -d_j tau_ij (central differences) to the body force
+ * (with the forcing, if any).  NULL detaches.  The network must outlive the
+ * attachment; a context with a closure cannot join a batch. */
+ark_status ark_set_closure(ark_ctx* ctx, ark_mlp* mlp, double delta);
+/* Initial state from host cell velocities u (3 x N1*N2*N3 interior values,
+ * component-major, x fastest): the cells take them, the faces the same
+ * Fourier series half a cell down (as ark_init_turbulence); uniform rho0 and
+ * t0, zero pressure.  Needs even N.  TIME and nStep restart at 0. */
+ark_status ark_init_velocity(ark_ctx* ctx, const double* u);
+
 /* Bytes of device memory held by the context. */
 ark_status ark_device_bytes(const ark_ctx* ctx, int64_t* bytes);
 

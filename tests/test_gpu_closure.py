@@ -1,0 +1,61 @@
+"""GPU: the learned-closure pipeline at a small size: forced DNS at 64^3
+generates filtered training pairs (width 2), the network trains on them, and
+coarse 32^3 runs from the filtered initial field run with and without the
+closure."""
+import numpy as np
+import pytest
+
+import arkamd
+from arkamd import closure as C
+import turb_oracle as T
+
+pytestmark = pytest.mark.gpu
+
+VIS, EPS, SEED = 1.0 / 800.0, 0.1, 21
+
+
+@pytest.fixture(scope="module")
+def pipeline():
+    d = C.dns(64, steps=120, snapshots=3, width=2, vis=VIS, eps=EPS, seed=SEED, series_every=40)
+    tr = C.train(d["X"], d["T"], steps=400, batch=8192, lr=2e-3, seed=1)
+    yield d, tr
+    tr["mlp"].close()
+
+
+def test_dns_data(pipeline):
+    d, _ = pipeline
+    assert d["nonfinite"] == 0
+    assert d["X"].shape == (3 * 32 ** 3, 10) and d["T"].shape == (3 * 32 ** 3, 6)
+    assert np.all(d["X"][:, 9] == np.float32(2 * d["dx"]))
+    # the subgrid stress tensor of a box filter is positive semi-definite on
+    # its diagonal (a variance)
+    assert (d["T"][:, :3] >= -1e-7).all()
+    ke = [r["ke"] for r in d["series"]]
+    assert all(np.isfinite(ke)) and all(r["eps"] > 0 for r in d["series"])
+
+
+def test_init_velocity_keeps_the_field(pipeline):
+    d, _ = pipeline
+    ub0 = d["ub0"]
+    with arkamd.Solver(32, 32, 32, VIS=VIS, precision=arkamd.PREC_MIXED) as s:
+        s.init_velocity(ub0)
+        got = [s.download(n)[1:-1, 1:-1, 1:-1] for n in ("u1nCon", "u2nCon", "u3nCon")]
+        assert max(np.abs(g - u).max() for g, u in zip(got, ub0)) < 1e-12
+        assert np.abs(s.spectrum(17) - T.spectrum(list(ub0), 17)).max() < 1e-14
+
+
+def test_training_learns_the_stresses(pipeline):
+    _, tr = pipeline
+    assert tr["loss_end"] < 0.5 * tr["loss0"], (tr["loss0"], tr["loss_end"])
+
+
+def test_les_with_and_without_closure(pipeline):
+    d, tr = pipeline
+    plain = C.les(d["ub0"], d["t_end"], VIS, EPS, SEED)
+    model = C.les(d["ub0"], d["t_end"], VIS, EPS, SEED, mlp=tr["mlp"], delta=2 * d["dx"])
+    assert plain["nonfinite"] == 0 and model["nonfinite"] == 0
+    assert model["time"] >= d["t_end"] and plain["time"] >= d["t_end"]
+    assert not np.array_equal(plain["E"], model["E"])          # the closure acts
+    Eref = C.spectrum_of(d["ub_end"], VIS)
+    for r in (plain, model):
+        assert np.isfinite(C.spectrum_error(r["E"], Eref, 8))
