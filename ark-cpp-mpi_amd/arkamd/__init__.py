@@ -144,7 +144,7 @@ def lib() -> ctypes.CDLL:
     L.ark_init_turbulence.argtypes = [vp, u64, ctypes.c_double, ctypes.c_double]
     L.ark_spectrum.argtypes = [vp, ip, dp]
     L.ark_dissipation.argtypes = [vp, dp]
-    L.ark_set_forcing.argtypes = [vp, u64, ctypes.c_double]
+    L.ark_set_forcing.argtypes = [vp, u64, ctypes.c_double, ctypes.c_double]
     L.ark_filter_box.argtypes = [vp, ip, dp, dp, dp]
     fp = ctypes.POINTER(ctypes.c_float)
     ll = ctypes.c_longlong
@@ -346,8 +346,8 @@ class Solver:
             raise ValueError(f"velocity shape {a.shape} != {(3, N3, N2, N1)}")
         self._check(self._L.ark_init_velocity(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
 
-    def set_forcing(self, seed: int, eps: float) -> None:
-        self._check(self._L.ark_set_forcing(self._h, int(seed), float(eps)))
+    def set_forcing(self, seed: int, eps: float, tau: float = 0.0) -> None:
+        self._check(self._L.ark_set_forcing(self._h, int(seed), float(eps), float(tau)))
 
     def filter_box(self, width: int):
         """(ubar[3], tau[6], grad[9]) on the coarse grid, numpy (Nc3, Nc2, Nc1)."""

@@ -44,19 +44,26 @@ def spectrum_of(u: np.ndarray, vis: float) -> np.ndarray:
 
 
 def run_to(s: Solver, t_end: float, chunk: int = 10, max_steps: int = 10 ** 6) -> int:
-    """Steps until TIME >= t_end (chunks of `chunk` steps); returns the step count."""
+    """Steps until TIME >= t_end: chunks of `chunk` steps while they stay short
+    of t_end, then single steps; returns the step count."""
     n = 0
-    while s.diagnostics(allow_nonfinite=True)["time"] < t_end and n < max_steps:
-        s.step(chunk)
-        n += chunk
+    while n < max_steps:
+        d = s.diagnostics(allow_nonfinite=True)
+        if d["time"] >= t_end:
+            break
+        k = chunk if d["time"] + 1.5 * chunk * max(d["dt"], 1e-30) < t_end else 1
+        s.step(k)
+        n += k
     return n
 
 
 def dns(N: int, steps: int, snapshots: int, width: int, vis: float, eps: float, seed: int,
-        k0: float = 4.0, urms: float = 1.0, series_every: int = 50) -> Dict:
+        k0: float = 4.0, urms: float = 1.0, series_every: int = 50, tau: float = 0.01) -> Dict:
+    """Forced DNS; the forcing is redrawn every `tau` in time so that the
+    coarse runs see the same force history."""
     s = Solver(N, N, N, VIS=vis, precision=PREC_MIXED)
     s.init_turbulence(seed, k0, urms)
-    s.set_forcing(seed + 1, eps)
+    s.set_forcing(seed + 1, eps, tau)
     _, _, ub0 = pairs_from(s, width)
     Xs, Ts, series = [], [], []
     snap_at = set(int(round(steps * (q + 1) / snapshots)) for q in range(snapshots))
@@ -97,11 +104,11 @@ def train(X: np.ndarray, T: np.ndarray, steps: int, batch: int, lr: float, seed:
 
 
 def les(ub0: np.ndarray, t_end: float, vis: float, eps: float, seed: int, mlp: Optional[MLP] = None,
-        delta: float = 0.0) -> Dict:
+        delta: float = 0.0, tau: float = 0.01) -> Dict:
     N3, N2, N1 = ub0.shape[1:]
     s = Solver(N1, N2, N3, VIS=vis, precision=PREC_MIXED)
     s.init_velocity(ub0)
-    s.set_forcing(seed + 1, eps)
+    s.set_forcing(seed + 1, eps, tau)
     if mlp is not None:
         s.set_closure(mlp, delta)
     s.synchronize()

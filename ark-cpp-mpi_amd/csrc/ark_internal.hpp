@@ -37,9 +37,11 @@ struct ark_ctx {
     // components at force + m*force_stride in the field index map, FP32
     float* force = nullptr;
     float* force_table = nullptr;      // this step's 16 forced modes
+    float* force_trig = nullptr;       // per-axis cos/sin of the forced wavenumbers at the cell centres
     long long force_stride = 0;
     unsigned long long force_seed = 0;
     double force_eps = 0.0;
+    double force_tau = 0.0;           // forcing correlation time (0: new amplitudes every step)
     // learned closure (ark_set_closure): network, grid scale, its inputs and outputs
     struct ark_mlp* closure = nullptr;
     double closure_delta = 0.0;

@@ -207,12 +207,17 @@ struct ForceMode { int m[3]; };
 __constant__ ForceMode c_modes[16];
 
 // 16 modes x (re, im) x 3 components.
-__global__ void k_force_modes(const Scalars* S, unsigned long long seed, double eps, double L0, double L1, double L2,
-                              float* table) {
+// With tau > 0 the amplitudes are redrawn once per time interval of length
+// tau (counter = the interval of the step's mid time) instead of every step,
+// with norm sqrt(eps/(16 tau)): a force constant over tau injects
+// <|F|^2> tau / 2 = eps on average, and runs of any resolution see the same
+// force history.
+__global__ void k_force_modes(const Scalars* S, unsigned long long seed, double eps, double tau, double L0, double L1,
+                              double L2, float* table) {
     const int n = threadIdx.x;
     if (n >= 16) return;
-    const double dt = S->dt;
-    const uint32_t step = (uint32_t)S->nstep;
+    const double dt = tau > 0.0 ? tau : S->dt;
+    const uint32_t step = tau > 0.0 ? (uint32_t)floor((S->time - 0.5 * S->dt) / tau) : (uint32_t)S->nstep;
     double g[8];
     gauss4(seed, (uint32_t)n, step, 0xf0c3u, 0u, g);
     gauss4(seed, (uint32_t)n, step, 0xf0c3u, 1u, g + 4);
@@ -235,29 +240,37 @@ __global__ void k_force_modes(const Scalars* S, unsigned long long seed, double 
 }
 
 // F at the cell centres x = (i - 1/2) dx, three FP32 components at force +
-// m*stride in the field index map.
-__global__ void k_force_field(Geo g, const float* table, double dx0, double dx1, double dx2, double L0, double L1,
-                              double L2, float* force, long long stride) {
+// m*stride in the field index map.  trig holds, per axis d and cell index
+// 1..N_d, {cos, sin} of (2 pi / L_d) m x for m = 1, 2 (axes one after the
+// other); each mode's phase is the product of three of them.
+__global__ void k_force_field(Geo g, const float* table, const float4* trig, float* force, long long stride) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
     __shared__ float tab[96];
     for (int q = threadIdx.x; q < 96; q += blockDim.x) tab[q] = table[q];
     __syncthreads();
     if (i > g.N1) return;
-    const double x[3] = {(i - 0.5) * dx0, (j - 0.5) * dx1, (k - 0.5) * dx2};
-    const double Ls[3] = {L0, L1, L2};
-    // e^{i m x_d} for m = 0, 1, 2 along each axis
+    const float4 t[3] = {trig[i - 1], trig[g.N1 + j - 1], trig[g.N1 + g.N2 + k - 1]};
     float cr[3][3], ci[3][3];
-    for (int d = 0; d < 3; ++d)
-        for (int m = 0; m < 3; ++m) {
-            const double ph = kTwoPi / Ls[d] * m * x[d];
-            cr[d][m] = (float)cos(ph);
-            ci[d][m] = (float)sin(ph);
-        }
+    for (int d = 0; d < 3; ++d) {
+        cr[d][0] = 1.f;
+        ci[d][0] = 0.f;
+        cr[d][1] = t[d].x;
+        ci[d][1] = t[d].y;
+        cr[d][2] = t[d].z;
+        ci[d][2] = t[d].w;
+    }
+    // the 16 forced wavevectors in the order of c_modes (ark_set_forcing),
+    // as compile-time constants so every index below is one
+    constexpr int kM[16][3] = {{0, 0, 1},  {0, 0, 2},  {0, 1, -1},  {0, 1, 0},  {0, 1, 1},  {0, 2, 0},
+                               {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1}, {1, 0, 0},  {1, 0, 1},
+                               {1, 1, -1},  {1, 1, 0},  {1, 1, 1},  {2, 0, 0}};
     float F[3] = {0.f, 0.f, 0.f};
+#pragma unroll
     for (int n = 0; n < 16; ++n) {
         float er = 1.f, ei = 0.f;
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const int m = c_modes[n].m[d];
+            const int m = kM[n][d];
             const float r = cr[d][m < 0 ? -m : m], s = m < 0 ? -ci[d][-m] : ci[d][m];
             const float nr = er * r - ei * s, ni = er * s + ei * r;
             er = nr;
@@ -480,11 +493,11 @@ ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st) {
     if (!c->force || c->force_eps <= 0.0) return ARK_OK;
     const double L0 = c->prm.dx1 * c->geo.N1, L1 = c->prm.dx2 * c->geo.N2, L2 = c->prm.dx3 * c->geo.N3;
     hipLaunchKernelGGL(k_force_modes, dim3(1), dim3(64), 0, st, c->S, (unsigned long long)c->force_seed,
-                       c->force_eps, L0, L1, L2, c->force_table);
+                       c->force_eps, c->force_tau, L0, L1, L2, c->force_table);
     ark_status s = c->hip(hipGetLastError(), "force modes");
     if (s != ARK_OK) return s;
     hipLaunchKernelGGL(k_force_field, interior_grid(c->geo, 0), dim3(256), 0, st, c->geo, c->force_table,
-                       c->prm.dx1, c->prm.dx2, c->prm.dx3, L0, L1, L2, c->force, (long long)c->force_stride);
+                       (const float4*)c->force_trig, c->force, (long long)c->force_stride);
     return c->hip(hipGetLastError(), "force field");
 }
 
@@ -500,6 +513,26 @@ static ark_status ensure_force(ark_ctx* c) {
     c->force_table = p;
     c->force = p + 96 + 15;   // element (0,0,0) of component 0
     c->force_stride = (long long)c->elems;
+    // per-axis phase tables of the forced wavenumbers (C library cos/sin in FP64)
+    const int N[3] = {c->geo.N1, c->geo.N2, c->geo.N3};
+    const double dx[3] = {c->prm.dx1, c->prm.dx2, c->prm.dx3};
+    std::vector<float> h;
+    for (int d = 0; d < 3; ++d)
+        for (int i = 1; i <= N[d]; ++i) {
+            const double ph = kTwoPi / (dx[d] * N[d]) * (i - 0.5) * dx[d];
+            h.push_back((float)cos(ph));
+            h.push_back((float)sin(ph));
+            h.push_back((float)cos(2.0 * ph));
+            h.push_back((float)sin(2.0 * ph));
+        }
+    float* tr = nullptr;
+    if (s == ARK_OK) s = c->hip(hipMalloc(&tr, h.size() * sizeof(float)), "hipMalloc force tables");
+    if (s == ARK_OK) {
+        c->allocs.push_back(tr);
+        c->bytes += h.size() * sizeof(float);
+        c->force_trig = tr;
+        s = c->hip(hipMemcpy(tr, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice), "force tables");
+    }
     return s;
 }
 
@@ -654,8 +687,8 @@ ark_status ark_dissipation(ark_ctx* c, double* eps) {
     return s;
 }
 
-ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps) {
-    if (!c || !(eps >= 0.0)) return ARK_ERR_INVALID;
+ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps, double tau) {
+    if (!c || !(eps >= 0.0) || !(tau >= 0.0)) return ARK_ERR_INVALID;
     if (c->prm.precision != ARK_PREC_MIXED)
         return c->fail(ARK_ERR_UNSUPPORTED, "forcing: mixed-precision contexts only");
     ark_status s = ARK_OK;
@@ -669,11 +702,19 @@ ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps) {
                     const int first = a != 0 ? a : b != 0 ? b : d;
                     if (k2 >= 1 && k2 <= 4 && first > 0) h[q++] = ForceMode{{a, b, d}};
                 }
+        // the force-field kernel holds the same list as compile-time constants
+        static const int kHostM[16][3] = {{0, 0, 1},  {0, 0, 2},  {0, 1, -1},  {0, 1, 0},  {0, 1, 1},  {0, 2, 0},
+                                          {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1}, {1, 0, 0},  {1, 0, 1},
+                                          {1, 1, -1},  {1, 1, 0},  {1, 1, 1},  {2, 0, 0}};
+        for (int n = 0; n < 16; ++n)
+            for (int d = 0; d < 3; ++d)
+                if (q != 16 || h[n].m[d] != kHostM[n][d]) return c->fail(ARK_ERR_STATE, "forcing: mode table mismatch");
         s = c->hip(hipMemcpyToSymbol(HIP_SYMBOL(c_modes), h, sizeof h), "force modes");
     }
     if (s == ARK_OK && eps > 0.0) s = ensure_force(c);
     c->force_seed = seed;
     c->force_eps = eps;
+    c->force_tau = tau;
     return s;
 }
 

@@ -17,6 +17,11 @@ Printed: ONE JSON line.  Beyond the common fields it carries
                 scheme: the reference scheme goes unstable on this vortex)
   batched       64 independent boxes of uneven size and Reynolds number in one
                 launch per step (graph-replayed, tile shape autotuned)
+  forced_turbulence_closure
+                512^3 forced turbulence (2000 steps, kinetic energy and
+                dissipation series), the closure network trained on its
+                filtered snapshots, and 128^3 runs with and without it compared
+                with the filtered 512^3 field
   cpu_baseline  the unmodified reference (oracle/_ref, built from its own
                 sources) under mpirun on this host's cores, bounded sample
   --calibrate-copy prints the HBM bytes a plain device copy of known size
@@ -173,6 +178,43 @@ def batched(nboxes, steps, seed=4):
             "nonfinite": nonfinite}
 
 
+def closure_experiment(N, steps, width, train_steps, batch):
+    """Forced homogeneous turbulence at N^3 (the plain fine-grid CABARET step,
+    mixed precision): seeded solenoidal initial field E(k) ~ k^4 exp(-2 (k/4)^2),
+    u_rms 1, random-phase forcing on 1 <= |k| <= 2 at eps = 0.1 (redrawn every
+    0.01 in time), nu = 1/1600.  Its box-filtered snapshots (width w) train the
+    closure network; (N/w)^3 runs from the filtered initial field, with and
+    without the closure, are compared with the filtered fine run at the same
+    time by the kinetic-energy spectrum and the dissipation rate."""
+    from arkamd import closure as C
+    vis, eps, seed = 1.0 / 1600.0, 0.1, 31
+    d = C.dns(N, steps=steps, snapshots=6, width=width, vis=vis, eps=eps, seed=seed, series_every=100)
+    tr = C.train(d["X"], d["T"], steps=train_steps, batch=batch, lr=1e-3, seed=1)
+    Eref = C.spectrum_of(d["ub_end"], vis)
+    plain = C.les(d["ub0"], d["t_end"], vis, eps, seed)
+    model = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=tr["mlp"], delta=width * d["dx"])
+    tr["mlp"].close()
+    nc = N // width
+    out = {"fine": {"grid": N, "steps": steps, "time": d["t_end"], "wall_s": round(d["wall"], 3),
+                    "Mcell_steps_per_s": round(N ** 3 * steps / d["wall"] / 1e6, 1), "nonfinite": d["nonfinite"],
+                    "dissipation_end": d["eps_end"],
+                    "series": [{k: (float("%.6g" % v) if isinstance(v, float) else v) for k, v in r.items()}
+                               for r in d["series"]]},
+           "training": {"pairs": int(d["X"].shape[0]), "steps": train_steps, "batch": batch,
+                        "loss0": tr["loss0"], "loss_end": tr["loss_end"], "wall_s": round(tr["wall"], 3)},
+           "coarse_grid": nc, "filter_width_cells": width}
+    for name, r in (("plain", plain), ("closure", model)):
+        out[name] = {"steps": r["steps"], "wall_s": round(r["wall"], 4), "nonfinite": r["nonfinite"],
+                     "spectrum_err_k<=%d" % (nc // 4): round(C.spectrum_error(r["E"], Eref, nc // 4), 4),
+                     "spectrum_err_k<=%d" % (nc // 2): round(C.spectrum_error(r["E"], Eref, nc // 2), 4),
+                     "dissipation_resolved": r["eps_resolved"],
+                     "wall_ratio_fine_over_coarse": round(d["wall"] / r["wall"], 1)}
+    out["spectrum_filtered_fine"] = [float("%.5g" % v) for v in Eref[: nc // 2 + 1]]
+    out["spectrum_plain"] = [float("%.5g" % v) for v in plain["E"][: nc // 2 + 1]]
+    out["spectrum_closure"] = [float("%.5g" % v) for v in model["E"][: nc // 2 + 1]]
+    return out
+
+
 def accuracy(N, steps):
     import numpy as np
     import arkamd
@@ -236,6 +278,9 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--accuracy-steps", type=int, default=500)
+    ap.add_argument("--closure-size", type=int, default=512, help="fine grid of the closure experiment (0: skip)")
+    ap.add_argument("--closure-steps", type=int, default=2000)
+    ap.add_argument("--closure-train-steps", type=int, default=20000)
     ap.add_argument("--batch-boxes", type=int, default=64, help="boxes of the batched run (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--phase-steps", type=int, default=3,
@@ -298,6 +343,9 @@ def main():
         line["phase_kernels_fp64"] = phase_kernels(N, args.phase_steps)
     if args.batch_boxes > 0 and not args.only:
         line["batched"] = batched(args.batch_boxes, args.batch_steps)
+    if args.closure_size > 0 and not args.only:
+        line["forced_turbulence_closure"] = closure_experiment(args.closure_size, args.closure_steps, 4,
+                                                               args.closure_train_steps, 65536)
     if args.accuracy_steps > 0 and not args.only:
         line["accuracy"] = accuracy(N, args.accuracy_steps)
     base = None

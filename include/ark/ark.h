@@ -235,10 +235,14 @@ ark_status ark_batch_get_config(const ark_batch* batch, int* shape, int* lz, int
  * ark_dissipation: nu <sum_ij (d_j u_i)^2> with central differences.
  *
  * ark_set_forcing (mixed precision): random-phase forcing of the 16 pairs of
- * wavevectors with 1 <= |m|^2 <= 4; each step every mode gets a fresh
- * random amplitude vector normal to k (Philox, key = seed, counter = (mode,
- * nStep, 0xf0c3, 0|1)) of norm sqrt(eps/(16 dt)), so the mean injection
- * rate is eps per unit mass.  eps = 0 switches it off.
+ * wavevectors with 1 <= |m|^2 <= 4.  With tau = 0 every step gives every
+ * mode a fresh random amplitude vector normal to k (Philox, key = seed,
+ * counter = (mode, nStep, 0xf0c3, 0|1)) of norm sqrt(eps/(16 dt)); with
+ * tau > 0 the amplitudes are redrawn once per interval of length tau
+ * (counter = the interval of the step's mid time) with norm
+ * sqrt(eps/(16 tau)), so runs of different resolution see the same force
+ * history.  Either way the mean injection rate is eps per unit mass.
+ * eps = 0 switches it off.
  *
  * ark_filter_box: the box filter of width w cells (w divides N) of the
  * level-n velocity: ubar (3 x Nc), the subgrid stresses tau = <u_i u_j> -
@@ -248,7 +252,7 @@ ark_status ark_batch_get_config(const ark_batch* batch, int* shape, int* lz, int
 ark_status ark_init_turbulence(ark_ctx* ctx, unsigned long long seed, double k0, double urms);
 ark_status ark_spectrum(ark_ctx* ctx, int nbins, double* E);
 ark_status ark_dissipation(ark_ctx* ctx, double* eps);
-ark_status ark_set_forcing(ark_ctx* ctx, unsigned long long seed, double eps);
+ark_status ark_set_forcing(ark_ctx* ctx, unsigned long long seed, double eps, double tau);
 ark_status ark_filter_box(ark_ctx* ctx, int width, double* ubar, double* tau, double* grad);
 
 /* Learned subgrid closure (no counterpart in the reference): an MLP per cell,

@@ -87,6 +87,19 @@ def test_forcing_injects_eps_dt_in_one_step():
         assert abs(ke / (eps * d["dt"]) - 1.0) < 1e-2, ke / (eps * d["dt"])
 
 
+def test_forcing_with_correlation_time():
+    """Amplitudes held over intervals of tau: from rest one step gives
+    u = F dt with <|F|^2> = 2 eps / tau, so KE = eps dt^2 / tau; and the force
+    depends on time only, not on the step count."""
+    eps, tau = 0.1, 0.05
+    with rest((32, 32, 32)) as s:
+        s.set_forcing(seed=3, eps=eps, tau=tau)
+        s.step(1)
+        d = s.diagnostics()
+        ke = 0.5 * sum((a * a).mean() for a in cells_of(s))
+        assert abs(ke / (eps * d["dt"] ** 2 / tau) - 1.0) < 1e-2
+
+
 def test_forcing_is_seeded_and_solenoidal():
     N = (32, 32, 32)
     out = []
