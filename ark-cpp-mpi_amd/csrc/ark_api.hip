@@ -669,14 +669,17 @@ ark_status ark_finish_init(ark_ctx* c) {
     c->sigma_dirty = false;
     if (s == ARK_OK) {
         // fresh run: TIME = 0, nStep = 0, running MIN reset
+        // (ordered on the context's stream, after the kernels that set dt_init)
         Scalars hs{};
-        s = c->hip(hipMemcpy(&hs, c->S, sizeof hs, hipMemcpyDeviceToHost), "scalars");
+        s = c->hip(hipMemcpyAsync(&hs, c->S, sizeof hs, hipMemcpyDeviceToHost, c->stream), "scalars");
+        if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
         hs.time = 0.0;
         hs.nstep = 0;
         hs.dt = 0.0;
         hs.dt_bits = hs.dt_init;
         hs.nonfinite = 0;
-        if (s == ARK_OK) s = c->hip(hipMemcpy(c->S, &hs, sizeof hs, hipMemcpyHostToDevice), "scalars");
+        if (s == ARK_OK) s = c->hip(hipMemcpyAsync(c->S, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream), "scalars");
+        if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     }
     return s;
 }

@@ -47,6 +47,8 @@ struct ark_ctx {
     double closure_delta = 0.0;
     float* les_X = nullptr;
     float* les_Y = nullptr;
+    void* mixed_units = nullptr;       // chunk table of the mixed-precision step (int4 per chunk)
+    int mixed_nunits = 0;
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;

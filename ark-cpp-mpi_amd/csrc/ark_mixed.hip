@@ -272,21 +272,25 @@ struct PlaneIn {
     float fo[3];    // body acceleration at the lane's cell (forcing)
 };
 
-// One tile (tile indices tbx, tby, tbz) of one box.  At most 256 VGPRs: two
-// waves per SIMD.
+// One tile of one box: tile column (tbx, tby), owned planes k0 .. kend-1.
+// A lane keeps in registers only what is not also in LDS: its own x- and
+// y-faces and the FP64 velocities of its cells are read back from the
+// staged planes, which keeps the tile within 168 VGPRs.
 template <bool Quirk, class S>
-__device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int tbz) {
+__device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int k0, int kend) {
     constexpr int TX = S::TX, TY = S::TY, RX = S::RX, RY = S::RY, NTHR = S::NTHR;
     __shared__ float4 s_h[NTHR];             // half level: u1, u2, u3, t'
     __shared__ float4 s_fx[NTHR], s_fy[NTHR];  // forward extrapolations along x, y
     __shared__ float s_m[10][NTHR];          // new x- and y-face fluxes
     // Level-n velocities of a plane and the x- and y-faces of each lane's
     // cell, with a halo that the edge lanes fill with the value beyond the
-    // region, so every neighbour read is one unconditional LDS load.
+    // region, so every neighbour read is one unconditional LDS load.  The
+    // velocities of two planes are held (p and p+1); the faces of one plane,
+    // read before the plane's first barrier and rewritten after it.
     constexpr int UX = RX + 2, UN = (RX + 2) * (RY + 2);   // velocities: one-cell halo all round
     constexpr int FX = RX + 1, FN = (RX + 1) * (RY + 1);   // faces: one more column and row
     __shared__ double s_u[2][3][UN];
-    __shared__ float s_f[2][10][FN];
+    __shared__ float s_f[10][FN];
     const Geo& g = a.g;
     const double dtd = a.S->dt;
     const float dt = (float)dtd, inv_dt2 = (float)(2.0 / dtd);
@@ -294,8 +298,8 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     const float cf = (float)cd;
     const int tid = threadIdx.x;
     const int tx = tid % RX, ty = tid / RX;
-    const int i0 = tbx * TX + 1, j0 = tby * TY + 1, k0 = tbz * a.LZ + 1;
-    const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = min(k0 + a.LZ, g.N3 + 1);
+    const int i0 = tbx * TX + 1, j0 = tby * TY + 1;
+    const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1);
     const int i = min(i0 - 1 + tx, iend), j = min(j0 - 1 + ty, jend);
     const bool in_region = (i0 - 1 + tx <= iend) && (j0 - 1 + ty <= jend);
     const bool own_x = i >= i0 && i < iend, own_y = j >= j0 && j < jend;
@@ -348,7 +352,7 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     // is written separately, from registers that are already there).
     auto stage = [&](int p, const PlaneIn& r) {
         if (!in_region) return;
-        float* f = &s_f[p & 1][0][0];
+        float* f = &s_f[0][0];
         put_face(f, fpos, r.xw);
         put_face(f + 5 * FN, fpos, r.ys);
         if (put_ex) {
@@ -362,8 +366,8 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         if (in_region)
             for (int m = 0; m < 3; ++m) s_u[p & 1][m][upos] = c.u[m];
     };
-    auto staged_face = [&](int p, int q, int off) {
-        const float* f = &s_f[p & 1][off][0];
+    auto staged_face = [&](int q, int off) {
+        const float* f = &s_f[off][0];
         FaceF r;
         r.u[0] = f[0 * FN + q]; r.u[1] = f[1 * FN + q]; r.u[2] = f[2 * FN + q];
         r.p = f[3 * FN + q]; r.tp = f[4 * FN + q];
@@ -397,14 +401,18 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         asm volatile("" : "+s"(A.fb));
         const bool inner = p >= k0 && p < kend;
         // prefetch plane p+1 (and the cell two planes up, for its z-stress)
+        // plane p+1's velocities into LDS (it is read there from now on), then
+        // prefetch plane p+1 (and the cell two planes up, for its z-stress)
+        put_u(p + 1, Anext);
         PlaneIn nxt = cur;
         CellD Anext2 = Anext;
         if (p < kend) {
-            put_u(p + 1, Anext);
             nxt = load_plane(p + 1);
             Anext2 = load_cell(A, base + wrapc(p + 2, g.N3) * sxy);
         }
-        const CellF Af = to_f(Acur, A);
+        CellD Ac = Acur;
+        for (int m = 0; m < 3; ++m) Ac.u[m] = s_u[p & 1][m][upos];
+        const CellF Af = to_f(Ac, A);
 
         // stress force (FP32 differences of FP64 velocities)
         float f[3];
@@ -412,14 +420,14 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
             const float roc = (float)Acur.ro;
             for (int m = 0; m < 3; ++m) {
                 const double* u = &s_u[p & 1][m][0];
-                const double uc = Acur.u[m];
+                const double uc = Ac.u[m];
                 const double uW = u[upos - 1], uE = u[upos + 1];
                 const double uS = u[upos - UX], uN = u[upos + UX];
                 const float sxl = A.VIS * (float)(uc - uW) * A.inv_dx[0];
                 const float sxh = A.VIS * (float)(uE - uc) * A.inv_dx[0];
                 const float syl = A.VIS * (float)(uc - uS) * A.inv_dx[1];
                 const float syh = A.VIS * (float)(uN - uc) * A.inv_dx[1];
-                const float szh = A.VIS * (float)(Anext.u[m] - uc) * A.inv_dx[2];
+                const float szh = A.VIS * (float)(s_u[(p + 1) & 1][m][upos] - uc) * A.inv_dx[2];
                 f[m] = ((sxh - sxl) * A.ds[0] + (syh - syl) * A.ds[1] + (szh - sz_lo[m]) * A.ds[2]) * roc +
                        roc * A.dvcf * cur.fo[m];
                 sz_lo[m] = szh;
@@ -431,12 +439,12 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         CellF H;
         FwdF fx, fy, fz;
         BwdF bx, by, bz;
-        float ox[3], oy[3], oz[3];
+        float oz[3];
         {
-            const FaceF& Xw = cur.xw;
-            const FaceF Xe = staged_face(p, fpos + 1, 0);
-            const FaceF& Ys = cur.ys;
-            const FaceF Yn = staged_face(p, fpos + FX, 5);
+            const FaceF Xw = staged_face(fpos, 0);
+            const FaceF Xe = staged_face(fpos + 1, 0);
+            const FaceF Ys = staged_face(fpos, 5);
+            const FaceF Yn = staged_face(fpos + FX, 5);
             const FaceF& Zt = cur.zt;
             {
                 const Flux a0 = flux<0>(Xw.u, Xw.p, Xw.tp, A), a1 = flux<0>(Xe.u, Xe.p, Xe.tp, A);
@@ -468,8 +476,6 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
                 fz = forward(s, dt, inv_dt2, A.inv_dx[2], A);
                 bz = backward<Quirk>(s, dt, inv_dt2, A.inv_dx[2], A);
             }
-            ox[0] = Xw.tp; ox[1] = Xw.u[1]; ox[2] = Xw.u[2];
-            oy[0] = Ys.tp; oy[1] = Ys.u[0]; oy[2] = Ys.u[2];
             oz[0] = zb.tp; oz[1] = zb.u[0]; oz[2] = zb.u[1];
         }
         if (owned && inner) {
@@ -519,8 +525,9 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
                 if (xface) {
                     const float4 nb = s_h[tW], fw = s_fx[tW];
                     const FwdF fwb{fw.x, fw.y, fw.z, fw.w};
-                    const FaceOutF o = combine(fwb, bx, nb.x, H.u[0], nb.w, H.tp, nb.y, H.u[1], nb.z, H.u[2], ox[0],
-                                               ox[1], ox[2], A);
+                    // the old tangential values of the lane's own x-face, from LDS
+                    const FaceOutF o = combine(fwb, bx, nb.x, H.u[0], nb.w, H.tp, nb.y, H.u[1], nb.z, H.u[2],
+                                               s_f[4][fpos], s_f[1][fpos], s_f[2][fpos], A);
                     if (own_x) {
                         store_face<0>(A, at(g, i, j, p), o);
                         if (i == 1) store_face<0>(A, at(g, g.N1 + 1, j, p), o);
@@ -530,8 +537,8 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
                 if (yface) {
                     const float4 nb = s_h[tS], fw = s_fy[tS];
                     const FwdF fwb{fw.x, fw.y, fw.z, fw.w};
-                    const FaceOutF o = combine(fwb, by, nb.y, H.u[1], nb.w, H.tp, nb.x, H.u[0], nb.z, H.u[2], oy[0],
-                                               oy[1], oy[2], A);
+                    const FaceOutF o = combine(fwb, by, nb.y, H.u[1], nb.w, H.tp, nb.x, H.u[0], nb.z, H.u[2],
+                                               s_f[9][fpos], s_f[5][fpos], s_f[7][fpos], A);
                     if (own_y) {
                         store_face<1>(A, at(g, i, j, p), o);
                         if (j == 1) store_face<1>(A, at(g, i, g.N2 + 1, p), o);
@@ -552,7 +559,7 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         for (int v = 0; v < 5; ++v) {
             double inc = (double)Sp[v] + (double)(Sxy[v] - mz.m[v] * A.ds[2]);
             if (v >= 1 && v <= 3) inc -= 2.0 * (double)f[v - 1];
-            const double q = v == 0 ? Acur.ro : Acur.ro * (v == 4 ? Acur.t - A.t0d : Acur.u[v - 1]);
+            const double q = v == 0 ? Acur.ro : Acur.ro * (v == 4 ? Acur.t - A.t0d : s_u[p & 1][v - 1][upos]);
             cons_prev[v] = q - cd * inc;
         }
         zb = cur.zt;
@@ -563,11 +570,21 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     block_min_atomic(dtbits, A.dt_next);
 }
 
+// One box, one workgroup per chunk of a tile column.  units[u] = (tile
+// column x fastest then y, first plane, end plane) lists the chunks in
+// rounds: round q holds chunk q of every tile column, so the workgroups that
+// run together work at the same height and their rings come from L2.  The
+// host sizes the chunks (mixed_units) so the count is a multiple of the
+// workgroups the chip holds at once, with near-equal heights, and no round
+// finishes with the chip part-empty.  Die (XCD) b % 8 takes a contiguous run
+// of chunks, so the columns in flight on one die are neighbours.
 template <bool Quirk, class S>
-__global__ void __launch_bounds__(S::NTHR) k_fused_mixed(Args a) {
-    const TileId t = die_major_tile((a.g.N1 + S::TX - 1) / S::TX, (a.g.N2 + S::TY - 1) / S::TY,
-                                    (a.g.N3 + a.LZ - 1) / a.LZ);
-    mixed_tile<Quirk, S>(a, t.x, t.y, t.z);
+__global__ void __launch_bounds__(S::NTHR) k_fused_mixed(Args a, const int4* __restrict__ units, int nbx) {
+    const int U = gridDim.x, b = blockIdx.x;
+    const int die = b % kDies, local = b / kDies;
+    const int q = U / kDies, r = U % kDies;
+    const int4 w = units[die * q + min(die, r) + local];
+    mixed_tile<Quirk, S>(a, w.x % nbx, w.x / nbx, w.y, w.z);
 }
 
 // Many boxes in one launch.  start[b] is the first global tile of box b
@@ -591,7 +608,8 @@ __global__ void __launch_bounds__(S::NTHR) k_fused_mixed_batch(const Args* __res
     const Args& a = boxes[lo];
     const int tl = t - start[lo];
     const int nbx = (a.g.N1 + S::TX - 1) / S::TX, nby = (a.g.N2 + S::TY - 1) / S::TY;
-    mixed_tile<Quirk, S>(a, tl % nbx, (tl / nbx) % nby, tl / (nbx * nby));
+    const int k0 = (tl / (nbx * nby)) * a.LZ + 1;
+    mixed_tile<Quirk, S>(a, tl % nbx, (tl / nbx) % nby, k0, min(k0 + a.LZ, a.g.N3 + 1));
 }
 
 // dt of a new step for every box of a batch (k_cfl_finalize, mode 1, per box).
@@ -642,16 +660,6 @@ int tiles_of(const Geo& g, int shape, int LZ) {
            ((g.N3 + LZ - 1) / LZ);
 }
 
-// Planes per chunk: enough tiles to fill the chip several times over, long
-// enough chunks that the two ring planes per chunk stay cheap.
-int choose_lz_mixed(const Geo& g, int shape) {
-    long long cols = (long long)((g.N1 + kShapeTX[shape] - 1) / kShapeTX[shape]) *
-                     ((g.N2 + kShapeTY[shape] - 1) / kShapeTY[shape]);
-    int lz = 64;
-    while (lz > 8 && cols * ((g.N3 + lz - 1) / lz) < 2048) lz /= 2;
-    return lz < g.N3 ? lz : g.N3;
-}
-
 // Kernel arguments of one mixed step of context c: read the current buffers,
 // write the other ones; dt_next is where the next step's dt MIN goes.
 mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, int flip = 0) {
@@ -693,9 +701,67 @@ mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, int
 }
 
 template <class S>
-void launch_single(bool quirk, dim3 grid, hipStream_t st, const mixed::Args& a) {
-    if (quirk) hipLaunchKernelGGL((mixed::k_fused_mixed<true, S>), grid, dim3(S::NTHR), 0, st, a);
-    else hipLaunchKernelGGL((mixed::k_fused_mixed<false, S>), grid, dim3(S::NTHR), 0, st, a);
+void launch_single(bool quirk, int units, const int4* table, hipStream_t st, const mixed::Args& a) {
+    const int nbx = (a.g.N1 + S::TX - 1) / S::TX;
+    if (quirk) hipLaunchKernelGGL((mixed::k_fused_mixed<true, S>), dim3(units), dim3(S::NTHR), 0, st, a, table, nbx);
+    else hipLaunchKernelGGL((mixed::k_fused_mixed<false, S>), dim3(units), dim3(S::NTHR), 0, st, a, table, nbx);
+}
+
+// Chunks of the single-box step.  With T tile columns, N3 planes and W
+// workgroups resident at once (CUs x workgroups per CU), U chunks give each
+// column U/T or U/T + 1 near-equal chunks (+2 ring planes each).  U is taken
+// among the multiples of W (or all T*N3 planes, if fewer) by the finishing
+// time of in-order dispatch onto each die's share of the workgroups, in
+// units of plane-steps.
+template <class S>
+std::vector<int4> mixed_units(const Geo& g, int W) {
+    const int nbx = (g.N1 + S::TX - 1) / S::TX, nby = (g.N2 + S::TY - 1) / S::TY;
+    const long long T = (long long)nbx * nby;
+    auto chunks_of = [&](long long U, long long t) { return (int)(U / T + (t < U % T ? 1 : 0)); };
+    auto build = [&](long long U) {
+        std::vector<int4> u;
+        u.reserve((size_t)U);
+        const int cmax = chunks_of(U, 0);
+        for (int c = 0; c < cmax; ++c)
+            for (long long t = 0; t < T; ++t) {
+                const int n = chunks_of(U, t);
+                if (c >= n) continue;
+                const int k0 = 1 + (int)((long long)g.N3 * c / n), k1 = 1 + (int)((long long)g.N3 * (c + 1) / n);
+                u.push_back(make_int4((int)t, k0, k1, 0));
+            }
+        return u;
+    };
+    auto finish = [&](const std::vector<int4>& u) {
+        const long long n = (long long)u.size();
+        const int per_die = std::max(1, W / kDies);
+        long long worst = 0;
+        for (int d = 0; d < kDies; ++d) {
+            const long long q = n / kDies, r = n % kDies;
+            const long long lo = d * q + std::min<long long>(d, r), hi = lo + q + (d < r ? 1 : 0);
+            std::vector<long long> free_at(per_die, 0);
+            for (long long i = lo; i < hi; ++i) {
+                auto m = std::min_element(free_at.begin(), free_at.end());
+                *m += u[i].z - u[i].y + 2;
+            }
+            worst = std::max(worst, *std::max_element(free_at.begin(), free_at.end()));
+        }
+        return worst;
+    };
+    const long long planes = T * g.N3;
+    if (planes <= W) return build(planes);
+    std::vector<int4> best;
+    long long best_t = -1;
+    for (long long U = W; U <= planes && U <= 64LL * W; U += W) {
+        if (U < T) continue;
+        std::vector<int4> u = build(U);
+        const long long f = finish(u);
+        if (best_t < 0 || f < best_t) {
+            best_t = f;
+            best.swap(u);
+        }
+    }
+    if (best_t < 0) best = build(std::min<long long>(planes, ((T + W - 1) / W) * W));
+    return best;
 }
 
 template <class S>
@@ -704,11 +770,14 @@ void launch_batch(bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxe
     else hipLaunchKernelGGL((mixed::k_fused_mixed_batch<false, S>), grid, dim3(S::NTHR), 0, st, boxes, start, n);
 }
 
-hipError_t launch_shape_single(int shape, bool quirk, dim3 grid, hipStream_t st, const mixed::Args& a) {
-    if (shape == 0) launch_single<mixed::Shape30x6>(quirk, grid, st, a);
-    else if (shape == 1) launch_single<mixed::Shape30x14>(quirk, grid, st, a);
-    else launch_single<mixed::Shape62x6>(quirk, grid, st, a);
-    return hipGetLastError();
+template <class S>
+int resident_blocks(bool quirk) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    const void* fn = quirk ? (const void*)mixed::k_fused_mixed<true, S> : (const void*)mixed::k_fused_mixed<false, S>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, S::NTHR, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return cus * per_cu;
 }
 
 hipError_t launch_shape_batch(int shape, bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxes,
@@ -719,17 +788,31 @@ hipError_t launch_shape_batch(int shape, bool quirk, dim3 grid, hipStream_t st, 
     return hipGetLastError();
 }
 
-struct MixedLaunch {
-    ark_ctx* c;
-    int LZ;
-};
+using SingleShape = mixed::Shape30x14;
 
 hipError_t run_mixed(void* p) {
-    MixedLaunch* ml = (MixedLaunch*)p;
-    ark_ctx* c = ml->c;
-    const mixed::Args a = make_args(c, ml->LZ, &c->S->dt_bits);
-    dim3 grid(tiles_of(c->geo, kDefaultShape, ml->LZ));
-    return launch_shape_single(kDefaultShape, c->prm.scheme == ARK_SCHEME_REFERENCE, grid, c->stream, a);
+    ark_ctx* c = (ark_ctx*)p;
+    const mixed::Args a = make_args(c, 0, &c->S->dt_bits);
+    launch_single<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE, c->mixed_nunits, (const int4*)c->mixed_units,
+                               c->stream, a);
+    return hipGetLastError();
+}
+
+// The chunk table of the single-box step, built once per context.
+ark_status ensure_units(ark_ctx* c) {
+    if (c->mixed_units) return ARK_OK;
+    const std::vector<int4> u =
+        mixed_units<SingleShape>(c->geo, resident_blocks<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE));
+    void* d = nullptr;
+    ark_status s = c->hip(hipMalloc(&d, u.size() * sizeof(int4)), "hipMalloc chunk table");
+    if (s != ARK_OK) return s;
+    c->allocs.push_back(d);
+    c->bytes += u.size() * sizeof(int4);
+    s = c->hip(hipMemcpy(d, u.data(), u.size() * sizeof(int4), hipMemcpyHostToDevice), "chunk table");
+    if (s != ARK_OK) return s;
+    c->mixed_units = d;
+    c->mixed_nunits = (int)u.size();
+    return ARK_OK;
 }
 
 int face_mode(int f) { return (f == F_t1 || f == F_t2 || f == F_t3) ? 1 : 0; }
@@ -740,12 +823,12 @@ ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)
         return c->fail(ARK_ERR_UNSUPPORTED, "mixed precision needs N1, N2, N3 >= 2");
     ark_status s = ARK_OK;
-    MixedLaunch ml{c, choose_lz_mixed(c->geo, kDefaultShape)};
+    s = ensure_units(c);
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
         if (s == ARK_OK) s = ark_turb_force_step(c, c->stream);
         if (s == ARK_OK) s = ark_closure_step(c, c->stream);
-        if (s == ARK_OK) s = c->launch("fused_mixed", run_mixed, &ml);
+        if (s == ARK_OK) s = c->launch("fused_mixed", run_mixed, c);
         if (s != ARK_OK) break;
         for (int g = 0; g < 4; ++g) ark_swap_group(c, g);
         c->dt_pending = true;

@@ -337,6 +337,7 @@ __global__ void __launch_bounds__(THREADS) k_mlp_forward(const float* X, long lo
 struct ark_mlp {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;                          // recorded on `stream` before another stream reads P, Wb
     float *P = nullptr, *M = nullptr, *V = nullptr;   // FP32 weights, Adam moments
     __bf16* Wb = nullptr;
     double* G = nullptr;                                // last gradient (and loss sum)
@@ -400,6 +401,7 @@ ark_status ark_mlp_create(int device, unsigned long long seed, ark_mlp** out) {
     m->seed = seed;
     ark_status s = m->hip(hipSetDevice(device), "hipSetDevice");
     if (s == ARK_OK) s = m->hip(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream");
+    if (s == ARK_OK) s = m->hip(hipEventCreateWithFlags(&m->done, hipEventDisableTiming), "event");
     if (s == ARK_OK) s = m->hip(hipMalloc(&m->P, 3 * NPARAM * sizeof(float)), "hipMalloc mlp");
     if (s == ARK_OK) s = m->hip(hipMalloc(&m->Wb, NBF * sizeof(__bf16)), "hipMalloc mlp");
     if (s == ARK_OK) s = m->hip(hipMalloc(&m->G, (NPARAM + 1) * sizeof(double)), "hipMalloc mlp");
@@ -418,9 +420,13 @@ ark_status ark_mlp_create(int device, unsigned long long seed, ark_mlp** out) {
         fill(OFF_W1, NH, NI, 1);
         fill(OFF_W2, NH, NH, 2);
         fill(OFF_W3, NO, NH, 3);
-        s = m->hip(hipMemcpy(m->P, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice), "mlp init");
+        s = m->hip(hipMemcpyAsync(m->P, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, m->stream),
+                   "mlp init");
+        if (s == ARK_OK) s = m->hip(hipStreamSynchronize(m->stream), "sync");   // h is freed at the end of this block
     }
-    if (s == ARK_OK) s = m->hip(hipMemset(m->Wb, 0, NBF * sizeof(__bf16)), "mlp init");
+    // Every fill, copy and kernel of the network is ordered on its own stream
+    // (a non-blocking stream, unordered with the null stream's hipMemset).
+    if (s == ARK_OK) s = m->hip(hipMemsetAsync(m->Wb, 0, NBF * sizeof(__bf16), m->stream), "mlp init");
     if (s == ARK_OK) {
         hipLaunchKernelGGL(k_refresh, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, m->P, m->Wb);
         s = m->hip(hipGetLastError(), "refresh");
@@ -441,6 +447,7 @@ void ark_mlp_destroy(ark_mlp* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (void* p : {(void*)m->P, (void*)m->Wb, (void*)m->G, (void*)m->partial, (void*)m->X, (void*)m->T})
         if (p) (void)hipFree(p);
+    if (m->done) (void)hipEventDestroy(m->done);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -449,8 +456,9 @@ const char* ark_mlp_last_error(const ark_mlp* m) { return m ? m->err.c_str() : "
 
 ark_status ark_mlp_set_params(ark_mlp* m, const float* params) {
     if (!m || !params) return ARK_ERR_INVALID;
-    ark_status s = m->hip(hipMemcpy(m->P, params, NPARAM * sizeof(float), hipMemcpyHostToDevice), "set params");
-    if (s == ARK_OK) s = m->hip(hipMemset(m->M, 0, 2 * NPARAM * sizeof(float)), "reset moments");
+    ark_status s = m->hip(hipMemcpyAsync(m->P, params, NPARAM * sizeof(float), hipMemcpyHostToDevice, m->stream),
+                          "set params");
+    if (s == ARK_OK) s = m->hip(hipMemsetAsync(m->M, 0, 2 * NPARAM * sizeof(float), m->stream), "reset moments");
     m->adam_t = 0;
     if (s == ARK_OK) {
         hipLaunchKernelGGL(k_refresh, dim3((NPARAM + 255) / 256), dim3(256), 0, m->stream, m->P, m->Wb);
@@ -461,8 +469,9 @@ ark_status ark_mlp_set_params(ark_mlp* m, const float* params) {
 
 ark_status ark_mlp_get_params(ark_mlp* m, float* params) {
     if (!m || !params) return ARK_ERR_INVALID;
-    ark_status s = m->hip(hipStreamSynchronize(m->stream), "sync");
-    return s == ARK_OK ? m->hip(hipMemcpy(params, m->P, NPARAM * sizeof(float), hipMemcpyDeviceToHost), "get params") : s;
+    ark_status s = m->hip(hipMemcpyAsync(params, m->P, NPARAM * sizeof(float), hipMemcpyDeviceToHost, m->stream),
+                          "get params");
+    return s == ARK_OK ? m->hip(hipStreamSynchronize(m->stream), "sync") : s;
 }
 
 ark_status ark_mlp_set_normalization(ark_mlp* m, const float* in_mean, const float* in_std, const float* out_mean,
@@ -484,13 +493,17 @@ ark_status ark_mlp_set_normalization(ark_mlp* m, const float* in_mean, const flo
 
 ark_status ark_mlp_set_data(ark_mlp* m, const float* X, const float* T, long long n) {
     if (!m || !X || !T || n < 1) return ARK_ERR_INVALID;
+    ark_status s = m->hip(hipStreamSynchronize(m->stream), "sync");
     for (float* p : {m->X, m->T})
         if (p) (void)hipFree(p);
     m->X = m->T = nullptr;
-    ark_status s = m->hip(hipMalloc(&m->X, n * NI * sizeof(float)), "hipMalloc data");
+    if (s == ARK_OK) s = m->hip(hipMalloc(&m->X, n * NI * sizeof(float)), "hipMalloc data");
     if (s == ARK_OK) s = m->hip(hipMalloc(&m->T, n * NO * sizeof(float)), "hipMalloc data");
-    if (s == ARK_OK) s = m->hip(hipMemcpy(m->X, X, n * NI * sizeof(float), hipMemcpyHostToDevice), "data");
-    if (s == ARK_OK) s = m->hip(hipMemcpy(m->T, T, n * NO * sizeof(float), hipMemcpyHostToDevice), "data");
+    if (s == ARK_OK)
+        s = m->hip(hipMemcpyAsync(m->X, X, n * NI * sizeof(float), hipMemcpyHostToDevice, m->stream), "data");
+    if (s == ARK_OK)
+        s = m->hip(hipMemcpyAsync(m->T, T, n * NO * sizeof(float), hipMemcpyHostToDevice, m->stream), "data");
+    if (s == ARK_OK) s = m->hip(hipStreamSynchronize(m->stream), "sync");
     m->ndata = s == ARK_OK ? n : 0;
     return s;
 }
@@ -500,8 +513,10 @@ ark_status ark_mlp_gradient(ark_mlp* m, const float* X, const float* T, long lon
     float *dX = nullptr, *dT = nullptr;
     ark_status s = m->hip(hipMalloc(&dX, n * NI * sizeof(float)), "hipMalloc");
     if (s == ARK_OK) s = m->hip(hipMalloc(&dT, n * NO * sizeof(float)), "hipMalloc");
-    if (s == ARK_OK) s = m->hip(hipMemcpy(dX, X, n * NI * sizeof(float), hipMemcpyHostToDevice), "copy");
-    if (s == ARK_OK) s = m->hip(hipMemcpy(dT, T, n * NO * sizeof(float), hipMemcpyHostToDevice), "copy");
+    if (s == ARK_OK)
+        s = m->hip(hipMemcpyAsync(dX, X, n * NI * sizeof(float), hipMemcpyHostToDevice, m->stream), "copy");
+    if (s == ARK_OK)
+        s = m->hip(hipMemcpyAsync(dT, T, n * NO * sizeof(float), hipMemcpyHostToDevice, m->stream), "copy");
     if (s == ARK_OK) s = run_grad(m, dX, dT, n, n, 0, 0, 0, 0.f);
     std::vector<double> h(NPARAM + 1);
     if (s == ARK_OK) s = m->hip(hipMemcpyAsync(h.data(), m->G, h.size() * sizeof(double), hipMemcpyDeviceToHost, m->stream), "copy");
@@ -535,13 +550,15 @@ ark_status ark_mlp_predict(ark_mlp* m, const float* X, float* Y, long long n) {
     float *dX = nullptr, *dY = nullptr;
     ark_status s = m->hip(hipMalloc(&dX, n * NI * sizeof(float)), "hipMalloc");
     if (s == ARK_OK) s = m->hip(hipMalloc(&dY, n * NO * sizeof(float)), "hipMalloc");
-    if (s == ARK_OK) s = m->hip(hipMemcpy(dX, X, n * NI * sizeof(float), hipMemcpyHostToDevice), "copy");
+    if (s == ARK_OK)
+        s = m->hip(hipMemcpyAsync(dX, X, n * NI * sizeof(float), hipMemcpyHostToDevice, m->stream), "copy");
     if (s == ARK_OK) {
         hipLaunchKernelGGL(k_mlp_forward, dim3(kGradBlocks), dim3(THREADS), 0, m->stream, dX, n, m->P, m->Wb, m->nm, dY);
         s = m->hip(hipGetLastError(), "mlp forward");
     }
+    if (s == ARK_OK)
+        s = m->hip(hipMemcpyAsync(Y, dY, n * NO * sizeof(float), hipMemcpyDeviceToHost, m->stream), "copy");
     if (s == ARK_OK) s = m->hip(hipStreamSynchronize(m->stream), "sync");
-    if (s == ARK_OK) s = m->hip(hipMemcpy(Y, dY, n * NO * sizeof(float), hipMemcpyDeviceToHost), "copy");
     if (dX) (void)hipFree(dX);
     if (dY) (void)hipFree(dY);
     return s;
@@ -549,8 +566,12 @@ ark_status ark_mlp_predict(ark_mlp* m, const float* X, float* Y, long long n) {
 
 }  // extern "C"
 
-// Inference on device arrays (used by the closure on the solver's stream).
+// Inference on device arrays (used by the closure on the solver's stream),
+// ordered after everything already enqueued on the network's own stream.
 ark_status ark_mlp_forward_device(ark_mlp* m, const float* X, float* Y, long long n, hipStream_t st) {
+    ark_status s = m->hip(hipEventRecord(m->done, m->stream), "event");
+    if (s == ARK_OK) s = m->hip(hipStreamWaitEvent(st, m->done, 0), "wait");
+    if (s != ARK_OK) return s;
     hipLaunchKernelGGL(k_mlp_forward, dim3(kGradBlocks), dim3(THREADS), 0, st, X, n, m->P, m->Wb, m->nm, Y);
     return m->hip(hipGetLastError(), "mlp forward");
 }

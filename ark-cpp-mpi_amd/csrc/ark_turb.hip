@@ -81,9 +81,34 @@ __global__ void k_shape(Spec s, hipfftDoubleComplex* u, double k0) {
     }
 }
 
+// Sums in a fixed order, so every result is the same bit for bit from run to
+// run: a grid-stride loop over a fixed grid of kSumBlocks x 256 threads, a
+// tree over each block (block_sum), one partial per block, and the partials
+// summed by one block (k_sum_parts).
+constexpr int kSumBlocks = 1024;
+
+__device__ double block_sum(double v) {
+    __shared__ double red[256];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+__global__ void k_sum_parts(const double* part, int n, double* out) {
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) acc += part[q];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) *out = t;
+}
+
 // sum over the full spectrum of |u_hat|^2 (three components), counting the
 // modes the half spectrum stands for twice: N^3 * sum_x |u|^2 (Parseval).
-__global__ void k_spec_energy(Spec s, const hipfftDoubleComplex* u, double* out) {
+// One partial per block into part[blockIdx.x].
+__global__ void k_spec_energy(Spec s, const hipfftDoubleComplex* u, double* part) {
     const long long nc = (long long)s.H1 * s.N2 * s.N3;
     double acc = 0.0;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (long long)gridDim.x * blockDim.x) {
@@ -94,14 +119,8 @@ __global__ void k_spec_energy(Spec s, const hipfftDoubleComplex* u, double* out)
             acc += w * (v.x * v.x + v.y * v.y);
         }
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // dst = src * exp(-i k_d dx_d / 2): the field sampled half a cell back along
@@ -140,42 +159,53 @@ __global__ void k_extract(Geo g, const double* f, double* out) {
 }
 
 // Shell sums of 1/2 |u_hat|^2 / (N^3)^2 over integer-index shells
-// round(|m|): E[b] summed over b equals 1/2 <u.u> (mean over cells).
-__global__ void k_shell_sum(Spec s, const hipfftDoubleComplex* u, int nb, double* E) {
-    extern __shared__ double bins[];
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0.0;
-    __syncthreads();
+// b = round(|m|): E[b] summed over b equals 1/2 <u.u> (mean over cells).
+// One block per shell, so no two blocks add into one bin: the block's threads
+// take the spectrum's rows (j, k) in turn and, on each, only the i where the
+// shell can lie (b - 1/2 <= |m| < b + 1/2, widened by one either side and
+// each i tested with the exact bin formula), then the block's tree.
+__global__ void k_shell_sum(Spec s, const hipfftDoubleComplex* u, double* E) {
+    const int b = blockIdx.x;
     const long long nc = (long long)s.H1 * s.N2 * s.N3;
     const double n3 = (double)s.N1 * s.N2 * s.N3;
     const double norm = 0.5 / (n3 * n3);
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(q % s.H1), j = (int)((q / s.H1) % s.N2), k = (int)(q / ((long long)s.H1 * s.N2));
-        const int m1 = i, m2 = wave_index(j, s.N2), m3 = wave_index(k, s.N3);
-        const int b = (int)(sqrt((double)(m1 * m1 + m2 * m2 + m3 * m3)) + 0.5);
-        if (b >= nb) continue;
-        const double w = (i == 0 || 2 * i == s.N1) ? 1.0 : 2.0;
-        double e = 0.0;
-        for (int d = 0; d < 3; ++d) {
-            const hipfftDoubleComplex v = u[d * nc + q];
-            e += v.x * v.x + v.y * v.y;
+    const double lo2 = (b - 0.5) * (b - 0.5), hi2 = (b + 0.5) * (b + 0.5);
+    double acc = 0.0;
+    for (int r = threadIdx.x; r < s.N2 * s.N3; r += blockDim.x) {
+        const int j = r % s.N2, k = r / s.N2;
+        const int m2 = wave_index(j, s.N2), m3 = wave_index(k, s.N3);
+        const int q2 = m2 * m2 + m3 * m3;
+        if ((double)q2 >= hi2) continue;   // |m| >= b + 1/2 for every i
+        const int ilo = max(0, (int)sqrt(fmax(0.0, lo2 - q2)) - 1);
+        const int ihi = min(s.H1 - 1, (int)sqrt(fmax(0.0, hi2 - q2)) + 1);
+        const long long row = ((long long)k * s.N2 + j) * s.H1;
+        for (int i = ilo; i <= ihi; ++i) {
+            if ((int)(sqrt((double)(i * i + q2)) + 0.5) != b) continue;
+            const double w = (i == 0 || 2 * i == s.N1) ? 1.0 : 2.0;
+            double e = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const hipfftDoubleComplex v = u[d * nc + row + i];
+                e += v.x * v.x + v.y * v.y;
+            }
+            acc += w * norm * e;
         }
-        atomicAdd(&bins[b], w * norm * e);
     }
-    __syncthreads();
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-        if (bins[b] != 0.0) atomicAdd(&E[b], bins[b]);
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) E[b] = t;
 }
 
-// nu * <sum_ij (d_j u_i)^2> with central differences of the level-n cells.
+// nu * <sum_ij (d_j u_i)^2> with central differences of the level-n cells:
+// the sum of the squares, one partial per block into part[blockIdx.x].
 __global__ void k_dissipation(Geo g, const double* u1, const double* u2, const double* u3, double inv2dx0,
-                              double inv2dx1, double inv2dx2, double* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+                              double inv2dx1, double inv2dx2, double* part) {
+    const long long n = (long long)g.N1 * g.N2 * g.N3;
+    const double* u[3] = {u1, u2, u3};
     double acc = 0.0;
-    if (i <= g.N1) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(q % g.N1) + 1, j = (int)((q / g.N1) % g.N2) + 1, k = (int)(q / ((long long)g.N1 * g.N2)) + 1;
         const int ip = i == g.N1 ? 1 : i + 1, im = i == 1 ? g.N1 : i - 1;
         const int jp = j == g.N2 ? 1 : j + 1, jm = j == 1 ? g.N2 : j - 1;
         const int kp = k == g.N3 ? 1 : k + 1, km = k == 1 ? g.N3 : k - 1;
-        const double* u[3] = {u1, u2, u3};
         for (int m = 0; m < 3; ++m) {
             const double dx = (u[m][at(g, ip, j, k)] - u[m][at(g, im, j, k)]) * inv2dx0;
             const double dy = (u[m][at(g, i, jp, k)] - u[m][at(g, i, jm, k)]) * inv2dx1;
@@ -183,14 +213,8 @@ __global__ void k_dissipation(Geo g, const double* u1, const double* u2, const d
             acc += dx * dx + dy * dy + dz * dz;
         }
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -612,7 +636,7 @@ ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, d
     s = c->hip(hipMalloc(&real.p, 3 * n * sizeof(double)), "hipMalloc turbulence");
     if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
     if (s == ARK_OK) s = c->hip(hipMalloc(&work.p, nc * sizeof(hipfftDoubleComplex)), "hipMalloc turbulence");
-    if (s == ARK_OK) s = c->hip(hipMalloc(&acc.p, sizeof(double)), "hipMalloc turbulence");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&acc.p, (kSumBlocks + 1) * sizeof(double)), "hipMalloc turbulence");
     if (s != ARK_OK) return s;
     double* R = (double*)real.p;
     hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
@@ -623,10 +647,13 @@ ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, d
         if (hipfftExecD2Z(f.plan_f, R + d * n, U + d * nc) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "D2Z");
     if (s != ARK_OK) return s;
     hipLaunchKernelGGL(k_shape, dim3(4096), dim3(256), 0, c->stream, f.s, U, k0);
-    s = c->hip(hipMemsetAsync(acc.p, 0, sizeof(double), c->stream), "memset");
-    hipLaunchKernelGGL(k_spec_energy, dim3(1024), dim3(256), 0, c->stream, f.s, U, (double*)acc.p);
+    double* part = (double*)acc.p;
+    hipLaunchKernelGGL(k_spec_energy, dim3(kSumBlocks), dim3(256), 0, c->stream, f.s, U, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, part, kSumBlocks, part + kSumBlocks);
+    s = c->hip(hipGetLastError(), "spectral energy");
     double e = 0.0;
-    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(&e, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK)
+        s = c->hip(hipMemcpyAsync(&e, part + kSumBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
     if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     if (s != ARK_OK) return s;
     // The unnormalised inverse R(x) = sum_k U_k e^{ikx} has (Parseval)
@@ -649,7 +676,6 @@ ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
     s = c->hip(hipMalloc(&real.p, n * sizeof(double)), "hipMalloc spectrum");
     if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc spectrum");
     if (s == ARK_OK) s = c->hip(hipMalloc(&bins.p, nbins * sizeof(double)), "hipMalloc spectrum");
-    if (s == ARK_OK) s = c->hip(hipMemsetAsync(bins.p, 0, nbins * sizeof(double), c->stream), "memset");
     const int cellu[3] = {F_u1nCon, F_u2nCon, F_u3nCon};
     hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
     for (int m = 0; m < 3 && s == ARK_OK; ++m) {
@@ -659,8 +685,7 @@ ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
             s = c->fail(ARK_ERR_HIP, "D2Z");
     }
     if (s == ARK_OK) {
-        hipLaunchKernelGGL(k_shell_sum, dim3(1024), dim3(256), nbins * sizeof(double), c->stream, f.s, U, nbins,
-                           (double*)bins.p);
+        hipLaunchKernelGGL(k_shell_sum, dim3(nbins), dim3(256), 0, c->stream, f.s, U, (double*)bins.p);
         s = c->hip(hipGetLastError(), "shell sum");
     }
     if (s == ARK_OK)
@@ -672,16 +697,18 @@ ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
 ark_status ark_dissipation(ark_ctx* c, double* eps) {
     if (!c || !eps) return ARK_ERR_INVALID;
     DevBuf acc;
-    ark_status s = c->hip(hipMalloc(&acc.p, sizeof(double)), "hipMalloc");
-    if (s == ARK_OK) s = c->hip(hipMemsetAsync(acc.p, 0, sizeof(double), c->stream), "memset");
+    ark_status s = c->hip(hipMalloc(&acc.p, (kSumBlocks + 1) * sizeof(double)), "hipMalloc");
+    double* part = (double*)acc.p;
     if (s == ARK_OK) {
-        hipLaunchKernelGGL(k_dissipation, interior_grid(c->geo, 0), dim3(256), 0, c->stream, c->geo,
-                           c->cur[F_u1nCon], c->cur[F_u2nCon], c->cur[F_u3nCon], 0.5 / c->prm.dx1, 0.5 / c->prm.dx2,
-                           0.5 / c->prm.dx3, (double*)acc.p);
+        hipLaunchKernelGGL(k_dissipation, dim3(kSumBlocks), dim3(256), 0, c->stream, c->geo, c->cur[F_u1nCon],
+                           c->cur[F_u2nCon], c->cur[F_u3nCon], 0.5 / c->prm.dx1, 0.5 / c->prm.dx2, 0.5 / c->prm.dx3,
+                           part);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, part, kSumBlocks, part + kSumBlocks);
         s = c->hip(hipGetLastError(), "dissipation");
     }
     double v = 0.0;
-    if (s == ARK_OK) s = c->hip(hipMemcpyAsync(&v, acc.p, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    if (s == ARK_OK)
+        s = c->hip(hipMemcpyAsync(&v, part + kSumBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
     if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     *eps = c->prm.VIS * v / ((double)c->geo.N1 * c->geo.N2 * c->geo.N3);
     return s;

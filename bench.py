@@ -16,12 +16,14 @@ Printed: ONE JSON line.  Beyond the common fields it carries
   accuracy      mixed vs FP64 after the same steps, per field (corrected
                 scheme: the reference scheme goes unstable on this vortex)
   batched       64 independent boxes of uneven size and Reynolds number in one
-                launch per step (graph-replayed, tile shape autotuned)
+                launch per step (graph-replayed, tile shape autotuned), half
+                Taylor-Green, half forced turbulence, filtered into training pairs
   forced_turbulence_closure
                 512^3 forced turbulence (2000 steps, kinetic energy and
                 dissipation series), the closure network trained on its
                 filtered snapshots, and 128^3 runs with and without it compared
                 with the filtered 512^3 field
+  hbm_ceiling   what plain copy / read / write kernels reach on this GPU
   cpu_baseline  the unmodified reference (oracle/_ref, built from its own
                 sources) under mpirun on this host's cores, bounded sample
   --calibrate-copy prints the HBM bytes a plain device copy of known size
@@ -143,20 +145,30 @@ def phase_kernels(N, steps):
     return out
 
 
-def batched(nboxes, steps, seed=4):
+def batched(nboxes, steps, seed=4, width=4):
     """Many independent boxes in one launch per step: cubes with edges drawn
     from {32, 48, 64, 96, 128}, Re uniform in [400, 3200] (nu = U0 L / Re with
-    the vortex scale L = 1), Taylor-Green initial state, mixed precision,
-    tile shape autotuned.  Aggregate Mcell-steps/s over all boxes."""
+    the vortex scale L = 1), alternately a Taylor-Green vortex and a seeded
+    solenoidal field (k0 = 4, u_rms 1) under random-phase forcing at eps = 0.1,
+    mixed precision, corrected scheme, tile shape autotuned.  Aggregate
+    Mcell-steps/s over all boxes; afterwards every box is box-filtered (width
+    4 cells) into closure training pairs (filtered gradients and exact
+    subgrid stresses)."""
     import numpy as np
     import arkamd
     rng = np.random.default_rng(seed)
     edges = rng.choice([32, 48, 64, 96, 128], size=nboxes)
     res = rng.uniform(400.0, 3200.0, size=nboxes)
-    boxes = [arkamd.Solver(int(e), int(e), int(e), VIS=1.0 / float(r), precision=arkamd.PREC_MIXED)
-             for e, r in zip(edges, res)]
-    for s in boxes:
-        s.init_tgv(1.0)
+    boxes = []
+    for q, (e, r) in enumerate(zip(edges, res)):
+        s = arkamd.Solver(int(e), int(e), int(e), VIS=1.0 / float(r), precision=arkamd.PREC_MIXED,
+                          scheme=arkamd.SCHEME_CORRECTED)
+        if q % 2:
+            s.init_turbulence(1000 + q, 4.0, 1.0)
+            s.set_forcing(2000 + q, 0.1, 0.01)
+        else:
+            s.init_tgv(1.0)
+        boxes.append(s)
     cells = int(sum(int(e) ** 3 for e in edges))
     with arkamd.Batch(boxes) as b:
         tune = b.autotune(reps=3)
@@ -168,14 +180,23 @@ def batched(nboxes, steps, seed=4):
         b.synchronize()
         el = time.perf_counter() - t0
     nonfinite = sum(s.diagnostics(allow_nonfinite=True)["nonfinite"] for s in boxes)
+    t0 = time.perf_counter()
+    pairs = 0
+    for s in boxes:
+        ub, tau, grad = s.filter_box(width)
+        pairs += ub[0].size
+    filt = time.perf_counter() - t0
     for s in boxes:
         s.close()
     return {"boxes": nboxes, "cells": cells, "steps": steps, "edges_hist": {str(e): int((edges == e).sum())
                                                                              for e in (32, 48, 64, 96, 128)},
             "re_range": [round(float(res.min()), 1), round(float(res.max()), 1)],
+            "kinds": {"taylor_green": (nboxes + 1) // 2, "forced_turbulence": nboxes // 2},
+            "scheme": "corrected",
             "value": round(cells * steps / el / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el * 1e3 / steps, 4),
             "config": cfg, "autotune_ms": {f"{k[0]}/lz{k[1]}": round(v, 4) for k, v in tune.items()},
-            "nonfinite": nonfinite}
+            "nonfinite": nonfinite,
+            "training_pairs": {"filter_width_cells": width, "pairs": pairs, "wall_s_incl_download": round(filt, 3)}}
 
 
 def closure_experiment(N, steps, width, train_steps, batch):
@@ -254,6 +275,21 @@ def cpu_baseline(N, steps, ranks):
         "sample": f"TGV {N}^3 x {steps} steps, FP64, mpirun -n {ranks} (dims {rec['dims']}), "
                   f"step loop only, max over ranks ({rec['loop_seconds']:.2f} s)",
     }
+
+
+def hbm_ceiling():
+    """Best bandwidths of plain streaming kernels on this GPU (copy, read-only,
+    write-only over 4 GiB; ark-cpp-mpi_amd/bin/ark_hbm_probe), the practical
+    ceiling beside the published 8 TB/s; None if the probe is not built."""
+    exe = os.path.join(REPO, "ark-cpp-mpi_amd", "bin", "ark_hbm_probe")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if out.returncode != 0:
+        return None
+    rec = json.loads(out.stdout.strip().splitlines()[-1])
+    rec["copy_frac_of_peak"] = round(max(rec["copy_GBs"], rec["copy_nt_GBs"]) / HBM_PEAK_GBS, 4)
+    return rec
 
 
 def calibrate_copy(nbytes):
@@ -339,6 +375,8 @@ def main():
     line["final"] = {p: {"dt": r["diag"]["dt"], "nstep": r["diag"]["nstep"], "nonfinite": r["diag"]["nonfinite"]}
                      for p, r in runs.items()}
     line["device_bytes"] = {p: r["device_bytes"] for p, r in runs.items()}
+    if not args.only:
+        line["hbm_ceiling"] = hbm_ceiling()
     if args.phase_steps > 0 and not args.only:
         line["phase_kernels_fp64"] = phase_kernels(N, args.phase_steps)
     if args.batch_boxes > 0 and not args.only:

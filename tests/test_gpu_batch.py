@@ -87,6 +87,35 @@ def test_batch_autotune_leaves_state_unchanged():
     assert_same(ref, bat, "after autotune")
 
 
+def make_mixed_kinds(boxes, tau):
+    """Corrected scheme; even boxes Taylor-Green, odd boxes seeded turbulence
+    with random-phase forcing (per step for tau = 0, per interval otherwise)."""
+    out = []
+    for q, (N, vis) in enumerate(boxes):
+        s = arkamd.Solver(*N, VIS=vis, precision=arkamd.PREC_MIXED, scheme=arkamd.SCHEME_CORRECTED)
+        if q % 2:
+            s.init_turbulence(100 + q, 2.0, 1.0)
+            s.set_forcing(200 + q, 0.1, tau)
+        else:
+            s.init_tgv(1.0)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("tau", [0.0, 0.003])
+def test_batch_with_forced_boxes_equals_standalone(tau):
+    boxes = [((32, 32, 32), 1 / 500), ((32, 32, 32), 1 / 2000), ((48, 48, 48), 1 / 800), ((16, 32, 24), 1 / 3000)]
+    ref, bat = make_mixed_kinds(boxes, tau), make_mixed_kinds(boxes, tau)
+    for s in ref:
+        s.step(9)
+    with arkamd.Batch(bat) as b:
+        b.step(4)
+        b.step(5)
+    assert_same(ref, bat, f"forced boxes, tau {tau}")
+    for s in bat:
+        assert s.diagnostics()["nonfinite"] == 0
+
+
 def test_batch_rejects_bad_boxes():
     f64 = arkamd.Solver(16, 16, 16)
     m = arkamd.Solver(16, 16, 16, precision=arkamd.PREC_MIXED)

@@ -53,6 +53,19 @@ def test_gradient_vs_oracle(n):
         assert np.linalg.norm(g - g64) / np.linalg.norm(g64) < 5e-2
 
 
+def test_fresh_networks_match_the_oracle():
+    """Networks created back to back each hold exactly their initial weights
+    (the BF16 copies' zero padding is written on the network's own stream,
+    before the copies themselves)."""
+    X, T = data(33, seed=5)
+    for seed in range(8):
+        with make(seed=seed) as m:
+            p = m.params().astype(np.float64)
+            _, loss = m.gradient(X, T)
+            l0 = M.forward_backward(p, X.astype(np.float64), T.astype(np.float64), NORM, bf=True)[0]
+            assert abs(loss - l0) < 1e-5 * l0, (seed, loss, l0)
+
+
 def test_training_converges_and_is_deterministic():
     X, T = data(20000, seed=9)
     runs = []

@@ -10,7 +10,6 @@ import turb_oracle as T
 pytestmark = pytest.mark.gpu
 
 
-
 def cells_of(s):
     return [s.download(n)[1:-1, 1:-1, 1:-1] for n in ("u1nCon", "u2nCon", "u3nCon")]
 
@@ -51,6 +50,23 @@ def test_spectrum_dissipation_filter_vs_oracle():
         gr0 = T.coarse_gradient(ub0, tuple(4 * d for d in s.dx))
         assert np.abs(ub - ub0).max() < 1e-14 and np.abs(tau - tau0).max() < 1e-14
         assert np.abs(gr - gr0).max() < 1e-12
+
+
+@pytest.mark.parametrize("N", [(16, 32, 32), (32, 16, 32), (48, 40, 24)])
+def test_turbulence_functions_are_reproducible(N):
+    """Two contexts given the same seed hold the same field bit for bit, and
+    the spectrum and dissipation repeat bit for bit (every sum over the grid
+    is taken in a fixed order)."""
+    out = []
+    for _ in range(2):
+        with arkamd.Solver(*N, VIS=0.01, precision=arkamd.PREC_MIXED) as s:
+            s.init_turbulence(seed=21, k0=4.0, urms=1.0)
+            out.append(([s.download(n) for n in ("u1nCon", "u2nCon", "u3nCon", "u11", "u22", "u33")],
+                        s.spectrum(), s.dissipation()))
+    (fa, ea, da), (fb, eb, db) = out
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    assert np.array_equal(ea.view(np.uint64), eb.view(np.uint64)) and da == db
 
 
 def test_single_mode_spectrum_on_device():
