@@ -84,6 +84,12 @@ ark_status ark_refresh_rho(ark_ctx* c);                 // store face densities 
 ark_status ark_phase_body(ark_ctx* c);                  // Phase1 .. UseForces of one step, dt already set
 }
 
+// Forcing of a batch of contexts in two launches per step (ark_turb.hip).
+struct ark_force_batch;
+ark_status ark_force_batch_build(ark_ctx* const* boxes, int nbox, ark_force_batch** f);   // replaces *f
+ark_status ark_force_batch_enqueue(const ark_force_batch* f, hipStream_t st);
+void ark_force_batch_free(ark_force_batch* f);
+
 // Field slots of the four groups, in group order.
 extern const int kGroupFields[4][5];
 // Exchange the current and the other buffer of group g.

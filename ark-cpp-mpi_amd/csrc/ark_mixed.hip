@@ -885,6 +885,7 @@ struct ark_batch {
     unsigned long long* scratch_dt = nullptr;     // device: dt MIN slots of the tuning launches
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    ark_force_batch* forcing = nullptr;           // the forced boxes' force kernels, two launches per step
     bool built = false;
     std::vector<int> gpar_at_build;               // box buffer parities the tables were built for
     int parity = 0;                               // which table the next step uses
@@ -947,8 +948,8 @@ ark_status enqueue_step(ark_batch* b, int parity, bool begin) {
     if (begin) {
         hipLaunchKernelGGL(mixed::k_batch_begin_step, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->sptr, n);
         ark_status s = b->hip(hipGetLastError(), "batch begin step");
-        for (ark_ctx* c : b->box)
-            if (s == ARK_OK) s = ark_turb_force_step(c, b->stream);
+        if (s == ARK_OK && ark_force_batch_enqueue(b->forcing, b->stream) != ARK_OK)
+            s = b->fail(ARK_ERR_HIP, "batch forcing");
         if (s != ARK_OK) return s;
     }
     return b->hip(launch_shape_batch(b->shape, b->quirk, dim3(b->tiles), b->stream, b->args[parity], b->start, n),
@@ -958,6 +959,8 @@ ark_status enqueue_step(ark_batch* b, int parity, bool begin) {
 ark_status build_graphs(ark_batch* b) {
     drop_graphs(b);
     ark_status s = build_tables(b, false);
+    if (s == ARK_OK && ark_force_batch_build(b->box.data(), (int)b->box.size(), &b->forcing) != ARK_OK)
+        s = b->fail(ARK_ERR_HIP, "batch forcing tables");
     for (int p = 0; p < 2 && s == ARK_OK; ++p) {
         s = b->hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
         if (s != ARK_OK) break;
@@ -1054,6 +1057,7 @@ void ark_batch_destroy(ark_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
+    ark_force_batch_free(b->forcing);
     for (int p = 0; p < 2; ++p)
         if (b->args[p]) (void)hipFree(b->args[p]);
     if (b->start) (void)hipFree(b->start);

@@ -236,10 +236,8 @@ __constant__ ForceMode c_modes[16];
 // with norm sqrt(eps/(16 tau)): a force constant over tau injects
 // <|F|^2> tau / 2 = eps on average, and runs of any resolution see the same
 // force history.
-__global__ void k_force_modes(const Scalars* S, unsigned long long seed, double eps, double tau, double L0, double L1,
-                              double L2, float* table) {
-    const int n = threadIdx.x;
-    if (n >= 16) return;
+__device__ void force_mode(const Scalars* S, unsigned long long seed, double eps, double tau, double L0, double L1,
+                           double L2, float* table, int n) {
     const double dt = tau > 0.0 ? tau : S->dt;
     const uint32_t step = tau > 0.0 ? (uint32_t)floor((S->time - 0.5 * S->dt) / tau) : (uint32_t)S->nstep;
     double g[8];
@@ -263,16 +261,35 @@ __global__ void k_force_modes(const Scalars* S, unsigned long long seed, double 
     }
 }
 
+__global__ void k_force_modes(const Scalars* S, unsigned long long seed, double eps, double tau, double L0, double L1,
+                              double L2, float* table) {
+    if (threadIdx.x < 16) force_mode(S, seed, eps, tau, L0, L1, L2, table, threadIdx.x);
+}
+
+// Forcing of many boxes at once (batched steps): one record per forced box.
+struct ForceBox {
+    Geo g;
+    const Scalars* S;
+    unsigned long long seed;
+    double eps, tau, L0, L1, L2;
+    float* table;
+    const float4* trig;
+    float* force;
+    long long stride;
+};
+
+// Block b: the 16 modes of box b.
+__global__ void k_force_modes_batch(const ForceBox* __restrict__ fb) {
+    const ForceBox& f = fb[blockIdx.x];
+    if (threadIdx.x < 16) force_mode(f.S, f.seed, f.eps, f.tau, f.L0, f.L1, f.L2, f.table, threadIdx.x);
+}
+
 // F at the cell centres x = (i - 1/2) dx, three FP32 components at force +
 // m*stride in the field index map.  trig holds, per axis d and cell index
 // 1..N_d, {cos, sin} of (2 pi / L_d) m x for m = 1, 2 (axes one after the
 // other); each mode's phase is the product of three of them.
-__global__ void k_force_field(Geo g, const float* table, const float4* trig, float* force, long long stride) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    __shared__ float tab[96];
-    for (int q = threadIdx.x; q < 96; q += blockDim.x) tab[q] = table[q];
-    __syncthreads();
-    if (i > g.N1) return;
+__device__ __forceinline__ void force_cell(const Geo& g, const float* tab, const float4* trig, float* force,
+                                           long long stride, int i, int j, int k) {
     const float4 t[3] = {trig[i - 1], trig[g.N1 + j - 1], trig[g.N1 + g.N2 + k - 1]};
     float cr[3][3], ci[3][3];
     for (int d = 0; d < 3; ++d) {
@@ -306,6 +323,35 @@ __global__ void k_force_field(Geo g, const float* table, const float4* trig, flo
     force[c] = F[0];
     force[stride + c] = F[1];
     force[2 * stride + c] = F[2];
+}
+
+__global__ void k_force_field(Geo g, const float* table, const float4* trig, float* force, long long stride) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    __shared__ float tab[96];
+    for (int q = threadIdx.x; q < 96; q += blockDim.x) tab[q] = table[q];
+    __syncthreads();
+    if (i > g.N1) return;
+    force_cell(g, tab, trig, force, stride, i, j, k);
+}
+
+// All forced boxes' fields in one launch: box b owns the workgroups
+// start[b] .. start[b+1]-1, 256 interior cells each (x fastest).
+__global__ void k_force_field_batch(const ForceBox* __restrict__ fb, const int* __restrict__ start, int nbox) {
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= (int)blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const ForceBox& f = fb[lo];
+    __shared__ float tab[96];
+    for (int q = threadIdx.x; q < 96; q += blockDim.x) tab[q] = f.table[q];
+    __syncthreads();
+    const long long cell = (long long)(blockIdx.x - start[lo]) * blockDim.x + threadIdx.x;
+    const long long n1 = f.g.N1, n12 = n1 * f.g.N2;
+    if (cell >= n12 * f.g.N3) return;
+    force_cell(f.g, tab, f.trig, f.force, f.stride, (int)(cell % n1) + 1, (int)((cell / n1) % f.g.N2) + 1,
+               (int)(cell / n12) + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,6 +569,70 @@ ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st) {
     hipLaunchKernelGGL(k_force_field, interior_grid(c->geo, 0), dim3(256), 0, st, c->geo, c->force_table,
                        (const float4*)c->force_trig, c->force, (long long)c->force_stride);
     return c->hip(hipGetLastError(), "force field");
+}
+
+// Forcing of a batch: one record per forced box and the first workgroup of
+// each box's field, in device memory, built once per set of boxes.
+struct ark_force_batch {
+    ForceBox* fb = nullptr;
+    int* start = nullptr;
+    int n = 0, blocks = 0;
+};
+
+void ark_force_batch_free(ark_force_batch* f) {
+    if (!f) return;
+    if (f->fb) (void)hipFree(f->fb);
+    if (f->start) (void)hipFree(f->start);
+    delete f;
+}
+
+ark_status ark_force_batch_build(ark_ctx* const* boxes, int nbox, ark_force_batch** out) {
+    ark_force_batch_free(*out);
+    *out = nullptr;
+    std::vector<ForceBox> h;
+    std::vector<int> start(1, 0);
+    for (int b = 0; b < nbox; ++b) {
+        const ark_ctx* c = boxes[b];
+        if (!c->force || c->force_eps <= 0.0) continue;
+        ForceBox f;
+        f.g = c->geo;
+        f.S = c->S;
+        f.seed = c->force_seed;
+        f.eps = c->force_eps;
+        f.tau = c->force_tau;
+        f.L0 = c->prm.dx1 * c->geo.N1;
+        f.L1 = c->prm.dx2 * c->geo.N2;
+        f.L2 = c->prm.dx3 * c->geo.N3;
+        f.table = c->force_table;
+        f.trig = (const float4*)c->force_trig;
+        f.force = c->force;
+        f.stride = c->force_stride;
+        h.push_back(f);
+        const long long cells = (long long)c->geo.N1 * c->geo.N2 * c->geo.N3;
+        start.push_back(start.back() + (int)((cells + 255) / 256));
+    }
+    ark_force_batch* f = new ark_force_batch();
+    f->n = (int)h.size();
+    f->blocks = start.back();
+    if (f->n > 0) {
+        if (hipMalloc(&f->fb, h.size() * sizeof(ForceBox)) != hipSuccess ||
+            hipMalloc(&f->start, start.size() * sizeof(int)) != hipSuccess ||
+            hipMemcpy(f->fb, h.data(), h.size() * sizeof(ForceBox), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(f->start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            ark_force_batch_free(f);
+            return ARK_ERR_HIP;
+        }
+    }
+    *out = f;
+    return ARK_OK;
+}
+
+// This step's forcing of every forced box: two launches, whatever the count.
+ark_status ark_force_batch_enqueue(const ark_force_batch* f, hipStream_t st) {
+    if (!f || f->n == 0) return ARK_OK;
+    hipLaunchKernelGGL(k_force_modes_batch, dim3(f->n), dim3(64), 0, st, f->fb);
+    hipLaunchKernelGGL(k_force_field_batch, dim3(f->blocks), dim3(256), 0, st, f->fb, f->start, f->n);
+    return hipGetLastError() == hipSuccess ? ARK_OK : ARK_ERR_HIP;
 }
 
 // The force buffer (acceleration, 3 x FP32 fields, and the 16-mode table).
