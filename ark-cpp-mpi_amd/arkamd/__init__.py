@@ -182,7 +182,12 @@ class Solver:
 
     def __init__(self, N1: int, N2: int, N3: int, dx=None, CFL: float = 0.3,
                  VIS: float = 1.0 / 1600.0, sound: float = 10.0, ro0: float = 1.0, t0: float = 1.0,
-                 precision: int = PREC_FP64, device: int = 0, scheme: int = SCHEME_REFERENCE):
+                 precision: int = PREC_FP64, device: int = 0, scheme: int = SCHEME_REFERENCE,
+                 periodic=(1, 1, 1)):
+        """periodic[d] = 0 makes axis d non-periodic as one rank of the
+        reference with periods={0,0,0} sees it: its exchange delivers zeros
+        and the reference's physical-boundary branches run (zero-gradient ends
+        on X1 and X3, inflow and outflow on X1; walls come from `condition`)."""
         self.N = (int(N1), int(N2), int(N3))
         if dx is None:
             two_pi = 2.0 * np.pi
@@ -192,7 +197,8 @@ class Solver:
         p.N1, p.N2, p.N3 = self.N
         p.dx1, p.dx2, p.dx3 = self.dx
         p.CFL, p.VIS, p.sound, p.ro0, p.t0 = CFL, VIS, sound, ro0, t0
-        p.periodic[0] = p.periodic[1] = p.periodic[2] = 1
+        for d in range(3):
+            p.periodic[d] = 1 if periodic[d] else 0
         p.precision, p.device, p.scheme = precision, device, scheme
         self.params = p
         self._L = lib()

@@ -101,8 +101,9 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
         *out = c;
         return ARK_ERR_INVALID;
     }
-    if (!(p->periodic[0] && p->periodic[1] && p->periodic[2])) {
-        c->err = "only the triply periodic box is implemented (physical boundaries are not)";
+    const bool all_periodic = p->periodic[0] && p->periodic[1] && p->periodic[2];
+    if (!all_periodic && p->precision == ARK_PREC_MIXED) {
+        c->err = "mixed precision needs a triply periodic box (non-periodic axes run the FP64 phase path)";
         *out = c;
         return ARK_ERR_UNSUPPORTED;
     }
@@ -362,17 +363,52 @@ hipError_t run_phase1(void* p) {
     ark_ctx* c = (ark_ctx*)p;
     return launch_phase1(c->stream, c->geo, c->fields(), c->K, c->S);
 }
+// Axes of a context: periodic ones exchange with themselves; a non-periodic
+// axis of one rank has MPI_PROC_NULL neighbours (its planes get zeros) and
+// runs the reference's physical-boundary blocks: zero-gradient ends on X1 and
+// X3 (source/main.cpp:803-845, 1633-1698), inflow and outflow in the X1 sweep.
+int periodic_mask(const ark_ctx* c) {
+    return (c->prm.periodic[0] ? 1 : 0) | (c->prm.periodic[1] ? 2 : 0) | (c->prm.periodic[2] ? 4 : 0);
+}
+
+void add_plan(GhostPlan& plan, double* p, int is_face, int mask) {
+    if (mask == 0) return;
+    plan.p[plan.n] = p;
+    plan.is_face[plan.n] = is_face;
+    plan.axis_mask[plan.n] = mask;
+    plan.n++;
+}
+
+// The exchange of the given cell arrays (every axis) and face arrays (pairs
+// of field and axis: along their own axis), then, on non-periodic X1/X3, the
+// zero-gradient ends of the arrays the reference sets there (x_ends on X1;
+// the cell arrays and z_faces on X3).
+hipError_t exchange(ark_ctx* c, const int* cells, int ncells, const int* faces, int nfaces, const int* x_ends,
+                    int nx_ends, const int* z_faces, int nz_faces) {
+    const int per = periodic_mask(c), non = 7 & ~per;
+    GhostPlan copy{}, zero{}, grad{};
+    for (int q = 0; q < ncells; ++q) {
+        add_plan(copy, c->cur[cells[q]], 0, per);
+        add_plan(zero, c->cur[cells[q]], 0, non);
+    }
+    for (int q = 0; q < nfaces; ++q) {
+        const int d = faces[2 * q + 1];
+        add_plan(copy, c->cur[faces[2 * q]], 1, per & (1 << d));
+        add_plan(zero, c->cur[faces[2 * q]], 1, non & (1 << d));
+    }
+    for (int q = 0; q < nx_ends; ++q) add_plan(grad, c->cur[x_ends[q]], 0, non & 1);
+    for (int q = 0; q < ncells; ++q) add_plan(grad, c->cur[cells[q]], 0, non & 4);
+    for (int q = 0; q < nz_faces; ++q) add_plan(grad, c->cur[z_faces[q]], 1, non & 4);
+    hipError_t e = launch_ghost(c->stream, c->geo, copy, 0);
+    if (e == hipSuccess) e = launch_ghost(c->stream, c->geo, zero, 1);
+    if (e == hipSuccess) e = launch_zero_gradient(c->stream, c->geo, grad);
+    return e;
+}
+
 hipError_t run_stress_ghost(void* p) {
     ark_ctx* c = (ark_ctx*)p;
-    GhostPlan plan{};
     const int cells[3] = {F_u1Con, F_u2Con, F_u3Con};
-    for (int q = 0; q < 3; ++q) {
-        plan.p[plan.n] = c->cur[cells[q]];
-        plan.is_face[plan.n] = 0;
-        plan.axis_mask[plan.n] = 7;
-        plan.n++;
-    }
-    return launch_ghost(c->stream, c->geo, plan);
+    return exchange(c, cells, 3, nullptr, 0, cells, 3, nullptr, 0);
 }
 hipError_t run_sigma_zero(void* p) {
     ark_ctx* c = (ark_ctx*)p;
@@ -392,24 +428,20 @@ hipError_t run_use_forces(void* p) {
 }
 hipError_t run_phase2_ghost(void* p) {
     ark_ctx* c = (ark_ctx*)p;
-    GhostPlan plan{};
     const int cells[7] = {F_u1nCon, F_u2nCon, F_u3nCon, F_roCon, F_ronCon, F_tCon, F_tnCon};
-    for (int q = 0; q < 7; ++q) {
-        plan.p[plan.n] = c->cur[cells[q]];
-        plan.is_face[plan.n] = 0;
-        plan.axis_mask[plan.n] = 7;
-        plan.n++;
-    }
+    int faces[30];
     for (int d = 0; d < 3; ++d) {
-        const int faces[5] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_ro(d), face_p(d)};
+        const int f[5] = {face_u(d, 0), face_u(d, 1), face_u(d, 2), face_ro(d), face_p(d)};
         for (int q = 0; q < 5; ++q) {
-            plan.p[plan.n] = c->cur[faces[q]];
-            plan.is_face[plan.n] = 1;
-            plan.axis_mask[plan.n] = 1 << d;
-            plan.n++;
+            faces[2 * (5 * d + q)] = f[q];
+            faces[2 * (5 * d + q) + 1] = d;
         }
     }
-    return launch_ghost(c->stream, c->geo, plan);
+    // X1 ends: the level-n velocities, density and t (source/main.cpp:1633-1658);
+    // X3 ends: all seven cell arrays and the z-faces (1660-1698)
+    const int x_ends[5] = {F_u1nCon, F_u2nCon, F_u3nCon, F_ronCon, F_tnCon};
+    const int z_faces[5] = {face_u(2, 0), face_u(2, 1), face_u(2, 2), face_ro(2), face_p(2)};
+    return exchange(c, cells, 7, faces, 15, x_ends, 5, z_faces, 5);
 }
 struct P2Args { ark_ctx* c; int axis; };
 hipError_t run_phase2_axis(void* p) {
@@ -424,20 +456,20 @@ hipError_t run_phase2_axis(void* p) {
     nx.p[3] = c->alt[face_p(d)];
     nx.p[4] = c->alt[face_ro(d)];
     nx.p[5] = c->alt[face_t(d)];
-    return launch_phase2(c->stream, d, c->prm.scheme, c->geo, c->fields(), nx, c->K, c->S);
+    return launch_phase2(c->stream, d, c->prm.scheme, c->geo, c->fields(), nx, c->K, c->S,
+                         d == 0 && !c->prm.periodic[0]);
 }
 
 }  // namespace
 
+// The phase-by-phase entry points are FP64 and take solid cells and
+// non-periodic axes; the fused and mixed-precision steps need a fluid,
+// triply periodic box.
 static ark_status require_fluid(ark_ctx* c) {
     if (c->prm.precision == ARK_PREC_MIXED)
         return c->fail(ARK_ERR_UNSUPPORTED,
                        "the phase-by-phase entry points are FP64; a mixed-precision context advances with ark_step "
                        "or ark_advance");
-    if (c->has_solid)
-        return c->fail(ARK_ERR_UNSUPPORTED,
-                       "solid cells (condition != 0) are not implemented on the GPU path; the periodic box must be "
-                       "all fluid");
     return ARK_OK;
 }
 
@@ -573,7 +605,8 @@ ark_status ark_step_phases(ark_ctx* c, int nsteps) {
 ark_status ark_step(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
     if (c->prm.precision == ARK_PREC_MIXED) {
-        if (c->has_solid) return c->fail(ARK_ERR_UNSUPPORTED, "solid cells are not implemented on the GPU path");
+        if (c->has_solid)
+            return c->fail(ARK_ERR_UNSUPPORTED, "solid cells run on the FP64 phase path, not in mixed precision");
         return ark_mixed_steps(c, nsteps);
     }
     ark_status s = require_fluid(c);

@@ -449,7 +449,7 @@ int choose_lz(const Geo& g) {
 
 bool ark_fused_eligible(const ark_ctx* c) {
     return c->geo.N1 >= 2 && c->geo.N2 >= 2 && c->geo.N3 >= 2 && c->rho_consistent && !c->has_solid &&
-           c->prm.precision == ARK_PREC_FP64;
+           c->prm.precision == ARK_PREC_FP64 && c->prm.periodic[0] && c->prm.periodic[1] && c->prm.periodic[2];
 }
 
 // One fused step per iteration: begin the step (dt), run the kernel, swap
@@ -499,7 +499,10 @@ ark_status ark_step_fused(ark_ctx* c, int nsteps) {
 // ark_advance: fused steps only.
 extern "C" ark_status ark_advance(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
-    if (c->has_solid) return c->fail(ARK_ERR_UNSUPPORTED, "solid cells are not implemented on the GPU path");
+    if (c->has_solid || !(c->prm.periodic[0] && c->prm.periodic[1] && c->prm.periodic[2]))
+        return c->fail(ARK_ERR_UNSUPPORTED,
+                       "the fused steps need a fluid, triply periodic box; solid cells and non-periodic axes run on "
+                       "the phase path (ark_step)");
     if (c->prm.precision == ARK_PREC_MIXED) return ark_mixed_steps(c, nsteps);
     if (!ark_fused_eligible(c))
         return c->fail(ARK_ERR_STATE,

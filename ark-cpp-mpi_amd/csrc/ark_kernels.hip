@@ -130,7 +130,12 @@ __global__ void __launch_bounds__(256) k_phase1(Geo g, Fields F, Consts K, const
 // axis: face 0 <- face N_d, face N_d+2 <- face 2.  Only the interior of the
 // two other axes is written (no edges or corners), and no written element is
 // a source of another: the whole plan runs as one launch.
-__global__ void __launch_bounds__(256) k_ghost(Geo g, GhostPlan plan) {
+//
+// With zero set, the same planes are set to 0 instead: the exchange along a
+// non-periodic axis of one rank, whose neighbour is MPI_PROC_NULL, copies a
+// receive buffer that nothing has written (the reference's buffers are
+// uninitialised, Arr2d.cpp:7-12; zero-filled buffers define the result).
+__global__ void __launch_bounds__(256) k_ghost(Geo g, GhostPlan plan, int zero) {
     int a = blockIdx.y, d = blockIdx.z;
     if (a >= plan.n || !((plan.axis_mask[a] >> d) & 1)) return;
     const int N[3] = {g.N1, g.N2, g.N3};
@@ -145,13 +150,38 @@ __global__ void __launch_bounds__(256) k_ghost(Geo g, GhostPlan plan) {
     int q = r % N[o1] + 1, p = r / N[o1] + 1;
     double* A = plan.p[a];
     int base = q * st[o1] + p * st[o2];
-    if (!plan.is_face[a]) {
+    if (zero) {
+        A[base + (side == 0 ? 0 : (N[d] + 1 + plan.is_face[a]) * st[d])] = 0.0;
+    } else if (!plan.is_face[a]) {
         if (side == 0) A[base] = A[base + N[d] * st[d]];
         else A[base + (N[d] + 1) * st[d]] = A[base + st[d]];
     } else {
         if (side == 0) A[base] = A[base + N[d] * st[d]];
         else A[base + (N[d] + 2) * st[d]] = A[base + 2 * st[d]];
     }
+}
+
+// Zero-gradient ends of a non-periodic axis (the physical-boundary blocks of
+// source/main.cpp:803-845 and 1633-1698): cell planes 0 <- 1 and N+1 <- N, face
+// planes 0 <- 1 and N+2 <- N+1, over the interior of the two other axes.
+__global__ void __launch_bounds__(256) k_zero_gradient(Geo g, GhostPlan plan) {
+    int a = blockIdx.y, d = blockIdx.z;
+    if (a >= plan.n || !((plan.axis_mask[a] >> d) & 1)) return;
+    const int N[3] = {g.N1, g.N2, g.N3};
+    const int st[3] = {1, g.px, g.sxy};
+    int o1 = d == 0 ? 1 : 0;
+    int o2 = d == 2 ? 1 : 2;
+    int plane = N[o1] * N[o2];
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * plane) return;
+    int side = t >= plane;
+    int r = t - side * plane;
+    int q = r % N[o1] + 1, p = r / N[o1] + 1;
+    double* A = plan.p[a];
+    int base = q * st[o1] + p * st[o2];
+    const int last = N[d] + plan.is_face[a];     // last interior plane: cell N, face N+1
+    if (side == 0) A[base] = A[base + st[d]];
+    else A[base + (last + 1) * st[d]] = A[base + last * st[d]];
 }
 
 // ---------------------------------------------------------------------------
@@ -267,8 +297,34 @@ __global__ void __launch_bounds__(256) k_use_forces(Geo g, Fields F, Consts K, c
 // read from the current buffer and written to the next one (the reference's
 // in-place update reads only old values).  Threads of the first and last face
 // also carry the ghost faces 0 and N_D+2, set by the ghost fill, across.
+// Backward extrapolation of t of one cell (the t part of cell_backward).
+__device__ __forceinline__ double backward_t(double t_f, double t_b, double t_cn, double t_c, double un_cn, double dt,
+                                            double dx) {
+    double t = 2 * t_cn - t_f;
+    double gt = 2 * (t_cn - t_c) / dt + un_cn * (t_f - t_b) / dx;
+    clip(t, max3d(t_f, t_c, t_b) + dt * gt, min3d(t_f, t_c, t_b) + dt * gt);
+    return t;
+}
+
+// The reference's X2 wall branch takes t from tbBuf[i] (source/main.cpp:2215),
+// the line buffer at the position of the line's x index instead of the face's
+// y index.  Position i <= N2+1 holds the backward t of cell (i, i, k) along y
+// (this line's own cell loop, source/main.cpp:2023-2100); beyond, what the
+// X1 sweep's last line (j = N2, k = N3) left: the backward t of cell
+// (i, N2, N3) along x, from the x-faces before that sweep.
+__device__ __forceinline__ double x2_wall_t(const Geo& g, const Fields& F, int i, int k, double dt, const Consts& K) {
+    double* const* a = F.a;
+    if (i <= g.N2 + 1) {
+        const int c = at(g, i, i, k);
+        return backward_t(a[F_t2][c + g.px], a[F_t2][c], a[F_tnCon][c], a[F_tCon][c], a[F_u2nCon][c], dt, K.dx[1]);
+    }
+    const int c = at(g, i, g.N2, g.N3);
+    return backward_t(a[F_t1][c + 1], a[F_t1][c], a[F_tnCon][c], a[F_tCon][c], a[F_u1nCon][c], dt, K.dx[0]);
+}
+
 template <int D, bool GqUsesR>
-__global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, Consts K, const Scalars* S) {
+__global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, Consts K, const Scalars* S,
+                                                int inflow) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     int k = blockIdx.z + 1;
@@ -317,8 +373,66 @@ __global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, C
 
     Fwd fw = cell_forward(sb, dt, dx, K);
     Bwd bw = cell_backward<GqUsesR>(sf, dt, dx, K);
-    FaceOut o = face_combine(fw, bw, sb.un_cn, sf.un_cn, sb.t_cn, sf.t_cn, sb.ua_cn, sf.ua_cn, sb.ub_cn,
-                             sf.ub_cn, T[c], Ua[c], Ub[c], K);
+    if (D == 0 && inflow) {
+        // non-periodic X1 (source/main.cpp:1729-1745): the line's cell loop
+        // skips both ghost cells; the inflow face takes R = 1 in front of a
+        // fluid cell (0 in front of a solid one), t = t0 and no tangential
+        // velocity, the outflow face Q = 5/6, t = t0, no tangential velocity
+        if (q == 1) fw = Fwd{a[F_condition][c] < 0.5 ? 1. : 0., K.t0, 0., 0.};
+        if (q == Nd + 1) bw = Bwd{5. / 6., K.t0, 0., 0.};
+    }
+    const double cond_b = a[F_condition][cb], cond_f = a[F_condition][c];
+    FaceOut o;
+    if (cond_b < 0.5 && cond_f < 0.5) {
+        o = face_combine(fw, bw, sb.un_cn, sf.un_cn, sb.t_cn, sf.t_cn, sb.ua_cn, sf.ua_cn, sb.ub_cn, sf.ub_cn, T[c],
+                         Ua[c], Ub[c], K);
+    } else {
+        // a face on a solid cell (source/main.cpp:1962-2001, 2206-2245,
+        // 2445-2484): no normal flow; the fluid side's outgoing invariant or
+        // its own state, by the sign of its normal velocity.  The X2 branch
+        // takes the first tangential velocity from the normal one (u2nCon
+        // where u1nCon belongs) and t from tbBuf[i].
+        const double sound = K.sound, ro0_g = K.ro0;
+        const double* ua_wall = D == 1 ? unN : uaN;
+        o.un = 0.;
+        if (cond_b > 0.5 && cond_f < 0.5) {
+            if (unN[c] <= 0) {
+                const double qn = bw.q;
+                o.a = bw.a;
+                o.b = bw.b;
+                o.p = -qn * sound * ro0_g;
+                o.ro = ro0_g + o.p / (sound * sound);
+                o.t = D == 1 ? x2_wall_t(g, F, i, k, dt, K) : bw.t;
+            } else {
+                o.a = ua_wall[c];
+                o.b = ubN[c];
+                o.ro = roN[c];
+                o.p = (o.ro - ro0_g) * (sound * sound);
+                o.t = tN[c];
+            }
+        } else if (cond_b < 0.5 && cond_f > 0.5) {
+            if (unN[cb] >= 0) {
+                const double rn = fw.r;
+                o.a = fw.a;
+                o.b = fw.b;
+                o.p = rn * sound * ro0_g;
+                o.ro = ro0_g + o.p / (sound * sound);
+                o.t = fw.t;
+            } else {
+                o.a = ua_wall[cb];
+                o.b = ubN[cb];
+                o.ro = roN[cb];
+                o.p = (o.ro - ro0_g) * (sound * sound);
+                o.t = tN[cb];
+            }
+        } else {
+            o.a = 0.;
+            o.b = 0.;
+            o.p = 0.;
+            o.ro = ro0_g;
+            o.t = K.t0;
+        }
+    }
     nx.p[0][c] = o.un;
     nx.p[1][c] = o.a;
     nx.p[2][c] = o.b;
@@ -493,12 +607,22 @@ hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Co
     return hipGetLastError();
 }
 
-hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan) {
+static dim3 plane_grid(const Geo& g, int n) {
     int m = g.N1 * g.N2;
     if (g.N1 * g.N3 > m) m = g.N1 * g.N3;
     if (g.N2 * g.N3 > m) m = g.N2 * g.N3;
-    dim3 grid((2 * m + 255) / 256, plan.n, 3);
-    hipLaunchKernelGGL(k_ghost, grid, dim3(256), 0, st, g, plan);
+    return dim3((2 * m + 255) / 256, n, 3);
+}
+
+hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan, int zero) {
+    if (plan.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ghost, plane_grid(g, plan.n), dim3(256), 0, st, g, plan, zero);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_gradient(hipStream_t st, const Geo& g, const GhostPlan& plan) {
+    if (plan.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_gradient, plane_grid(g, plan.n), dim3(256), 0, st, g, plan);
     return hipGetLastError();
 }
 
@@ -523,17 +647,23 @@ hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, cons
 }
 
 hipError_t launch_phase2(hipStream_t st, int axis, int scheme, const Geo& g, const Fields& F, const NextFaces& nx,
-                         const Consts& K, const Scalars* S) {
+                         const Consts& K, const Scalars* S, int inflow) {
     // reference scheme: gq from R on the X1 and X3 sweeps; corrected: from Q everywhere
     const bool quirk = scheme == 0;
     if (axis == 0) {
-        if (quirk) hipLaunchKernelGGL((k_phase2<0, true>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S);
-        else hipLaunchKernelGGL((k_phase2<0, false>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S);
+        if (quirk)
+            hipLaunchKernelGGL((k_phase2<0, true>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S,
+                               inflow);
+        else
+            hipLaunchKernelGGL((k_phase2<0, false>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S,
+                               inflow);
     } else if (axis == 1) {
-        hipLaunchKernelGGL((k_phase2<1, false>), cell_grid(g.N1, g.N2 + 1, g.N3), kBlock, 0, st, g, F, nx, K, S);
+        hipLaunchKernelGGL((k_phase2<1, false>), cell_grid(g.N1, g.N2 + 1, g.N3), kBlock, 0, st, g, F, nx, K, S, 0);
     } else {
-        if (quirk) hipLaunchKernelGGL((k_phase2<2, true>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S);
-        else hipLaunchKernelGGL((k_phase2<2, false>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S);
+        if (quirk)
+            hipLaunchKernelGGL((k_phase2<2, true>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S, 0);
+        else
+            hipLaunchKernelGGL((k_phase2<2, false>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S, 0);
     }
     return hipGetLastError();
 }

@@ -32,7 +32,8 @@ hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, 
                       Scalars* S);
 hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
 hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
-hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan);
+hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan, int zero);   // zero: non-periodic exchange
+hipError_t launch_zero_gradient(hipStream_t st, const Geo& g, const GhostPlan& plan);
 struct BcFields { double* f[5]; };   // u1, u2, u3, ro, t
 hipError_t launch_boundary(hipStream_t st, const Geo& g, const BcFields& F, int axis, int periodic, double t0);
 hipError_t launch_plane(hipStream_t st, const Geo& g, double* field, int axis, int q, double* buf, int to_buf);
@@ -42,7 +43,7 @@ hipError_t launch_zero(hipStream_t st, double* const* dev_ptrs, int narr, long l
 hipError_t launch_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_phase2(hipStream_t st, int axis, int scheme, const Geo& g, const Fields& F, const NextFaces& nx,
-                         const Consts& K, const Scalars* S);
+                         const Consts& K, const Scalars* S, int inflow);   // inflow: non-periodic X1 ends
 hipError_t launch_rho_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_diag(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, double* part, int nblocks,
                        double* out);

@@ -69,8 +69,17 @@ typedef enum {
 
 /* Replaces the reference's scalar globals (headers/constants.h:20,
  * headers/bulk.h:17-34) for one rank: N_d = n_d - 1 interior cells per axis,
- * grid steps, CFL, VIS, sound, ro0_g, t0.  periodic must be 1 on every axis
- * (the reference's MPI exchange with periods={1,1,1}, source/main.cpp:56-61). */
+ * grid steps, CFL, VIS, sound, ro0_g, t0.  periodic[d] = 1: the reference's
+ * MPI exchange with periods[d] = 1 on one rank (no physical-boundary branch
+ * on that axis).  periodic[d] = 0: one rank of the reference's own set-up,
+ * periods={0,0,0} (source/main.cpp:55-61): the exchange along d delivers
+ * zeros (MPI_PROC_NULL neighbours; the reference copies receive buffers that
+ * nothing wrote) and the physical-boundary branches of d run (X1: zero-gradient
+ * ends and the inflow/outflow invariants, source/main.cpp:803-823, 1633-1658,
+ * 1729-1745; X3: zero-gradient ends, 825-845, 1660-1698; X2 has none: walls
+ * are solid cells in `condition`).  Non-periodic axes and solid cells run on
+ * the FP64 phase path; the fused and mixed-precision steps need a fluid,
+ * triply periodic box. */
 typedef struct {
     int N1, N2, N3;
     double dx1, dx2, dx3;

@@ -78,6 +78,64 @@ static void fill_face_axis(const geom* g, double* a, int d) {
         }
 }
 
+/* ---- the channel (boundary = 1): one rank of the reference's own set-up,
+ * periods={0,0,0} (source/main.cpp:55-61), coordX=coordY=coordZ=0=p-1.
+ * Every neighbour is MPI_PROC_NULL, so an exchange leaves the receive buffer
+ * as allocated and copies it into the ghost plane.  The reference allocates
+ * the buffers uninitialised (Arr2d.cpp:7-12); its results are defined here
+ * for buffers that hold zeros, which is what glibc delivers with
+ * MALLOC_PERTURB_=255 and tcache off.  Density and velocity do not depend on
+ * the buffers' contents (checked with other fill patterns); the temperature
+ * does, through the X2 sweep's tbBuf[i] (source/main.cpp:2215).  Then the
+ * physical-boundary blocks run: zero-gradient planes at the x ends
+ * (source/main.cpp:803-823, 1633-1658) and the z ends (825-845, 1660-1698). */
+static void zero_cell_axis(const geom* g, double* a, int d) {
+    int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+    for (int p = 1; p <= g->N[o2]; ++p)
+        for (int q = 1; q <= g->N[o1]; ++q) {
+            long base = q * g->cs[o1] + p * g->cs[o2];
+            a[base] = 0.0;
+            a[base + (long)(g->N[d] + 1) * g->cs[d]] = 0.0;
+        }
+}
+static void zero_face_axis(const geom* g, double* a, int d) {
+    int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+    for (int p = 1; p <= g->N[o2]; ++p)
+        for (int q = 1; q <= g->N[o1]; ++q) {
+            long base = q * g->fs[o1] + p * g->fs[o2];
+            a[base] = 0.0;
+            a[base + (long)(g->N[d] + 2) * g->fs[d]] = 0.0;
+        }
+}
+/* cell planes 0 <- 1 and N_d+1 <- N_d (interior of the two other axes) */
+static void zero_gradient_cells(const geom* g, double* a, int d) {
+    int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+    for (int p = 1; p <= g->N[o2]; ++p)
+        for (int q = 1; q <= g->N[o1]; ++q) {
+            long base = q * g->cs[o1] + p * g->cs[o2];
+            a[base + (long)(g->N[d] + 1) * g->cs[d]] = a[base + (long)g->N[d] * g->cs[d]];
+            a[base] = a[base + g->cs[d]];
+        }
+}
+/* face planes 0 <- 1 and N_d+2 <- N_d+1 */
+static void zero_gradient_faces(const geom* g, double* a, int d) {
+    int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+    for (int p = 1; p <= g->N[o2]; ++p)
+        for (int q = 1; q <= g->N[o1]; ++q) {
+            long base = q * g->fs[o1] + p * g->fs[o2];
+            a[base] = a[base + g->fs[d]];
+            a[base + (long)(g->N[d] + 2) * g->fs[d]] = a[base + (long)(g->N[d] + 1) * g->fs[d]];
+        }
+}
+static void exchange_cells(const ark_oracle_state* s, const geom* g, double* a, int d) {
+    if (s->boundary) zero_cell_axis(g, a, d);
+    else fill_cell_axis(g, a, d);
+}
+static void exchange_faces(const ark_oracle_state* s, const geom* g, double* a, int d) {
+    if (s->boundary) zero_face_axis(g, a, d);
+    else fill_face_axis(g, a, d);
+}
+
 /* ---- TimeStepSize, source/main.cpp:374-415 (one rank: the MIN allreduce is
  * the identity). */
 void ark_oracle_time_step_size(ark_oracle_state* s) {
@@ -216,10 +274,16 @@ void ark_oracle_stress_tensor(ark_oracle_state* s) {
     long nf = (long)(s->N1 + 3) * (s->N2 + 3) * (s->N3 + 3);
     for (int a = ORC_sigm11; a <= ORC_sigm33; ++a) memset(f[a], 0, sizeof(double) * nf);
     for (int d = 0; d < 3; ++d) {
-        fill_cell_axis(&g, f[ORC_u1Con], d);
-        fill_cell_axis(&g, f[ORC_u2Con], d);
-        fill_cell_axis(&g, f[ORC_u3Con], d);
+        exchange_cells(s, &g, f[ORC_u1Con], d);
+        exchange_cells(s, &g, f[ORC_u2Con], d);
+        exchange_cells(s, &g, f[ORC_u3Con], d);
     }
+    if (s->boundary)   /* source/main.cpp:803-845 */
+        for (int d = 0; d < 3; d += 2) {
+            zero_gradient_cells(&g, f[ORC_u1Con], d);
+            zero_gradient_cells(&g, f[ORC_u2Con], d);
+            zero_gradient_cells(&g, f[ORC_u3Con], d);
+        }
     for (int d = 0; d < 3; ++d) stress_faces(s, &g, d);
     const double ds1 = s->dx2 * s->dx3, ds2 = s->dx1 * s->dx3, ds3 = s->dx1 * s->dx2;
     const int sig[3][3] = {{ORC_sigm11, ORC_sigm12, ORC_sigm13},
@@ -313,7 +377,20 @@ static void phase2_sweep(ark_oracle_state* s, const geom* g, int d, line_buffers
         for (int in = 1; in <= g->N[inner]; ++in) {
             long cbase = o * g->cs[outer] + in * g->cs[inner];
             long fbase = o * g->fs[outer] + in * g->fs[inner];
-            for (int q = 0; q <= n; q++) {
+            int q0 = 0, q1 = n;
+            if (s->boundary && d == 0) {   /* inflow and outflow, source/main.cpp:1729-1745 */
+                q0 = 1;
+                B->r[1] = (cond[cbase + g->cs[d]] < 0.5) ? 1. : 0.;
+                B->tf[1] = t0;
+                B->af[1] = 0.;
+                B->bf[1] = 0.;
+                q1 = n - 1;
+                B->q[n] = 5. / 6.;
+                B->tb[n] = t0;
+                B->ab[n] = 0.;
+                B->bb[n] = 0.;
+            }
+            for (int q = q0; q <= q1; q++) {
                 long c = cbase + q * g->cs[d];
                 long ff = fbase + (q + 1) * g->fs[d], fb = fbase + q * g->fs[d];
                 double u1_f = Un[ff], u1_b = Un[fb], u1_cn = un_n[c], u1_c = un_c[c];
@@ -473,8 +550,14 @@ void ark_oracle_phase2(ark_oracle_state* s) {
                                    {ORC_u12, ORC_u22, ORC_u32, ORC_ro2, ORC_p2},
                                    {ORC_u13, ORC_u23, ORC_u33, ORC_ro3, ORC_p3}};
     for (int d = 0; d < 3; ++d) {
-        for (int a = 0; a < 7; ++a) fill_cell_axis(&g, f[cell_fields[a]], d);
-        for (int a = 0; a < 5; ++a) fill_face_axis(&g, f[face_fields[d][a]], d);
+        for (int a = 0; a < 7; ++a) exchange_cells(s, &g, f[cell_fields[a]], d);
+        for (int a = 0; a < 5; ++a) exchange_faces(s, &g, f[face_fields[d][a]], d);
+    }
+    if (s->boundary) {   /* source/main.cpp:1633-1698 */
+        const int x_cells[] = {ORC_u1nCon, ORC_u2nCon, ORC_u3nCon, ORC_ronCon, ORC_tnCon};
+        for (int a = 0; a < 5; ++a) zero_gradient_cells(&g, f[x_cells[a]], 0);
+        for (int a = 0; a < 7; ++a) zero_gradient_cells(&g, f[cell_fields[a]], 2);
+        for (int a = 0; a < 5; ++a) zero_gradient_faces(&g, f[face_fields[2][a]], 2);
     }
     int nmax = s->N1 + 1;
     if (s->N2 + 1 > nmax) nmax = s->N2 + 1;
@@ -581,4 +664,74 @@ void ark_oracle_init_tgv(ark_oracle_state* s, double U0) {
                     f[R[d]][q] = s->ro0 + p / (s->sound * s->sound);
                     f[T[d]][q] = s->t0;
                 }
+}
+
+/* ---- the reference's own problem, source/main.cpp:118-372 (Input's
+ * constants, InitializeData's fields) on one rank (coords 0). */
+void ark_oracle_init_channel(ark_oracle_state* s, int n1_g, int n2_g, int n3_g, double x1_e, double x2_n,
+                             double x3_t, double u10) {
+    geom g = make_geom(s);
+    double** f = s->f;
+    const int n1 = s->N1 + 1, n2 = s->N2 + 1, n3 = s->N3 + 1;
+    for (int a = 0; a < ORC_NUM_FIELDS; ++a) {
+        long n = ark_oracle_is_face(a) ? (long)(s->N1 + 3) * (s->N2 + 3) * (s->N3 + 3)
+                                       : (long)(s->N1 + 2) * (s->N2 + 2) * (s->N3 + 2);
+        memset(f[a], 0, sizeof(double) * n);
+    }
+    const double x1_w = 0., x2_s = 0., x3_b = 0.;
+    s->dx1 = (x1_e - x1_w) / (n1_g - 1);
+    s->dx2 = (x2_n - x2_s) / (n2_g - 1);
+    s->dx3 = (x3_t - x3_b) / (n3_g - 1);
+    /* node coordinates by running sums, as source/main.cpp:291-312 */
+    double* x1 = (double*)malloc(sizeof(double) * (n1 + 2));
+    double* x2 = (double*)malloc(sizeof(double) * (n2 + 2));
+    x1[0] = x1_w + ((n1 - 1) * s->dx1 * 0) - s->dx1;
+    x2[0] = x2_s + ((n2 - 1) * s->dx2 * 0) - s->dx2;
+    for (int i = 0; i < n1 + 1; ++i) x1[i + 1] = x1[i] + s->dx1;
+    for (int j = 0; j < n2 + 1; ++j) x2[j + 1] = x2[j] + s->dx2;
+    const double ro0_g = s->ro0, t0 = s->t0;
+    for (int i = 0; i <= n1; ++i)
+        for (int j = 0; j <= n2; ++j)
+            for (int k = 0; k <= n3; ++k) {
+                double y = 0.5 * (x2[j] + x2[j + 1]);
+                long c = C(&g, i, j, k);
+                f[ORC_u1nCon][c] = y <= 1 ? 0. : u10;
+                f[ORC_u2nCon][c] = 0.;
+                f[ORC_u3nCon][c] = 0.;
+                f[ORC_ronCon][c] = ro0_g;
+                f[ORC_tnCon][c] = t0;
+            }
+    for (int i = 1; i <= n1; ++i)
+        for (int j = 1; j <= n2; ++j)
+            for (int k = 1; k <= n3; ++k) {
+                double y = x2[j], y_c = 0.5 * (x2[j] + x2[j + 1]);
+                long c = F(&g, i, j, k);
+                f[ORC_p1][c] = f[ORC_p2][c] = f[ORC_p3][c] = 0.;
+                f[ORC_u11][c] = y_c <= 1 ? 0. : u10;
+                f[ORC_u12][c] = y <= 1 ? 0. : u10;
+                f[ORC_u13][c] = y_c <= 1 ? 0. : u10;
+                f[ORC_u21][c] = f[ORC_u22][c] = f[ORC_u23][c] = 0.;
+                f[ORC_u31][c] = f[ORC_u32][c] = f[ORC_u33][c] = 0.;
+                f[ORC_ro1][c] = f[ORC_ro2][c] = f[ORC_ro3][c] = ro0_g;
+            }
+    for (int i = 0; i < n1 + 1; ++i)
+        for (int j = 0; j < n2 + 1; ++j)
+            for (int k = 0; k < n3 + 1; ++k) {
+                double x = 0.5 * (x1[i] + x1[i + 1]), y = 0.5 * (x2[j] + x2[j + 1]);
+                f[ORC_condition][C(&g, i, j, k)] = (x <= 5 && y <= 1) ? 1 : 0;
+            }
+    for (int i = 0; i < n1 + 1; ++i)
+        for (int k = 0; k < n3 + 1; ++k) {
+            f[ORC_condition][C(&g, i, 0, k)] = 1;
+            f[ORC_condition][C(&g, i, n2, k)] = 1;
+        }
+    long nc = (long)(s->N1 + 2) * (s->N2 + 2) * (s->N3 + 2);
+    memcpy(f[ORC_roCon], f[ORC_ronCon], sizeof(double) * nc);
+    memcpy(f[ORC_u1Con], f[ORC_u1nCon], sizeof(double) * nc);
+    memcpy(f[ORC_u2Con], f[ORC_u2nCon], sizeof(double) * nc);
+    memcpy(f[ORC_u3Con], f[ORC_u3nCon], sizeof(double) * nc);
+    memcpy(f[ORC_tCon], f[ORC_tnCon], sizeof(double) * nc);
+    s->boundary = 1;
+    free(x1);
+    free(x2);
 }

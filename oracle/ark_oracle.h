@@ -38,7 +38,14 @@ typedef struct {
     double dt, CFL, VIS, sound, ro0, t0;
     double* f[ORC_NUM_FIELDS];
     int scheme;   /* 0: the reference (gq from R on X1/X3); 1: gq from Q on every axis */
+    int boundary; /* 0: periodic box; 1: the reference's channel (see ark_oracle.c) */
 } ark_oracle_state;
+
+/* The reference's own problem (Input/InitializeData, source/main.cpp:118-372)
+ * on one rank with the given node counts and lengths: level-n fields, interior
+ * faces and the solid mask; every other element 0; Con := nCon. */
+void ark_oracle_init_channel(ark_oracle_state* s, int n1_g, int n2_g, int n3_g, double x1_e, double x2_n,
+                             double x3_t, double u10);
 
 /* 1 for face/stress-sized fields, 0 for cell-sized ones. */
 int ark_oracle_is_face(int slot);

@@ -52,6 +52,7 @@ class _State(ctypes.Structure):
         ("sound", ctypes.c_double), ("ro0", ctypes.c_double), ("t0", ctypes.c_double),
         ("f", ctypes.POINTER(ctypes.c_double) * len(FIELDS)),
         ("scheme", ctypes.c_int),
+        ("boundary", ctypes.c_int),
     ]
 
 
@@ -78,6 +79,9 @@ def lib():
         L.ark_oracle_step.restype = None
         L.ark_oracle_init_tgv.argtypes = [P, ctypes.c_double]
         L.ark_oracle_init_tgv.restype = None
+        L.ark_oracle_init_channel.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.ark_oracle_init_channel.restype = None
         L.ark_oracle_energy.argtypes = [P]
         L.ark_oracle_energy.restype = ctypes.c_double
         L.ark_oracle_mass.argtypes = [P]
@@ -91,9 +95,10 @@ class OracleState:
 
     def __init__(self, N1: int, N2: int, N3: int, dx=(1.0, 1.0, 1.0), dt: float = 0.0,
                  CFL: float = 0.3, VIS: float = 1.0 / 1600.0, sound: float = 10.0,
-                 ro0: float = 1.0, t0: float = 1.0, scheme: int = 0):
+                 ro0: float = 1.0, t0: float = 1.0, scheme: int = 0, boundary: int = 0):
         self.N = (int(N1), int(N2), int(N3))
         self.scheme = int(scheme)
+        self.boundary = int(boundary)
         self.dx = tuple(float(v) for v in dx)
         self.dt, self.CFL, self.VIS = float(dt), float(CFL), float(VIS)
         self.sound, self.ro0, self.t0 = float(sound), float(ro0), float(t0)
@@ -118,7 +123,8 @@ class OracleState:
 
     def copy(self) -> "OracleState":
         o = OracleState(*self.N, dx=self.dx, dt=self.dt, CFL=self.CFL, VIS=self.VIS,
-                        sound=self.sound, ro0=self.ro0, t0=self.t0, scheme=self.scheme)
+                        sound=self.sound, ro0=self.ro0, t0=self.t0, scheme=self.scheme,
+                        boundary=self.boundary)
         for k, v in self.arrays.items():
             o.arrays[k] = v.copy()
         return o
@@ -130,6 +136,7 @@ class OracleState:
         s.dt, s.CFL, s.VIS, s.sound, s.ro0, s.t0 = (self.dt, self.CFL, self.VIS,
                                                    self.sound, self.ro0, self.t0)
         s.scheme = self.scheme
+        s.boundary = self.boundary
         for q, name in enumerate(FIELDS):
             a = self.arrays[name]
             assert a.flags.c_contiguous and a.dtype == np.float64
@@ -190,6 +197,19 @@ class OracleState:
         st = cls(N1, N2, N3, dx=dx, CFL=CFL, VIS=VIS, sound=sound, scheme=scheme)
         s = st._struct()
         lib().ark_oracle_init_tgv(ctypes.byref(s), U0)
+        return st
+
+    @classmethod
+    def channel(cls, n1_g: int = 201, n2_g: int = 61, n3_g: int = 2, x1_e: float = 20.0, x2_n: float = 6.0,
+                x3_t: float = 0.1, u10: float = 1.0, VIS: float = 1.0 / 1000.0, CFL: float = 0.2,
+                sound: float = 10.0) -> "OracleState":
+        """The reference's own problem (source/main.cpp:118-372; defaults are
+        its Input() values): the channel with a backward-facing step, inflow
+        at x = 0, outflow at x = x1_e, walls at y = 0 and y = x2_n."""
+        st = cls(n1_g - 1, n2_g - 1, n3_g - 1, CFL=CFL, VIS=VIS, sound=sound, boundary=1)
+        s = st._struct()
+        lib().ark_oracle_init_channel(ctypes.byref(s), n1_g, n2_g, n3_g, x1_e, x2_n, x3_t, u10)
+        st.dx = (s.dx1, s.dx2, s.dx3)
         return st
 
     # ---- harness file format (oracle/ref_harness.cpp) -------------------------

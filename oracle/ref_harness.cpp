@@ -21,9 +21,14 @@
 //   * Con := nCon at t=0 (the first TimeStepSize reads Con, source/main.cpp:382).
 //
 // Modes:
-//   tgv   N1 N2 N3 STEPS OUTDIR [DUMPSTEP...]   Taylor-Green run, any rank count
-//   phase NAME INDIR OUTDIR                      one reference phase on a saved state (1 rank)
-//   run   INDIR STEPS OUTDIR                     STEPS full steps from a saved state (1 rank)
+//   tgv     N1 N2 N3 STEPS OUTDIR [DUMPSTEP...]  Taylor-Green run, any rank count
+//   phase   NAME INDIR OUTDIR                     one reference phase on a saved state (1 rank)
+//   run     INDIR STEPS OUTDIR                    STEPS full steps from a saved state (1 rank)
+//   channel STEPS OUTDIR [DUMPSTEP...]            the reference's own problem (1 rank): its
+//           Input() and InitializeData() (source/main.cpp:118-372), periods={0,0,0}
+//           as in its main (source/main.cpp:55-61), so every physical-boundary
+//           branch is live; arrays Input() allocates and InitializeData() leaves
+//           unset are zero-filled first, and Con := nCon as above.
 // Options (before the mode): -nu V  -cfl V  -c V  -quiet
 #include <mpi.h>
 #include <stdio.h>
@@ -95,10 +100,9 @@ void allocate_state() {
     u3bBuf = new double[nmax + 2];
 }
 
-void setup_comm(int periodic_dims_from_world) {
-    (void)periodic_dims_from_world;
+void setup_comm(int periodic) {
     dims[0] = 0; dims[1] = 0; dims[2] = 1;   // the reference never splits z (main.cpp:54)
-    int periods[3] = {1, 1, 1};
+    int periods[3] = {periodic, periodic, periodic};
     MPI_Comm_size(MPI_COMM_WORLD, &nproc);
     MPI_Dims_create(nproc, 3, dims);
     MPI_Cart_create(MPI_COMM_WORLD, 3, dims, periods, 0, &newComm);
@@ -307,7 +311,8 @@ int usage() {
     fprintf(stderr,
             "usage: ark_ref_harness [-nu V] [-cfl V] [-c V] [-quiet] tgv N1 N2 N3 STEPS OUTDIR [DUMPSTEP...]\n"
             "       ark_ref_harness phase NAME INDIR OUTDIR\n"
-            "       ark_ref_harness run INDIR STEPS OUTDIR\n");
+            "       ark_ref_harness run INDIR STEPS OUTDIR\n"
+            "       ark_ref_harness channel STEPS OUTDIR [DUMPSTEP...]\n");
     return 2;
 }
 
@@ -328,7 +333,7 @@ int main(int argc, char** argv) {
     }
     if (a >= argc) { MPI_Finalize(); return usage(); }
     std::string mode(argv[a++]);
-    setup_comm(0);
+    setup_comm(mode == "channel" ? 0 : 1);
     nStep = 0; TIME = 0.0; l = 1;
     needSwap = true; useTecplot = false; dirPath = "./";
     int rc = 0;
@@ -395,6 +400,55 @@ int main(int argc, char** argv) {
                    "\"loop_seconds\": %.6f, \"mcell_steps_per_s\": %.6f}\n",
                    nproc, px, py, pz, N1, N2, N3, steps, max_time, mcs);
         }
+    } else if (mode == "channel") {
+        if (nproc != 1) { if (rank == 0) fprintf(stderr, "channel mode needs exactly one rank\n"); MPI_Abort(MPI_COMM_WORLD, 2); }
+        if (argc - a < 2) { MPI_Finalize(); return usage(); }
+        int steps = atoi(argv[a]);
+        std::string out(argv[a + 1]);
+        std::vector<int> dumps;
+        for (int q = a + 2; q < argc; ++q) dumps.push_back(atoi(argv[q]));
+        ark_ref_Input();
+        for (int q = 0; q < kNumArrays; ++q) {
+            double* p = (*g_arrays[q].arr)->ref();
+            for (long e = 0; e < count_of(g_arrays[q]); ++e) p[e] = 0.0;
+        }
+        int nmax = (int)max3d(n1, n2, n3);
+        for (int e = 0; e < nmax + 2; ++e) {
+            rBuf[e] = tfBuf[e] = tbBuf[e] = u2fBuf[e] = u2bBuf[e] = u3fBuf[e] = u3bBuf[e] = 0.0;
+            if (e < nmax + 1) qBuf[e] = 0.0;
+        }
+        ark_ref_InitializeData();
+        for (int e = 0; e < (n1 + 1) * (n2 + 1) * (n3 + 1); ++e) {
+            roCon->ref()[e] = ronCon->ref()[e];
+            u1Con->ref()[e] = u1nCon->ref()[e];
+            u2Con->ref()[e] = u2nCon->ref()[e];
+            u3Con->ref()[e] = u3nCon->ref()[e];
+            tCon->ref()[e] = tnCon->ref()[e];
+        }
+        const int N1 = n1 - 1, N2 = n2 - 1, N3 = n3 - 1;
+        auto dump_if = [&](int s) {
+            for (int d : dumps)
+                if (d == s) {
+                    std::string dir = out + "/s" + std::to_string(s);
+                    std::string cmd = "mkdir -p " + dir;
+                    if (system(cmd.c_str()) != 0) MPI_Abort(MPI_COMM_WORLD, 2);
+                    dump_local(dir, N1, N2, N3);
+                }
+        };
+        std::string fn = out + "/series.txt";
+        std::string mk = "mkdir -p " + out;
+        if (system(mk.c_str()) != 0) MPI_Abort(MPI_COMM_WORLD, 2);
+        FILE* series = fopen(fn.c_str(), "w");
+        if (!series) { fprintf(stderr, "cannot write %s\n", fn.c_str()); MPI_Abort(MPI_COMM_WORLD, 2); }
+        fprintf(series, "# step dt TIME ke_reference mass\n");
+        fprintf(series, "0 %a %a %a %a\n", 0.0, 0.0, energy_like_reference(), mass_sum());
+        dump_if(0);
+        for (int s = 1; s <= steps; ++s) {
+            one_step();
+            fprintf(series, "%d %a %a %a %a\n", nStep, dt, TIME, energy_like_reference(), mass_sum());
+            dump_if(s);
+        }
+        fclose(series);
     } else if (mode == "phase" || mode == "run") {
         if (nproc != 1) { if (rank == 0) fprintf(stderr, "%s mode needs exactly one rank\n", mode.c_str()); MPI_Abort(MPI_COMM_WORLD, 2); }
         std::string name, in, out;

@@ -20,6 +20,16 @@ Fixtures written:
       BoundaryConditions class's output (oracle/_ref/ark_ref_bc, built from
       source/BoundaryConditions.cpp and source/Arr3d.cpp) for alongX1/2/3,
       periodic and not.  ``python make_fixtures.py bc`` writes only this file.
+  channel_digests.json
+      the reference's own problem (its Input() and InitializeData(), the
+      200x60x1 channel with a backward-facing step, periods={0,0,0}, one rank):
+      per-array SHA-256 of all 41 arrays after 0, 1, 10, 100 and 200 steps and
+      the dt/KE/mass series.  The run has its MPI receive buffers zero-filled
+      (glibc MALLOC_PERTURB_=255, tcache off): with MPI_PROC_NULL neighbours
+      the reference copies those uninitialised buffers into its ghost planes.
+      Two more runs with other heap contents show which arrays depend on them
+      (recorded as "buffer_dependence").  ``python make_fixtures.py channel``
+      writes only this file.
   tgv_digests.json
       per-array SHA-256 and norms of the 16^3 x 10-step and 64^3 x 200-step
       Taylor-Green runs (1 rank, full arrays), their per-step dt, KE and mass
@@ -164,7 +174,65 @@ def bc_fixture(path: str, work: str) -> None:
     np.savez_compressed(path, **data)
 
 
+CHANNEL_ZERO_ENV = {"MALLOC_PERTURB_": "255", "GLIBC_TUNABLES": "glibc.malloc.tcache_count=0"}
+
+
+def channel_run(work: str, steps: int, dumps, env_extra) -> str:
+    d = os.path.join(work, "channel_" + "_".join(f"{k}{v}" for k, v in sorted(env_extra.items())).replace("=", "")
+                     .replace(".", ""))
+    shutil.rmtree(d, ignore_errors=True)
+    os.makedirs(d)
+    env = dict(os.environ)
+    for k in ("MALLOC_PERTURB_", "GLIBC_TUNABLES"):
+        env.pop(k, None)
+    env.update(env_extra)
+    subprocess.run([MPIRUN, "-n", "1", HARNESS, "channel", str(steps), d] + [str(x) for x in dumps], check=True,
+                   capture_output=True, text=True, env=env)
+    return d
+
+
+def channel_fixture(path: str, work: str) -> None:
+    steps, dumps = 200, [0, 1, 10, 100, 200]
+    ref = channel_run(work, steps, dumps, CHANNEL_ZERO_ENV)
+    out = {"problem": "reference Input()/InitializeData(): 201x61x2 nodes, 20 x 6 x 0.1, step x<=5 y<=1, "
+                      "u10=1, c=10, VIS=1e-3, CFL=0.2, periods={0,0,0}, 1 rank",
+           "N": [200, 60, 1], "env": CHANNEL_ZERO_ENV, "series": read_series(ref), "states": {}}
+    for s in dumps:
+        st = OracleState.from_dir(os.path.join(ref, f"s{s}"))
+        out["states"][str(s)] = {n: digest(st[n]) for n in FIELDS}
+    # which arrays depend on the uninitialised receive buffers' contents
+    dep = {}
+    for env_extra in ({}, {"MALLOC_PERTURB_": "165"}):
+        other = channel_run(work, steps, [steps], env_extra)
+        a = OracleState.from_dir(os.path.join(ref, f"s{steps}"))
+        b = OracleState.from_dir(os.path.join(other, f"s{steps}"))
+        N1, N2, N3 = a.N
+        for n in FIELDS:
+            x, y = a[n], b[n]
+            e = 1 if n in FIELDS[FIELDS.index("ro1"):] else 0
+            interior = (slice(1, N3 + 1 + e), slice(1, N2 + 1 + e), slice(1, N1 + 1 + e))
+            dep.setdefault(n, {"whole_array": True, "interior": True})
+            if not np.array_equal(x.view(np.uint64), y.view(np.uint64)):
+                dep[n]["whole_array"] = False
+            if not np.array_equal(x[interior].view(np.uint64), y[interior].view(np.uint64)):
+                dep[n]["interior"] = False
+    out["buffer_dependence"] = {"independent_whole_array": sorted(n for n, v in dep.items() if v["whole_array"]),
+                                "independent_interior": sorted(n for n, v in dep.items() if v["interior"]),
+                                "dependent_interior": sorted(n for n, v in dep.items() if not v["interior"])}
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def main() -> None:
+    if len(sys.argv) > 1 and sys.argv[1] == "channel":
+        if not os.path.exists(HARNESS):
+            sys.exit("reference harness not built: make -C oracle ref")
+        work = tempfile.mkdtemp(prefix="ark_golden_")
+        try:
+            channel_fixture(os.path.join(HERE, "channel_digests.json"), work)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "bc":
         if not os.path.exists(BC_HARNESS):
             sys.exit("reference BoundaryConditions harness not built: make -C oracle ref")

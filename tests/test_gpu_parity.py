@@ -219,8 +219,9 @@ def test_errors_are_reported():
         cond = np.zeros(s.shape("condition"))
         cond[2, 2, 2] = 1.0
         s.upload("condition", cond)
+        # solid cells run on the phase path; the fused-only entry refuses them
         with pytest.raises(arkamd.ArkError) as e:
-            s.phase2()
+            s.advance(1)
         assert e.value.status == arkamd.ARK_ERR_UNSUPPORTED
     with pytest.raises(arkamd.ArkError) as e:
         arkamd.Solver(0, 4, 4, dx=(1.0, 1.0, 1.0))

@@ -214,3 +214,47 @@ def test_corrected_scheme_keeps_tgv_bounded():
     st.step(150)
     umax = max(float(np.abs(st[n][1:-1, 1:-1, 1:-1]).max()) for n in ("u1nCon", "u2nCon", "u3nCon"))
     assert umax < 1.0
+
+
+# ---- the reference's own problem: channel with a backward-facing step -------
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def test_channel_oracle_matches_reference_digests():
+    """The oracle's restatement of Input()/InitializeData() and of the
+    physical-boundary branches (inflow, outflow, zero-gradient ends, solid
+    walls and step) reproduces the reference's channel run on all 41 arrays,
+    ghosts included, bit for bit, and its dt and mass series."""
+    dig = json.load(open(os.path.join(GOLDEN, "channel_digests.json")))
+    st = OracleState.channel()
+    assert list(st.N) == dig["N"]
+    done, dts = 0, []
+    for s in sorted(int(k) for k in dig["states"]):
+        if s > done:
+            dts.extend(st.step(s - done).tolist())
+            done = s
+        bad = [n for n in FIELDS if _sha(st[n]) != dig["states"][str(s)][n]["sha256"]]
+        assert not bad, (s, bad)
+    want_dt = [float.fromhex(v) for v in dig["series"]["dt"][1:done + 1]]
+    assert dts == want_dt
+    assert float.fromhex(dig["series"]["mass"][done]) == st.mass()
+
+
+def test_channel_flow_is_physical():
+    """Checks of the channel that do not rest on the digests: inflow enters
+    at u = 1 (the R invariant the inlet imposes, with p = 0 there), no normal
+    flow crosses the step's faces or the walls, and the field stays finite."""
+    st = OracleState.channel()
+    st.step(50)
+    N1, N2, _ = st.N
+    solid = st["condition"][1, 1:N2 + 1, 1:N1 + 1] > 0.5          # (N2, N1) interior cells
+    inflow = st["u11"][1, 1:N2 + 1, 1][~solid[:, 0]]                # inlet faces of fluid rows
+    assert np.all(np.abs(inflow - 1.0) < 0.05), inflow
+    # x-faces between a fluid and a solid cell, and y-faces on the walls, carry no normal flow
+    xf = st["u11"][1, 1:N2 + 1, 2:N1 + 1]
+    between = solid[:, :-1] != solid[:, 1:]
+    assert np.all(xf[between] == 0.0)
+    assert np.all(st["u22"][1, 1, 1:N1 + 1] == 0.0) and np.all(st["u22"][1, N2 + 1, 1:N1 + 1] == 0.0)
+    for n in ("ronCon", "u1nCon", "u2nCon", "tnCon"):
+        assert np.all(np.isfinite(st[n][1, 1:N2 + 1, 1:N1 + 1]))
