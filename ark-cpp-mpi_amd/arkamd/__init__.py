@@ -54,6 +54,7 @@ EXPORTED_SYMBOLS = [
     "ark_mlp_param_count", "ark_mlp_create", "ark_mlp_destroy", "ark_mlp_last_error", "ark_mlp_set_params",
     "ark_mlp_get_params", "ark_mlp_set_normalization", "ark_mlp_set_data", "ark_mlp_gradient", "ark_mlp_train",
     "ark_mlp_predict", "ark_set_closure", "ark_init_velocity",
+    "ark_write_paraview", "ark_average_values", "ark_write_average_values",
 ]
 
 
@@ -162,6 +163,9 @@ def lib() -> ctypes.CDLL:
     L.ark_mlp_train.argtypes = [vp, ip, ll, ctypes.c_float, dp]
     L.ark_mlp_predict.argtypes = [vp, fp, fp, ll]
     L.ark_set_closure.argtypes = [vp, vp, ctypes.c_double]
+    L.ark_write_paraview.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    L.ark_average_values.argtypes = [vp]
+    L.ark_write_average_values.argtypes = [vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int]
     L.ark_init_velocity.argtypes = [vp, dp]
     for fn in EXPORTED_SYMBOLS:
         if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error",
@@ -305,6 +309,26 @@ class Solver:
     def step_phases(self, nsteps: int = 1) -> None:
         """nsteps through one launch per reference phase."""
         self._check(self._L.ark_step_phases(self._h, int(nsteps)))
+
+    # ---- the reference's output path (source/main.cpp:2577-2891) ----------------
+    def write_paraview(self, dir_path: str, need_swap: bool = True) -> str:
+        """WriteDataParaView: <dir_path>out_<nStep>.vtk; returns the path."""
+        if not dir_path.endswith(os.sep):
+            dir_path += os.sep
+        self._check(self._L.ark_write_paraview(self._h, dir_path.encode(), 1 if need_swap else 0))
+        return os.path.join(dir_path, f"out_{self.diagnostics(allow_nonfinite=True)['nstep']}.vtk")
+
+    def average_values(self) -> None:
+        """AverageValues: averageU += u * dt (device)."""
+        self._check(self._L.ark_average_values(self._h))
+
+    def write_average_values(self, dir_path: str, start_time: float, need_swap: bool = True) -> str:
+        """WriteAverageValues: <dir_path>average.vtk; returns the path."""
+        if not dir_path.endswith(os.sep):
+            dir_path += os.sep
+        self._check(self._L.ark_write_average_values(self._h, dir_path.encode(), float(start_time),
+                                                     1 if need_swap else 0))
+        return os.path.join(dir_path, "average.vtk")
 
     def init_tgv(self, U0: float = 1.0) -> None:
         self._check(self._L.ark_init_tgv(self._h, float(U0)))

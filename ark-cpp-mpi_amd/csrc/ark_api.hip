@@ -18,7 +18,6 @@ using namespace arkd;
 
 namespace {
 
-constexpr int kOriginPad = 15;  // element (0,0,0) sits 15 elements into the allocation
 
 bool is_face_field(int f) { return f >= F_ro1; }
 // The eighteen persistent face fields are double-buffered (Phase2 reads the

@@ -14,6 +14,10 @@ struct ark_prof_event {
     hipEvent_t a, b;
 };
 
+// Element (0,0,0) of every field sits 15 elements into its allocation, so
+// interior element i = 1 of every row starts a 128-byte line.
+constexpr int kOriginPad = 15;
+
 struct ark_ctx {
     ark_params prm{};
     arkd::Geo geo{};
@@ -47,6 +51,7 @@ struct ark_ctx {
     double closure_delta = 0.0;
     float* les_X = nullptr;
     float* les_Y = nullptr;
+    double* average = nullptr;         // AverageValues' three accumulators (cell fields, common index map)
     void* mixed_units = nullptr;       // chunk table of the mixed-precision step (int4 per chunk)
     int mixed_nunits = 0;
     int diag_blocks = 0;

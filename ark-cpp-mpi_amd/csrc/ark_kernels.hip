@@ -451,6 +451,48 @@ __global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, C
 
 
 // ---------------------------------------------------------------------------
+// Vorticity at the cell centres from the face velocities, the expression of
+// WriteDataParaView (source/main.cpp:2712-2733), interior cells; T is the
+// face storage type (double, or float for mixed precision).  out[m] holds
+// component m in the common index map.
+template <class T>
+__global__ void __launch_bounds__(256) k_vorticity(Geo g, Fields F, Consts K, double* o1, double* o2, double* o3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.N1) return;
+    double* const* a = F.a;
+    const T* u32 = (const T*)a[F_u32];
+    const T* u23 = (const T*)a[F_u23];
+    const T* u13 = (const T*)a[F_u13];
+    const T* u31 = (const T*)a[F_u31];
+    const T* u21 = (const T*)a[F_u21];
+    const T* u12 = (const T*)a[F_u12];
+    const int c = at(g, i, j, k);
+    const double dx1 = K.dx[0], dx2 = K.dx[1], dx3 = K.dx[2];
+    double d2u3 = ((double)u32[c + g.px] - (double)u32[c]) / dx2,
+           d3u2 = ((double)u23[c + g.sxy] - (double)u23[c]) / dx3,
+           d3u1 = ((double)u13[c + g.sxy] - (double)u13[c]) / dx3,
+           d1u3 = ((double)u31[c + 1] - (double)u31[c]) / dx1,
+           d1u2 = ((double)u21[c + 1] - (double)u21[c]) / dx1,
+           d2u1 = ((double)u12[c + g.px] - (double)u12[c]) / dx2;
+    o1[c] = d2u3 - d3u2;
+    o2[c] = d3u1 - d1u3;
+    o3[c] = d1u2 - d2u1;
+}
+
+// AverageValues (source/main.cpp:2878-2891): averageU_m += u_m * dt over the
+// interior cells.
+__global__ void __launch_bounds__(256) k_average(Geo g, Fields F, const Scalars* S, double* a1, double* a2,
+                                                 double* a3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.N1) return;
+    const int c = at(g, i, j, k);
+    const double dt = S->dt;
+    a1[c] += F.a[F_u1nCon][c] * dt;
+    a2[c] += F.a[F_u2nCon][c] * dt;
+    a3[c] += F.a[F_u3nCon][c] * dt;
+}
+
+// ---------------------------------------------------------------------------
 // Face densities from face pressures, rho = ro0 + p/c^2 (source/main.cpp:1930),
 // on the faces a Phase2 sweep writes (1..N_d+1 along d, interior across).
 // The fused step does not store face densities (it derives them); this
@@ -665,6 +707,20 @@ hipError_t launch_phase2(hipStream_t st, int axis, int scheme, const Geo& g, con
         else
             hipLaunchKernelGGL((k_phase2<2, false>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S, 0);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_vorticity(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, bool float_faces,
+                            double* o1, double* o2, double* o3) {
+    dim3 grid((g.N1 + 255) / 256, g.N2, g.N3);
+    if (float_faces) hipLaunchKernelGGL(k_vorticity<float>, grid, dim3(256), 0, st, g, F, K, o1, o2, o3);
+    else hipLaunchKernelGGL(k_vorticity<double>, grid, dim3(256), 0, st, g, F, K, o1, o2, o3);
+    return hipGetLastError();
+}
+
+hipError_t launch_average(hipStream_t st, const Geo& g, const Fields& F, const Scalars* S, double* a1, double* a2,
+                          double* a3) {
+    hipLaunchKernelGGL(k_average, dim3((g.N1 + 255) / 256, g.N2, g.N3), dim3(256), 0, st, g, F, S, a1, a2, a3);
     return hipGetLastError();
 }
 

@@ -34,6 +34,10 @@ hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
 hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan, int zero);   // zero: non-periodic exchange
 hipError_t launch_zero_gradient(hipStream_t st, const Geo& g, const GhostPlan& plan);
+hipError_t launch_vorticity(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, bool float_faces,
+                            double* o1, double* o2, double* o3);
+hipError_t launch_average(hipStream_t st, const Geo& g, const Fields& F, const Scalars* S, double* a1, double* a2,
+                          double* a3);
 struct BcFields { double* f[5]; };   // u1, u2, u3, ro, t
 hipError_t launch_boundary(hipStream_t st, const Geo& g, const BcFields& F, int axis, int periodic, double t0);
 hipError_t launch_plane(hipStream_t st, const Geo& g, double* field, int axis, int q, double* buf, int to_buf);

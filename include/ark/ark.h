@@ -305,6 +305,23 @@ ark_status ark_set_closure(ark_ctx* ctx, ark_mlp* mlp, double delta);
  * t0, zero pressure.  Needs even N.  TIME and nStep restart at 0. */
 ark_status ark_init_velocity(ark_ctx* ctx, const double* u);
 
+/* The reference's output path (source/main.cpp:2577-2891), for one rank.
+ * ark_write_paraview replaces WriteDataParaView (2577-2745): the file
+ * <dir_path>out_<nStep>.vtk, binary legacy VTK on the rectilinear grid of the
+ * N+1 nodes per axis ((i-1) dx), with the level-n velocity (U), density (PC)
+ * and the vorticity from the face velocities (R, the reference's expression,
+ * 2712-2733) of every interior cell; need_swap writes big-endian as the
+ * reference's swap8 with needSwap = true (120, 2975-2988).
+ * ark_average_values replaces AverageValues (2878-2891): averageU += u * dt
+ * on the device, the accumulators zero at the first call.
+ * ark_write_average_values replaces WriteAverageValues (2747-2849): the file
+ * <dir_path>average.vtk with averageU / (TIME - start_time).
+ * The files are byte for byte the reference's for the same state; they are
+ * created or truncated. */
+ark_status ark_write_paraview(ark_ctx* ctx, const char* dir_path, int need_swap);
+ark_status ark_average_values(ark_ctx* ctx);
+ark_status ark_write_average_values(ark_ctx* ctx, const char* dir_path, double start_time, int need_swap);
+
 /* Bytes of device memory held by the context. */
 ark_status ark_device_bytes(const ark_ctx* ctx, int64_t* bytes);
 

@@ -30,6 +30,12 @@
 //           branch is live; arrays Input() allocates and InitializeData() leaves
 //           unset are zero-filled first, and Con := nCon as above.
 // Options (before the mode): -nu V  -cfl V  -c V  -quiet
+//   channel only:  -vtk      the reference's WriteDataParaView() at each dump step
+//                            (OUTDIR/out_<step>.vtk, source/main.cpp:2577-2745)
+//                  -avg S    AverageValues() after every step from step S on, the
+//                            start time taken at step S as the reference's main
+//                            does (source/main.cpp:93-96), and WriteAverageValues()
+//                            at the end (OUTDIR/average.vtk, 2747-2849)
 #include <mpi.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -321,6 +327,8 @@ int usage() {
 int main(int argc, char** argv) {
     MPI_Init(&argc, &argv);
     double nu = 1.0 / 1600.0, cfl = 0.3, c = 10.0;
+    bool want_vtk = false;
+    int avg_from = -1;
     int a = 1;
     while (a < argc && argv[a][0] == '-') {
         std::string o(argv[a]);
@@ -328,6 +336,8 @@ int main(int argc, char** argv) {
         else if (o == "-cfl" && a + 1 < argc) cfl = strtod(argv[++a], nullptr);
         else if (o == "-c" && a + 1 < argc) c = strtod(argv[++a], nullptr);
         else if (o == "-quiet") g_quiet = true;
+        else if (o == "-vtk") want_vtk = true;
+        else if (o == "-avg" && a + 1 < argc) avg_from = atoi(argv[++a]);
         else { MPI_Finalize(); return usage(); }
         ++a;
     }
@@ -426,6 +436,9 @@ int main(int argc, char** argv) {
             tCon->ref()[e] = tnCon->ref()[e];
         }
         const int N1 = n1 - 1, N2 = n2 - 1, N3 = n3 - 1;
+        static std::string vtk_dir;
+        vtk_dir = out + "/";
+        dirPath = vtk_dir.c_str();
         auto dump_if = [&](int s) {
             for (int d : dumps)
                 if (d == s) {
@@ -433,8 +446,14 @@ int main(int argc, char** argv) {
                     std::string cmd = "mkdir -p " + dir;
                     if (system(cmd.c_str()) != 0) MPI_Abort(MPI_COMM_WORLD, 2);
                     dump_local(dir, N1, N2, N3);
+                    if (want_vtk) WriteDataParaView();
                 }
         };
+        for (int k = 0; k <= n3; ++k)
+            for (int j = 0; j <= n2 + 1; ++j)
+                for (int i = 0; i <= n1; ++i)
+                    averageU1->elem(i, j, k) = averageU2->elem(i, j, k) = averageU3->elem(i, j, k) = 0.0;
+        double start_time = 0.0;
         std::string fn = out + "/series.txt";
         std::string mk = "mkdir -p " + out;
         if (system(mk.c_str()) != 0) MPI_Abort(MPI_COMM_WORLD, 2);
@@ -445,10 +464,15 @@ int main(int argc, char** argv) {
         dump_if(0);
         for (int s = 1; s <= steps; ++s) {
             one_step();
+            if (avg_from >= 0 && s >= avg_from) {
+                if (s == avg_from) start_time = TIME;
+                AverageValues();
+            }
             fprintf(series, "%d %a %a %a %a\n", nStep, dt, TIME, energy_like_reference(), mass_sum());
             dump_if(s);
         }
         fclose(series);
+        if (avg_from >= 0) WriteAverageValues(start_time);
     } else if (mode == "phase" || mode == "run") {
         if (nproc != 1) { if (rank == 0) fprintf(stderr, "%s mode needs exactly one rank\n", mode.c_str()); MPI_Abort(MPI_COMM_WORLD, 2); }
         std::string name, in, out;

@@ -28,8 +28,10 @@ Fixtures written:
       (glibc MALLOC_PERTURB_=255, tcache off): with MPI_PROC_NULL neighbours
       the reference copies those uninitialised buffers into its ghost planes.
       Two more runs with other heap contents show which arrays depend on them
-      (recorded as "buffer_dependence").  ``python make_fixtures.py channel``
-      writes only this file.
+      (recorded as "buffer_dependence").  Also the SHA-256 of the reference's
+      own output files for the channel (WriteDataParaView after 10 and 20
+      steps, WriteAverageValues of the averages from step 5 to 20).
+      ``python make_fixtures.py channel`` writes only this file.
   tgv_digests.json
       per-array SHA-256 and norms of the 16^3 x 10-step and 64^3 x 200-step
       Taylor-Green runs (1 rank, full arrays), their per-step dt, KE and mass
@@ -216,6 +218,19 @@ def channel_fixture(path: str, work: str) -> None:
                 dep[n]["whole_array"] = False
             if not np.array_equal(x[interior].view(np.uint64), y[interior].view(np.uint64)):
                 dep[n]["interior"] = False
+    # the reference's own output files: WriteDataParaView at steps 10 and 20,
+    # AverageValues from step 5 on and WriteAverageValues after step 20
+    vd = os.path.join(work, "channel_vtk")
+    shutil.rmtree(vd, ignore_errors=True)
+    os.makedirs(vd)
+    env = dict(os.environ)
+    env.update(CHANNEL_ZERO_ENV)
+    subprocess.run([MPIRUN, "-n", "1", HARNESS, "-vtk", "-avg", "5", "channel", "20", vd, "10", "20"], check=True,
+                   capture_output=True, text=True, env=env)
+    out["vtk"] = {"command": "ark_ref_harness -vtk -avg 5 channel 20 OUT 10 20", "files": {}}
+    for fn in ("out_10.vtk", "out_20.vtk", "average.vtk"):
+        data = open(os.path.join(vd, fn), "rb").read()
+        out["vtk"]["files"][fn] = {"sha256": hashlib.sha256(data).hexdigest(), "bytes": len(data)}
     out["buffer_dependence"] = {"independent_whole_array": sorted(n for n, v in dep.items() if v["whole_array"]),
                                 "independent_interior": sorted(n for n, v in dep.items() if v["interior"]),
                                 "dependent_interior": sorted(n for n, v in dep.items() if not v["interior"])}

@@ -96,3 +96,28 @@ def test_solid_cells_on_a_periodic_box_vs_oracle():
         got = s.download_all()
         bad = [describe_mismatch(n, got[n], ref[n]) for n in FIELDS if not bits_equal(got[n], ref[n])]
         assert not bad, bad
+
+
+def test_channel_output_files_match_reference(tmp_path):
+    """The reference's output path, byte for byte: WriteDataParaView after 10
+    and 20 steps (velocity, density, vorticity from the faces, big-endian)
+    and WriteAverageValues of the running averages from step 5 to step 20."""
+    files = json.load(open(os.path.join(GOLDEN, "channel_digests.json")))["vtk"]["files"]
+    st = OracleState.channel()
+    with channel_solver(st) as s:
+        start = None
+        for n in range(1, 21):
+            s.step_phases(1)
+            if n >= 5:
+                if n == 5:
+                    start = s.diagnostics()["time"]
+                s.average_values()
+            if n in (10, 20):
+                path = s.write_paraview(str(tmp_path))
+                assert os.path.basename(path) == f"out_{n}.vtk"
+                data = open(path, "rb").read()
+                assert len(data) == files[f"out_{n}.vtk"]["bytes"]
+                assert hashlib.sha256(data).hexdigest() == files[f"out_{n}.vtk"]["sha256"], n
+        data = open(s.write_average_values(str(tmp_path), start), "rb").read()
+        assert len(data) == files["average.vtk"]["bytes"]
+        assert hashlib.sha256(data).hexdigest() == files["average.vtk"]["sha256"]
