@@ -54,7 +54,7 @@ EXPORTED_SYMBOLS = [
     "ark_mlp_param_count", "ark_mlp_create", "ark_mlp_destroy", "ark_mlp_last_error", "ark_mlp_set_params",
     "ark_mlp_get_params", "ark_mlp_set_normalization", "ark_mlp_set_data", "ark_mlp_gradient", "ark_mlp_train",
     "ark_mlp_predict", "ark_set_closure", "ark_init_velocity",
-    "ark_write_paraview", "ark_average_values", "ark_write_average_values",
+    "ark_write_paraview", "ark_average_values", "ark_write_average_values", "ark_init_channel",
 ]
 
 
@@ -165,6 +165,7 @@ def lib() -> ctypes.CDLL:
     L.ark_set_closure.argtypes = [vp, vp, ctypes.c_double]
     L.ark_write_paraview.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.ark_average_values.argtypes = [vp]
+    L.ark_init_channel.argtypes = [vp, ctypes.c_double]
     L.ark_write_average_values.argtypes = [vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int]
     L.ark_init_velocity.argtypes = [vp, dp]
     for fn in EXPORTED_SYMBOLS:
@@ -329,6 +330,22 @@ class Solver:
         self._check(self._L.ark_write_average_values(self._h, dir_path.encode(), float(start_time),
                                                      1 if need_swap else 0))
         return os.path.join(dir_path, "average.vtk")
+
+    def init_channel(self, u10: float = 1.0) -> None:
+        """The reference's own problem (InitializeData, source/main.cpp:280-372)."""
+        self._check(self._L.ark_init_channel(self._h, float(u10)))
+
+    @classmethod
+    def channel(cls, n1_g: int = 201, n2_g: int = 61, n3_g: int = 2, x1_e: float = 20.0, x2_n: float = 6.0,
+                x3_t: float = 0.1, u10: float = 1.0, VIS: float = 1.0 / 1000.0, CFL: float = 0.2,
+                sound: float = 10.0, device: int = 0) -> "Solver":
+        """A context holding the reference's own problem; the defaults are its
+        Input() values (source/main.cpp:118-195), periods={0,0,0}."""
+        dx = ((x1_e - 0.) / (n1_g - 1), (x2_n - 0.) / (n2_g - 1), (x3_t - 0.) / (n3_g - 1))
+        s = cls(n1_g - 1, n2_g - 1, n3_g - 1, dx=dx, CFL=CFL, VIS=VIS, sound=sound, device=device,
+                periodic=(0, 0, 0))
+        s.init_channel(u10)
+        return s
 
     def init_tgv(self, U0: float = 1.0) -> None:
         self._check(self._L.ark_init_tgv(self._h, float(U0)))

@@ -636,6 +636,71 @@ ark_status ark_diagnostics(ark_ctx* c, ark_diag* out) {
     return ARK_OK;
 }
 
+// The reference's own problem, InitializeData (source/main.cpp:280-372) on
+// one rank (coords 0) of Input's set-up (118-195): node coordinates by running
+// sums from -dx, u1 = u10 above y = 1 (cells, x- and z-faces at the cell
+// centre's y, y-faces at the node's y) and 0 below, u2 = u3 = p = 0, rho = ro0,
+// t = t0 in the cells (the faces' t stays 0: the reference never sets it),
+// solid cells where x <= 5 and y <= 1 and the two ghost rows in y (walls),
+// Con := nCon; every other element 0.
+ark_status ark_init_channel(ark_ctx* c, double u10) {
+    if (!c) return ARK_ERR_INVALID;
+    const int n1 = c->geo.N1 + 1, n2 = c->geo.N2 + 1, n3 = c->geo.N3 + 1;
+    const double dx1 = c->prm.dx1, dx2 = c->prm.dx2;
+    std::vector<double> x1(n1 + 2), x2(n2 + 2);
+    x1[0] = 0. + ((n1 - 1) * dx1 * 0) - dx1;
+    x2[0] = 0. + ((n2 - 1) * dx2 * 0) - dx2;
+    for (int i = 0; i < n1 + 1; ++i) x1[i + 1] = x1[i] + dx1;
+    for (int j = 0; j < n2 + 1; ++j) x2[j + 1] = x2[j] + dx2;
+    const size_t e1c = n1 + 1, e2c = n2 + 1, e3c = n3 + 1, e1f = n1 + 2, e2f = n2 + 2, e3f = n3 + 2;
+    auto cidx = [&](int i, int j, int k) { return (size_t)k * e1c * e2c + (size_t)j * e1c + i; };
+    auto fidx = [&](int i, int j, int k) { return (size_t)k * e1f * e2f + (size_t)j * e1f + i; };
+    std::vector<std::vector<double>> h(F_NUM);
+    for (int f = 0; f < F_NUM; ++f) h[f].assign(f >= F_ro1 ? e1f * e2f * e3f : e1c * e2c * e3c, 0.0);
+    const double ro0_g = c->prm.ro0, t0 = c->prm.t0;
+    for (int i = 0; i <= n1; ++i)
+        for (int j = 0; j <= n2; ++j)
+            for (int k = 0; k <= n3; ++k) {
+                const double y = 0.5 * (x2[j] + x2[j + 1]);
+                const size_t q = cidx(i, j, k);
+                h[F_u1nCon][q] = y <= 1 ? 0. : u10;
+                h[F_ronCon][q] = ro0_g;
+                h[F_tnCon][q] = t0;
+            }
+    for (int i = 1; i <= n1; ++i)
+        for (int j = 1; j <= n2; ++j)
+            for (int k = 1; k <= n3; ++k) {
+                const double y = x2[j], y_c = 0.5 * (x2[j] + x2[j + 1]);
+                const size_t q = fidx(i, j, k);
+                h[F_u11][q] = y_c <= 1 ? 0. : u10;
+                h[F_u12][q] = y <= 1 ? 0. : u10;
+                h[F_u13][q] = y_c <= 1 ? 0. : u10;
+                h[F_ro1][q] = h[F_ro2][q] = h[F_ro3][q] = ro0_g;
+            }
+    for (int i = 0; i < n1 + 1; ++i)
+        for (int j = 0; j < n2 + 1; ++j)
+            for (int k = 0; k < n3 + 1; ++k) {
+                const double x = 0.5 * (x1[i] + x1[i + 1]), y = 0.5 * (x2[j] + x2[j + 1]);
+                h[F_condition][cidx(i, j, k)] = (x <= 5 && y <= 1) ? 1 : 0;
+            }
+    for (int i = 0; i < n1 + 1; ++i)
+        for (int k = 0; k < n3 + 1; ++k) h[F_condition][cidx(i, 0, k)] = h[F_condition][cidx(i, n2, k)] = 1;
+    h[F_roCon] = h[F_ronCon];
+    h[F_u1Con] = h[F_u1nCon];
+    h[F_u2Con] = h[F_u2nCon];
+    h[F_u3Con] = h[F_u3nCon];
+    h[F_tCon] = h[F_tnCon];
+    ark_status s = ARK_OK;
+    for (int f = 0; f < F_NUM && s == ARK_OK; ++f) {
+        const int e = f >= F_ro1 ? 3 : 2;
+        s = ark_upload(c, f, h[f].data(), c->geo.N1 + e, c->geo.N2 + e, c->geo.N3 + e);
+    }
+    if (s == ARK_OK) s = ark_finish_init(c);
+    c->has_solid = true;
+    c->rho_consistent = false;
+    return s;
+}
+
 ark_status ark_init_tgv(ark_ctx* c, double U0) {
     if (!c) return ARK_ERR_INVALID;
     const int N[3] = {c->geo.N1, c->geo.N2, c->geo.N3};

@@ -23,6 +23,10 @@ Printed: ONE JSON line.  Beyond the common fields it carries
                 dissipation series), the closure network trained on its
                 filtered snapshots, and 128^3 runs with and without it compared
                 with the filtered 512^3 field
+  reference_channel
+                the reference's own problem (its 200x60x1 channel with a step,
+                inflow/outflow) on the FP64 phase path, against the reference
+                on one host core
   hbm_ceiling   what plain copy / read / write kernels reach on this GPU
   cpu_baseline  the unmodified reference (oracle/_ref, built from its own
                 sources) under mpirun on this host's cores, bounded sample
@@ -277,6 +281,44 @@ def cpu_baseline(N, steps, ranks):
     }
 
 
+def reference_channel(steps, cpu_steps):
+    """The reference's own problem (its Input()/InitializeData(): 200x60x1
+    cells, backward-facing step, inflow/outflow, periods={0,0,0}) on the FP64
+    phase path, bitwise equal to the reference, against the unmodified
+    reference on one host core (the same harness, step loop only)."""
+    import arkamd
+    out = {"workload": "reference channel 200x60x1 (its Input/InitializeData), FP64, bitwise reference",
+           "steps": steps}
+    with arkamd.Solver.channel() as s:
+        s.step(10)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        d = s.diagnostics(allow_nonfinite=True)
+    cells = 200 * 60 * 1
+    out.update({"gpu_ms_per_step": round(el * 1e3 / steps, 4), "gpu_Mcell_steps_per_s": round(cells * steps / el / 1e6, 2),
+                "nonfinite": d["nonfinite"], "time": d["time"]})
+    harness = os.path.join(REPO, "oracle", "_ref", "ark_ref_harness")
+    mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
+    if cpu_steps > 0 and os.path.exists(harness) and os.path.exists(mpirun):
+        tmp = os.path.join("/tmp", f"ark_channel_{os.getpid()}")
+        try:
+            res = subprocess.run([mpirun, "-n", "1", harness, "channel", str(cpu_steps), tmp], capture_output=True,
+                                 text=True, timeout=300, check=True).stdout
+            rec = json.loads([l for l in res.splitlines() if l.startswith("{")][-1])
+            ms = rec["loop_seconds"] * 1e3 / cpu_steps
+            out["cpu_reference"] = {"ms_per_step": round(ms, 3), "cores": 1, "steps": cpu_steps,
+                                    "Mcell_steps_per_s": round(rec["mcell_steps_per_s"], 3)}
+            out["speedup_vs_reference"] = round(ms / out["gpu_ms_per_step"], 1)
+        except Exception as e:  # reported, never required
+            out["cpu_reference"] = {"error": str(e)[:200]}
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def hbm_ceiling():
     """Best bandwidths of plain streaming kernels on this GPU (copy, read-only,
     write-only over 4 GiB; ark-cpp-mpi_amd/bin/ark_hbm_probe), the practical
@@ -314,6 +356,7 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--accuracy-steps", type=int, default=500)
+    ap.add_argument("--channel-steps", type=int, default=2000, help="steps of the reference's own channel (0: skip)")
     ap.add_argument("--closure-size", type=int, default=512, help="fine grid of the closure experiment (0: skip)")
     ap.add_argument("--closure-steps", type=int, default=2000)
     ap.add_argument("--closure-train-steps", type=int, default=20000)
@@ -381,6 +424,8 @@ def main():
         line["phase_kernels_fp64"] = phase_kernels(N, args.phase_steps)
     if args.batch_boxes > 0 and not args.only:
         line["batched"] = batched(args.batch_boxes, args.batch_steps)
+    if args.channel_steps > 0 and not args.only:
+        line["reference_channel"] = reference_channel(args.channel_steps, 0 if args.no_cpu_baseline else 200)
     if args.closure_size > 0 and not args.only:
         line["forced_turbulence_closure"] = closure_experiment(args.closure_size, args.closure_steps, 4,
                                                                args.closure_train_steps, 65536)

@@ -169,6 +169,14 @@ ark_status ark_diagnostics(ark_ctx* ctx, ark_diag* out);
  * reference's InitializeData() (source/main.cpp:280-372) for the periodic
  * box; Con := nCon, ghosts are periodic copies, f and sigma are zero. */
 ark_status ark_init_tgv(ark_ctx* ctx, double U0);
+/* The reference's own problem: its InitializeData() (source/main.cpp:280-372)
+ * on one rank of its Input() set-up (118-195), for a context with N+1 nodes
+ * per axis and the grid steps Input implies (20/200, 6/60, 0.1/1 for the
+ * reference's 201x61x2 nodes) and periodic = {0,0,0}: the channel with a
+ * backward-facing step (solid where x <= 5 and y <= 1), walls as solid ghost
+ * rows in y, u1 = u10 above y = 1; every element InitializeData leaves unset
+ * is 0; Con := nCon; TIME and nStep 0. */
+ark_status ark_init_channel(ark_ctx* ctx, double u10);
 
 /* Block until all work on the context's stream is done. */
 ark_status ark_synchronize(ark_ctx* ctx);

@@ -462,8 +462,11 @@ int main(int argc, char** argv) {
         fprintf(series, "# step dt TIME ke_reference mass\n");
         fprintf(series, "0 %a %a %a %a\n", 0.0, 0.0, energy_like_reference(), mass_sum());
         dump_if(0);
+        double loop_time = 0.0;
         for (int s = 1; s <= steps; ++s) {
+            double t_start = MPI_Wtime();
             one_step();
+            loop_time += MPI_Wtime() - t_start;
             if (avg_from >= 0 && s >= avg_from) {
                 if (s == avg_from) start_time = TIME;
                 AverageValues();
@@ -473,6 +476,9 @@ int main(int argc, char** argv) {
         }
         fclose(series);
         if (avg_from >= 0) WriteAverageValues(start_time);
+        printf("{\"ranks\": 1, \"grid\": [%d, %d, %d], \"steps\": %d, \"loop_seconds\": %.6f, "
+               "\"mcell_steps_per_s\": %.6f}\n",
+               N1, N2, N3, steps, loop_time, (double)N1 * N2 * N3 * steps / (loop_time > 0 ? loop_time : 1e-30) / 1e6);
     } else if (mode == "phase" || mode == "run") {
         if (nproc != 1) { if (rank == 0) fprintf(stderr, "%s mode needs exactly one rank\n", mode.c_str()); MPI_Abort(MPI_COMM_WORLD, 2); }
         std::string name, in, out;

@@ -121,3 +121,16 @@ def test_channel_output_files_match_reference(tmp_path):
         data = open(s.write_average_values(str(tmp_path), start), "rb").read()
         assert len(data) == files["average.vtk"]["bytes"]
         assert hashlib.sha256(data).hexdigest() == files["average.vtk"]["sha256"]
+
+
+def test_library_channel_init_matches_reference():
+    """ark_init_channel (the product's restatement of InitializeData) gives the
+    reference's initial arrays bit for bit, and runs to its step-10 state."""
+    dig = json.load(open(os.path.join(GOLDEN, "channel_digests.json")))
+    with arkamd.Solver.channel() as s:
+        for k in (0, 10):
+            if k:
+                s.step(k)
+            got = s.download_all()
+            bad = [n for n in FIELDS if sha(got[n]) != dig["states"][str(k)][n]["sha256"]]
+            assert not bad, (k, bad)
