@@ -7,7 +7,7 @@
 // cells, the stresses, the forces, the face densities -- stays on chip.
 //
 // Blocking ("2.5-D"): a block owns TX x TY columns of cells and marches a
-// chunk of LZ planes in z.  Its threads cover the owned columns plus a ring
+// chunk of planes in z (a host table sizes the chunks).  Its threads cover the owned columns plus a ring
 // of one cell (RX x RY threads), because the new faces on the tile's edge
 // need the half-level state of the neighbouring column, which the ring
 // threads recompute.  Per plane the block computes, in order:
@@ -30,10 +30,11 @@
 
 namespace arkd {
 
-// 62x6 owned columns plus the ring: 64x8 = 512 threads, eight full
+// 30x14 owned columns plus the ring: 32x16 = 512 threads, eight full
 // wavefronts, two per SIMD (256 VGPRs).  A multiple of four wavefronts keeps
-// the four SIMDs of a CU equally loaded.
-constexpr int TX = 62, TY = 6;
+// the four SIMDs of a CU equally loaded.  30x14 covers a 256^2 plane with
+// 1.34 threads per owned column (62x6: 1.68).
+constexpr int TX = 30, TY = 14;
 constexpr int RX = TX + 2, RY = TY + 2;
 constexpr int NTHR = RX * RY;
 
@@ -51,7 +52,6 @@ struct FusedArgs {
     unsigned long long fb;     // bytes between the fields of a group
     const Scalars* S;          // S->dt is this step's dt
     unsigned long long* dt_next;
-    int LZ;
 };
 
 __device__ __forceinline__ double gld(const char* grp, unsigned long long fb, int v, int idx) {
@@ -184,17 +184,25 @@ union LdsSlot {
     LdsFlux f;
 };
 
+// One workgroup per chunk of a tile column.  units[u] = (tile column x
+// fastest then y, first plane, end plane) lists the chunks in rounds (round q
+// holds chunk q of every column), so the workgroups that run together work at
+// the same height and share their rings through L2; die (XCD) b % 8 takes a
+// contiguous run of chunks, so the columns in flight on one die are
+// neighbours.
 template <bool Quirk>
-__global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a) {
+__global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __restrict__ units, int nbx) {
     __shared__ LdsSlot lds[NTHR];
     const Geo& g = a.g;
     const Consts& K = a.K;
     const double dt = a.S->dt;
     const int tid = threadIdx.x;
     const int tx = tid % RX, ty = tid / RX;
-    const TileId tile = die_major_tile((g.N1 + TX - 1) / TX, (g.N2 + TY - 1) / TY, (g.N3 + a.LZ - 1) / a.LZ);
-    const int i0 = tile.x * TX + 1, j0 = tile.y * TY + 1, k0 = tile.z * a.LZ + 1;
-    const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = min(k0 + a.LZ, g.N3 + 1);
+    const int U = gridDim.x, blk = blockIdx.x;
+    const int die = blk % kDies, local = blk / kDies;
+    const int4 w = units[die * (U / kDies) + min(die, U % kDies) + local];
+    const int i0 = (w.x % nbx) * TX + 1, j0 = (w.x / nbx) * TY + 1, k0 = w.y;
+    const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = w.z;
     // logical cell of this thread (clamped into the region so every access stays in bounds)
     const int i = min(i0 - 1 + tx, iend), j = min(j0 - 1 + ty, jend);
     const bool in_region = (i0 - 1 + tx <= iend) && (j0 - 1 + ty <= jend);
@@ -404,14 +412,8 @@ namespace {
 
 using namespace arkd;
 
-struct FusedLaunch {
-    ark_ctx* c;
-    int LZ;
-};
-
 hipError_t run_fused(void* p) {
-    FusedLaunch* fl = (FusedLaunch*)p;
-    ark_ctx* c = fl->c;
+    ark_ctx* c = (ark_ctx*)p;
     FusedArgs a;
     a.g = c->geo;
     a.K = c->K;
@@ -427,22 +429,27 @@ hipError_t run_fused(void* p) {
     a.fb = c->field_bytes;
     a.S = c->S;
     a.dt_next = &c->S->dt_bits;
-    a.LZ = fl->LZ;
-    dim3 grid(((c->geo.N1 + TX - 1) / TX) * ((c->geo.N2 + TY - 1) / TY) * ((c->geo.N3 + fl->LZ - 1) / fl->LZ));
+    const dim3 grid(c->fused_nunits);
+    const int4* units = (const int4*)c->fused_units;
+    const int nbx = (c->geo.N1 + TX - 1) / TX;
     if (c->prm.scheme == ARK_SCHEME_REFERENCE)
-        hipLaunchKernelGGL(k_fused_step<true>, grid, dim3(NTHR), 0, c->stream, a);
+        hipLaunchKernelGGL(k_fused_step<true>, grid, dim3(NTHR), 0, c->stream, a, units, nbx);
     else
-        hipLaunchKernelGGL(k_fused_step<false>, grid, dim3(NTHR), 0, c->stream, a);
+        hipLaunchKernelGGL(k_fused_step<false>, grid, dim3(NTHR), 0, c->stream, a, units, nbx);
     return hipGetLastError();
 }
 
-// Planes per block: enough blocks to fill 256 CUs several times over, long
-// enough chunks that the two extra half-level planes per chunk stay cheap.
-int choose_lz(const Geo& g) {
-    long long cols = (long long)((g.N1 + TX - 1) / TX) * ((g.N2 + TY - 1) / TY);
-    int lz = 64;
-    while (lz > 8 && cols * ((g.N3 + lz - 1) / lz) < 2048) lz /= 2;
-    return lz < g.N3 ? lz : g.N3;
+// The chunk table of the FP64 fused step, built once per context for the
+// workgroups the chip holds at once.
+ark_status ensure_fused_units(ark_ctx* c) {
+    if (c->fused_units) return ARK_OK;
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    const void* fn = c->prm.scheme == ARK_SCHEME_REFERENCE ? (const void*)k_fused_step<true>
+                                                          : (const void*)k_fused_step<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHR, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return ark_upload_chunks(c, ark_chunk_table(c->geo, TX, TY, cus * per_cu), &c->fused_units, &c->fused_nunits);
 }
 
 }  // namespace
@@ -456,11 +463,10 @@ bool ark_fused_eligible(const ark_ctx* c) {
 // the double-buffered persistent fields.  The kernel leaves the next step's
 // dt as the running MIN in S->dt_bits (dt_pending).
 static ark_status fused_steps(ark_ctx* c, int nsteps) {
-    ark_status s = ARK_OK;
-    FusedLaunch fl{c, choose_lz(c->geo)};
+    ark_status s = ensure_fused_units(c);
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
-        if (s == ARK_OK) s = c->launch("fused_step", run_fused, &fl);
+        if (s == ARK_OK) s = c->launch("fused_step", run_fused, c);
         if (s != ARK_OK) break;
         for (int g = 0; g < 4; ++g) ark_swap_group(c, g);
         c->rho_stored = false;

@@ -54,6 +54,8 @@ struct ark_ctx {
     double* average = nullptr;         // AverageValues' three accumulators (cell fields, common index map)
     void* mixed_units = nullptr;       // chunk table of the mixed-precision step (int4 per chunk)
     int mixed_nunits = 0;
+    void* fused_units = nullptr;       // chunk table of the FP64 fused step
+    int fused_nunits = 0;
     int diag_blocks = 0;
     double host_dt = 0.0;
     bool has_solid = false;
@@ -105,6 +107,13 @@ void ark_swap_group(ark_ctx* c, int g);
 ark_status ark_mixed_steps(ark_ctx* c, int nsteps);
 ark_status ark_mixed_face_to_host_layout(ark_ctx* c, int f, double* staging);
 ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* staging);
+
+// Chunk tables of the z-marching fused steps (ark_mixed.hip): (tile column
+// x fastest then y, first plane, end plane) per chunk, for tiles of TX x TY
+// owned columns and W workgroups resident at once; and their upload into a
+// device allocation owned by c.
+std::vector<int4> ark_chunk_table(const arkd::Geo& g, int TX, int TY, int W);
+ark_status ark_upload_chunks(ark_ctx* c, const std::vector<int4>& u, void** dst, int* n);
 
 // The fused step (ark_fused.hip).
 ark_status ark_step_fused(ark_ctx* c, int nsteps);

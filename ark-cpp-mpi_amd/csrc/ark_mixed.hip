@@ -707,15 +707,66 @@ void launch_single(bool quirk, int units, const int4* table, hipStream_t st, con
     else hipLaunchKernelGGL((mixed::k_fused_mixed<false, S>), dim3(units), dim3(S::NTHR), 0, st, a, table, nbx);
 }
 
-// Chunks of the single-box step.  With T tile columns, N3 planes and W
-// workgroups resident at once (CUs x workgroups per CU), U chunks give each
-// column U/T or U/T + 1 near-equal chunks (+2 ring planes each).  U is taken
-// among the multiples of W (or all T*N3 planes, if fewer) by the finishing
-// time of in-order dispatch onto each die's share of the workgroups, in
-// units of plane-steps.
 template <class S>
 std::vector<int4> mixed_units(const Geo& g, int W) {
-    const int nbx = (g.N1 + S::TX - 1) / S::TX, nby = (g.N2 + S::TY - 1) / S::TY;
+    return ark_chunk_table(g, S::TX, S::TY, W);
+}
+
+template <class S>
+void launch_batch(bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxes, const int* start, int n) {
+    if (quirk) hipLaunchKernelGGL((mixed::k_fused_mixed_batch<true, S>), grid, dim3(S::NTHR), 0, st, boxes, start, n);
+    else hipLaunchKernelGGL((mixed::k_fused_mixed_batch<false, S>), grid, dim3(S::NTHR), 0, st, boxes, start, n);
+}
+
+template <class S>
+int resident_blocks(bool quirk) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    const void* fn = quirk ? (const void*)mixed::k_fused_mixed<true, S> : (const void*)mixed::k_fused_mixed<false, S>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, S::NTHR, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return cus * per_cu;
+}
+
+hipError_t launch_shape_batch(int shape, bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxes,
+                              const int* start, int n) {
+    if (shape == 0) launch_batch<mixed::Shape30x6>(quirk, grid, st, boxes, start, n);
+    else if (shape == 1) launch_batch<mixed::Shape30x14>(quirk, grid, st, boxes, start, n);
+    else launch_batch<mixed::Shape62x6>(quirk, grid, st, boxes, start, n);
+    return hipGetLastError();
+}
+
+using SingleShape = mixed::Shape30x14;
+
+hipError_t run_mixed(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    const mixed::Args a = make_args(c, 0, &c->S->dt_bits);
+    launch_single<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE, c->mixed_nunits, (const int4*)c->mixed_units,
+                               c->stream, a);
+    return hipGetLastError();
+}
+
+// The chunk table of the single-box step, built once per context.
+ark_status ensure_units(ark_ctx* c) {
+    if (c->mixed_units) return ARK_OK;
+    const std::vector<int4> u =
+        mixed_units<SingleShape>(c->geo, resident_blocks<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE));
+    return ark_upload_chunks(c, u, &c->mixed_units, &c->mixed_nunits);
+}
+
+int face_mode(int f) { return (f == F_t1 || f == F_t2 || f == F_t3) ? 1 : 0; }
+
+}  // namespace
+
+// Chunks of a z-marching step with TX x TY owned columns per tile.  With T
+// tile columns, N3 planes and W workgroups resident at once (CUs x
+// workgroups per CU), U chunks give each column U/T or U/T + 1 near-equal
+// chunks (+2 ring planes each).  U is taken among the multiples of W (or all
+// T*N3 planes, if fewer) by the finishing time of in-order dispatch onto each
+// die's share of the workgroups, in units of plane-steps.  The table lists
+// the chunks in rounds: round q holds chunk q of every tile column.
+std::vector<int4> ark_chunk_table(const Geo& g, int TX, int TY, int W) {
+    const int nbx = (g.N1 + TX - 1) / TX, nby = (g.N2 + TY - 1) / TY;
     const long long T = (long long)nbx * nby;
     auto chunks_of = [&](long long U, long long t) { return (int)(U / T + (t < U % T ? 1 : 0)); };
     auto build = [&](long long U) {
@@ -764,45 +815,7 @@ std::vector<int4> mixed_units(const Geo& g, int W) {
     return best;
 }
 
-template <class S>
-void launch_batch(bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxes, const int* start, int n) {
-    if (quirk) hipLaunchKernelGGL((mixed::k_fused_mixed_batch<true, S>), grid, dim3(S::NTHR), 0, st, boxes, start, n);
-    else hipLaunchKernelGGL((mixed::k_fused_mixed_batch<false, S>), grid, dim3(S::NTHR), 0, st, boxes, start, n);
-}
-
-template <class S>
-int resident_blocks(bool quirk) {
-    int dev = 0, cus = 0, per_cu = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-    const void* fn = quirk ? (const void*)mixed::k_fused_mixed<true, S> : (const void*)mixed::k_fused_mixed<false, S>;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, S::NTHR, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    return cus * per_cu;
-}
-
-hipError_t launch_shape_batch(int shape, bool quirk, dim3 grid, hipStream_t st, const mixed::Args* boxes,
-                              const int* start, int n) {
-    if (shape == 0) launch_batch<mixed::Shape30x6>(quirk, grid, st, boxes, start, n);
-    else if (shape == 1) launch_batch<mixed::Shape30x14>(quirk, grid, st, boxes, start, n);
-    else launch_batch<mixed::Shape62x6>(quirk, grid, st, boxes, start, n);
-    return hipGetLastError();
-}
-
-using SingleShape = mixed::Shape30x14;
-
-hipError_t run_mixed(void* p) {
-    ark_ctx* c = (ark_ctx*)p;
-    const mixed::Args a = make_args(c, 0, &c->S->dt_bits);
-    launch_single<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE, c->mixed_nunits, (const int4*)c->mixed_units,
-                               c->stream, a);
-    return hipGetLastError();
-}
-
-// The chunk table of the single-box step, built once per context.
-ark_status ensure_units(ark_ctx* c) {
-    if (c->mixed_units) return ARK_OK;
-    const std::vector<int4> u =
-        mixed_units<SingleShape>(c->geo, resident_blocks<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE));
+ark_status ark_upload_chunks(ark_ctx* c, const std::vector<int4>& u, void** dst, int* n) {
     void* d = nullptr;
     ark_status s = c->hip(hipMalloc(&d, u.size() * sizeof(int4)), "hipMalloc chunk table");
     if (s != ARK_OK) return s;
@@ -810,14 +823,10 @@ ark_status ensure_units(ark_ctx* c) {
     c->bytes += u.size() * sizeof(int4);
     s = c->hip(hipMemcpy(d, u.data(), u.size() * sizeof(int4), hipMemcpyHostToDevice), "chunk table");
     if (s != ARK_OK) return s;
-    c->mixed_units = d;
-    c->mixed_nunits = (int)u.size();
+    *dst = d;
+    *n = (int)u.size();
     return ARK_OK;
 }
-
-int face_mode(int f) { return (f == F_t1 || f == F_t2 || f == F_t3) ? 1 : 0; }
-
-}  // namespace
 
 ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)

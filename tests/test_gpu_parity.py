@@ -145,7 +145,7 @@ def test_tgv_digest(N, steps):
         assert d["nstep"] == steps
 
 
-@pytest.mark.parametrize("N", [(32, 32, 32), (20, 13, 7)])
+@pytest.mark.parametrize("N", [(32, 32, 32), (20, 13, 7), (70, 40, 90)])
 def test_fused_equals_phases(N):
     with arkamd.Solver(*N) as a, arkamd.Solver(*N) as b:
         a.init_tgv()
