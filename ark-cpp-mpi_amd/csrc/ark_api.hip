@@ -141,6 +141,11 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
     K.ds3 = p->dx1 * p->dx2;
     K.dvc = p->dx1 * p->dx2 * p->dx3;
     K.VIS = p->VIS; K.sound = p->sound; K.ro0 = p->ro0; K.t0 = p->t0;
+    // IEEE double division on the host: the correctly rounded reciprocals
+    for (int d = 0; d < 3; ++d) K.rdx[d] = 1.0 / K.dx[d];
+    K.rc2 = 1.0 / (K.sound * K.sound);
+    K.rrs = 1.0 / (K.ro0 * K.sound);
+    K.rdvc = 1.0 / K.dvc;
 
     ark_status s = c->hip(hipSetDevice(p->device), "hipSetDevice");
     if (s == ARK_OK) s = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");

@@ -118,7 +118,24 @@ struct Consts {
     double dx[3];
     double ds1, ds2, ds3, dvc;   // face areas and cell volume (source/main.cpp:455-460)
     double VIS, sound, ro0, t0;
+    // correctly rounded reciprocals of the divisors the fused step uses:
+    // 1/dx[d], 1/(sound*sound), 1/(ro0*sound), 1/dvc
+    double rdx[3], rc2, rrs, rdvc;
 };
+
+// x / b from rb = 1/b (correctly rounded): the IEEE quotient, bit for bit,
+// whenever that is finite and not subnormal.  q0 = x*rb lies within 1.5 ulp
+// of x/b; one residual correction fma(fma(-q, b, x), rb, q) brings it within
+// one ulp, and a second one then yields the correctly rounded quotient
+// (Markstein, IBM J. Res. Develop. 34 (1990) 111-119).  The sign of a zero
+// quotient is that of x*rb, which is the IEEE sign.  An overflowing quotient
+// comes out NaN where IEEE division gives inf.  Five FP64 operations at full
+// rate, against eleven, one of them a quarter-rate reciprocal, for a division.
+__device__ __forceinline__ double div_rn(double x, double b, double rb) {
+    const double q0 = x * rb;
+    const double q1 = fma(fma(-q0, b, x), rb, q0);
+    return copysign(fma(fma(-q1, b, x), rb, q1), q0);
+}
 
 // ---------------------------------------------------------------------------
 // Phase2 cell extrapolation (source/main.cpp:1747-1874 and the X2/X3
@@ -144,24 +161,45 @@ __device__ __forceinline__ void clip(double& v, double vmax, double vmin) {
     if (v < vmin) v = vmin;
 }
 
+// The divisions of the shared expressions below: IEEE division (Rcp false),
+// or div_rn with the correctly rounded reciprocals (the same bits for every
+// finite quotient).  rdt and rdx are 1/dt and 1/dx; 1/(ro0*sound) and
+// 1/(sound*sound) come from K.
+template <bool Rcp>
+struct Divide {
+    double dt, rdt, dx, rdx;
+    const Consts& K;
+    __device__ __forceinline__ double by_dt(double x) const { return Rcp ? div_rn(x, dt, rdt) : x / dt; }
+    __device__ __forceinline__ double by_dx(double x) const { return Rcp ? div_rn(x, dx, rdx) : x / dx; }
+    __device__ __forceinline__ double by_rs(double x) const {
+        return Rcp ? div_rn(x, K.ro0 * K.sound, K.rrs) : x / (K.ro0 * K.sound);
+    }
+    __device__ __forceinline__ double by_c2(double x) const {
+        return Rcp ? div_rn(x, K.sound * K.sound, K.rc2) : x / (K.sound * K.sound);
+    }
+};
+
 // Forward extrapolations of one cell.
-__device__ __forceinline__ Fwd cell_forward(const CellSide& s, double dt, double dx, const Consts& K) {
+template <bool Rcp = false>
+__device__ __forceinline__ Fwd cell_forward(const CellSide& s, double dt, double dx, const Consts& K, double rdt = 0.0,
+                                            double rdx = 0.0) {
+    const Divide<Rcp> D{dt, rdt, dx, rdx, K};
     const double sound = K.sound, ro0_g = K.ro0;
     double p_cn = sound * sound * (s.ro_cn - ro0_g);
     double p_c = sound * sound * (s.ro_c - ro0_g);
-    double r_f = s.un_f + s.p_f / (ro0_g * sound);
-    double r_b = s.un_b + s.p_b / (ro0_g * sound);
-    double r_cn = s.un_cn + p_cn / (ro0_g * sound);
-    double r_c = s.un_c + p_c / (ro0_g * sound);
+    double r_f = s.un_f + D.by_rs(s.p_f);
+    double r_b = s.un_b + D.by_rs(s.p_b);
+    double r_cn = s.un_cn + D.by_rs(p_cn);
+    double r_c = s.un_c + D.by_rs(p_c);
     Fwd o;
     o.r = 2 * r_cn - r_b;
     o.t = 2 * s.t_cn - s.t_b;
     o.a = 2 * s.ua_cn - s.ua_b;
     o.b = 2 * s.ub_cn - s.ub_b;
-    double gr = 2 * (r_cn - r_c) / dt + (s.un_cn + sound) * (r_f - r_b) / dx;
-    double gt = 2 * (s.t_cn - s.t_c) / dt + s.un_cn * (s.t_f - s.t_b) / dx;
-    double ga = 2 * (s.ua_cn - s.ua_c) / dt + s.un_cn * (s.ua_f - s.ua_b) / dx;
-    double gb = 2 * (s.ub_cn - s.ub_c) / dt + s.un_cn * (s.ub_f - s.ub_b) / dx;
+    double gr = D.by_dt(2 * (r_cn - r_c)) + D.by_dx((s.un_cn + sound) * (r_f - r_b));
+    double gt = D.by_dt(2 * (s.t_cn - s.t_c)) + D.by_dx(s.un_cn * (s.t_f - s.t_b));
+    double ga = D.by_dt(2 * (s.ua_cn - s.ua_c)) + D.by_dx(s.un_cn * (s.ua_f - s.ua_b));
+    double gb = D.by_dt(2 * (s.ub_cn - s.ub_c)) + D.by_dx(s.un_cn * (s.ub_f - s.ub_b));
     clip(o.r, max3d(r_f, r_c, r_b) + dt * gr, min3d(r_f, r_c, r_b) + dt * gr);
     clip(o.t, max3d(s.t_f, s.t_c, s.t_b) + dt * gt, min3d(s.t_f, s.t_c, s.t_b) + dt * gt);
     clip(o.a, max3d(s.ua_f, s.ua_c, s.ua_b) + dt * ga, min3d(s.ua_f, s.ua_c, s.ua_b) + dt * ga);
@@ -172,15 +210,17 @@ __device__ __forceinline__ Fwd cell_forward(const CellSide& s, double dt, double
 // Backward extrapolations of one cell.  GqUsesR reproduces the X1/X3 sweeps,
 // whose gq takes its time derivative from r (source/main.cpp:1804, 2321); the
 // X2 sweep uses q (source/main.cpp:2083).
-template <bool GqUsesR>
-__device__ __forceinline__ Bwd cell_backward(const CellSide& s, double dt, double dx, const Consts& K) {
+template <bool GqUsesR, bool Rcp = false>
+__device__ __forceinline__ Bwd cell_backward(const CellSide& s, double dt, double dx, const Consts& K,
+                                             double rdt = 0.0, double rdx = 0.0) {
+    const Divide<Rcp> D{dt, rdt, dx, rdx, K};
     const double sound = K.sound, ro0_g = K.ro0;
     double p_cn = sound * sound * (s.ro_cn - ro0_g);
     double p_c = sound * sound * (s.ro_c - ro0_g);
-    double q_f = s.un_f - s.p_f / (ro0_g * sound);
-    double q_b = s.un_b - s.p_b / (ro0_g * sound);
-    double q_cn = s.un_cn - p_cn / (ro0_g * sound);
-    double q_c = s.un_c - p_c / (ro0_g * sound);
+    double q_f = s.un_f - D.by_rs(s.p_f);
+    double q_b = s.un_b - D.by_rs(s.p_b);
+    double q_cn = s.un_cn - D.by_rs(p_cn);
+    double q_c = s.un_c - D.by_rs(p_c);
     Bwd o;
     o.q = 2 * q_cn - q_f;
     o.t = 2 * s.t_cn - s.t_f;
@@ -188,16 +228,16 @@ __device__ __forceinline__ Bwd cell_backward(const CellSide& s, double dt, doubl
     o.b = 2 * s.ub_cn - s.ub_f;
     double dq;
     if (GqUsesR) {
-        double r_cn = s.un_cn + p_cn / (ro0_g * sound);
-        double r_c = s.un_c + p_c / (ro0_g * sound);
-        dq = 2 * (r_cn - r_c) / dt;
+        double r_cn = s.un_cn + D.by_rs(p_cn);
+        double r_c = s.un_c + D.by_rs(p_c);
+        dq = D.by_dt(2 * (r_cn - r_c));
     } else {
-        dq = 2 * (q_cn - q_c) / dt;
+        dq = D.by_dt(2 * (q_cn - q_c));
     }
-    double gq = dq + (s.un_cn - sound) * (q_f - q_b) / dx;
-    double gt = 2 * (s.t_cn - s.t_c) / dt + s.un_cn * (s.t_f - s.t_b) / dx;
-    double ga = 2 * (s.ua_cn - s.ua_c) / dt + s.un_cn * (s.ua_f - s.ua_b) / dx;
-    double gb = 2 * (s.ub_cn - s.ub_c) / dt + s.un_cn * (s.ub_f - s.ub_b) / dx;
+    double gq = dq + D.by_dx((s.un_cn - sound) * (q_f - q_b));
+    double gt = D.by_dt(2 * (s.t_cn - s.t_c)) + D.by_dx(s.un_cn * (s.t_f - s.t_b));
+    double ga = D.by_dt(2 * (s.ua_cn - s.ua_c)) + D.by_dx(s.un_cn * (s.ua_f - s.ua_b));
+    double gb = D.by_dt(2 * (s.ub_cn - s.ub_c)) + D.by_dx(s.un_cn * (s.ub_f - s.ub_b));
     clip(o.q, max3d(q_f, q_c, q_b) + dt * gq, min3d(q_f, q_c, q_b) + dt * gq);
     clip(o.t, max3d(s.t_f, s.t_c, s.t_b) + dt * gt, min3d(s.t_f, s.t_c, s.t_b) + dt * gt);
     clip(o.a, max3d(s.ua_f, s.ua_c, s.ua_b) + dt * ga, min3d(s.ua_f, s.ua_c, s.ua_b) + dt * ga);
@@ -211,6 +251,7 @@ __device__ __forceinline__ Bwd cell_backward(const CellSide& s, double dt, doubl
 // cells; the divergent case needs the cells' n+1/2 values and the old face.
 struct FaceOut { double un, p, ro, t, a, b; };
 
+template <bool Rcp = false>
 __device__ __forceinline__ FaceOut face_combine(const Fwd& fb, const Bwd& bf, double ucb, double ucf,
                                                 double tn_b, double tn_f, double ta_b, double ta_f,
                                                 double tb_b, double tb_f, double t_old, double a_old,
@@ -220,7 +261,7 @@ __device__ __forceinline__ FaceOut face_combine(const Fwd& fb, const Bwd& bf, do
     double rn = fb.r, qn = bf.q;
     o.un = (rn + qn) / 2;
     o.p = (rn - qn) * sound * ro0_g / 2;
-    o.ro = ro0_g + o.p / (sound * sound);
+    o.ro = ro0_g + (Rcp ? div_rn(o.p, sound * sound, K.rc2) : o.p / (sound * sound));
     if (ucf >= 0 && ucb >= 0) {
         o.t = fb.t; o.a = fb.a; o.b = fb.b;
     } else if (ucf <= 0 && ucb <= 0) {

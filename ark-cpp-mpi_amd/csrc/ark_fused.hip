@@ -20,6 +20,9 @@
 // Flux terms such as rho*u1*u2 are computed once per face and used by both
 // cells that share it; the operands and their association are the
 // reference's, so FP64 results are bitwise equal to the phase-by-phase path.
+// Divisions by the step's constants (dx, dt, sound^2, ro0*sound, dvc) and by
+// a cell's rho*dvc use div_rn with a correctly rounded reciprocal: the same
+// bits as IEEE division for every finite quotient, at half the instructions.
 //
 // Periodic box: cell indices wrap (ghost cell 0 is cell N, N+1 is cell 1,
 // which is what the reference's halo exchange delivers), face N+1 is a real
@@ -100,25 +103,27 @@ __device__ __forceinline__ Flux5 flux_terms(double ro, double u1, double u2, dou
 
 template <int D>
 __device__ __forceinline__ Flux5 old_flux(const Face5& f, const Consts& K) {
-    double ro = K.ro0 + f.p / (K.sound * K.sound);   // the reference's face density, source/main.cpp:1930
+    double ro = K.ro0 + div_rn(f.p, K.sound * K.sound, K.rc2);   // the reference's face density, source/main.cpp:1930
     return flux_terms<D>(ro, f.u1, f.u2, f.u3, f.p, f.t);
 }
 
 // Conservative half-step from the flux differences (Phase1's update) followed
-// by the force half-step (UseForces).
+// by the force half-step (UseForces).  The seven divisions by ro_cn*dvc (the
+// same product as dvc*ro_cn) share one reciprocal.
 __device__ __forceinline__ Cell5 conservative_update(const Cell5& c, const double S[5], const double f[3], double dt,
                                                      const Consts& K) {
     const double dvc = K.dvc;
     Cell5 n;
-    double ro_cn = (c.ro * dvc - 0.5 * dt * S[0]) / dvc;
-    double u1_cn = (c.ro * c.u1 * dvc - 0.5 * dt * S[1]) / (ro_cn * dvc);
-    double u2_cn = (c.ro * c.u2 * dvc - 0.5 * dt * S[2]) / (ro_cn * dvc);
-    double u3_cn = (c.ro * c.u3 * dvc - 0.5 * dt * S[3]) / (ro_cn * dvc);
-    double t_cn = (c.ro * c.t * dvc - 0.5 * dt * S[4]) / (dvc * ro_cn);
+    double ro_cn = div_rn(c.ro * dvc - 0.5 * dt * S[0], dvc, K.rdvc);
+    const double m = ro_cn * dvc, rm = 1.0 / m;
+    double u1_cn = div_rn(c.ro * c.u1 * dvc - 0.5 * dt * S[1], m, rm);
+    double u2_cn = div_rn(c.ro * c.u2 * dvc - 0.5 * dt * S[2], m, rm);
+    double u3_cn = div_rn(c.ro * c.u3 * dvc - 0.5 * dt * S[3], m, rm);
+    double t_cn = div_rn(c.ro * c.t * dvc - 0.5 * dt * S[4], m, rm);
     n.ro = ro_cn;
-    n.u1 = (ro_cn * dvc * u1_cn + 0.5 * dt * f[0]) / (dvc * ro_cn);
-    n.u2 = (ro_cn * dvc * u2_cn + 0.5 * dt * f[1]) / (dvc * ro_cn);
-    n.u3 = (ro_cn * dvc * u3_cn + 0.5 * dt * f[2]) / (dvc * ro_cn);
+    n.u1 = div_rn(ro_cn * dvc * u1_cn + 0.5 * dt * f[0], m, rm);
+    n.u2 = div_rn(ro_cn * dvc * u2_cn + 0.5 * dt * f[1], m, rm);
+    n.u3 = div_rn(ro_cn * dvc * u3_cn + 0.5 * dt * f[2], m, rm);
     n.t = t_cn;
     return n;
 }
@@ -148,8 +153,8 @@ __device__ __forceinline__ FaceOut new_face(const Fwd& fw, const Cell5& hb, cons
                                             const Face5& old, const Consts& K) {
     constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
     auto fu = [](const Face5& f, int m) { return m == 0 ? f.u1 : m == 1 ? f.u2 : f.u3; };
-    return face_combine(fw, bw, ucomp(hb, D), ucomp(hf, D), hb.t, hf.t, ucomp(hb, TA), ucomp(hf, TA), ucomp(hb, TB),
-                        ucomp(hf, TB), old.t, fu(old, TA), fu(old, TB), K);
+    return face_combine<true>(fw, bw, ucomp(hb, D), ucomp(hf, D), hb.t, hf.t, ucomp(hb, TA), ucomp(hf, TA),
+                              ucomp(hb, TB), ucomp(hf, TB), old.t, fu(old, TA), fu(old, TB), K);
 }
 
 template <int D>
@@ -195,7 +200,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
     __shared__ LdsSlot lds[NTHR];
     const Geo& g = a.g;
     const Consts& K = a.K;
-    const double dt = a.S->dt;
+    const double dt = a.S->dt, rdt = 1.0 / dt;
     const int tid = threadIdx.x;
     const int tx = tid % RX, ty = tid / RX;
     const int U = gridDim.x, blk = blockIdx.x;
@@ -224,9 +229,9 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
     double sz_lo[3];
     {
         const Cell5 A_below = load_cell(A, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
-        sz_lo[0] = VIS * (A_cur.u1 - A_below.u1) / K.dx[2];
-        sz_lo[1] = VIS * (A_cur.u2 - A_below.u2) / K.dx[2];
-        sz_lo[2] = VIS * (A_cur.u3 - A_below.u3) / K.dx[2];
+        sz_lo[0] = div_rn(VIS * (A_cur.u1 - A_below.u1), K.dx[2], K.rdx[2]);
+        sz_lo[1] = div_rn(VIS * (A_cur.u2 - A_below.u2), K.dx[2], K.rdx[2]);
+        sz_lo[2] = div_rn(VIS * (A_cur.u3 - A_below.u3), K.dx[2], K.rdx[2]);
     }
     Cell5 H_prev{};
     Fwd fz_prev{};
@@ -252,11 +257,11 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
                 const double uc = ucomp(A_cur, m);
                 const double uw = gld(A.A, A.fb, 1 + m, cw), ue = gld(A.A, A.fb, 1 + m, ce);
                 const double us = gld(A.A, A.fb, 1 + m, cs), un = gld(A.A, A.fb, 1 + m, cn);
-                double sx_lo = VIS * (uc - uw) / K.dx[0];
-                double sx_hi = VIS * (ue - uc) / K.dx[0];
-                double sy_lo = VIS * (uc - us) / K.dx[1];
-                double sy_hi = VIS * (un - uc) / K.dx[1];
-                double sz_hi = VIS * (ucomp(A_next, m) - uc) / K.dx[2];
+                double sx_lo = div_rn(VIS * (uc - uw), K.dx[0], K.rdx[0]);
+                double sx_hi = div_rn(VIS * (ue - uc), K.dx[0], K.rdx[0]);
+                double sy_lo = div_rn(VIS * (uc - us), K.dx[1], K.rdx[1]);
+                double sy_hi = div_rn(VIS * (un - uc), K.dx[1], K.rdx[1]);
+                double sz_hi = div_rn(VIS * (ucomp(A_next, m) - uc), K.dx[2], K.rdx[2]);
                 f[m] = ((sx_hi - sx_lo) * K.ds1 + (sy_hi - sy_lo) * K.ds2 + (sz_hi - sz_lo[m]) * K.ds3) * A_cur.ro;
                 sz_lo[m] = sz_hi;
             }
@@ -310,18 +315,18 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
             }
             {
                 const CellSide s = make_side<0>(A_cur, H, Pxb, Pxf);
-                fx = cell_forward(s, dt, K.dx[0], K);
-                bx = cell_backward<Quirk>(s, dt, K.dx[0], K);
+                fx = cell_forward<true>(s, dt, K.dx[0], K, rdt, K.rdx[0]);
+                bx = cell_backward<Quirk, true>(s, dt, K.dx[0], K, rdt, K.rdx[0]);
             }
             {
                 const CellSide s = make_side<1>(A_cur, H, Pyb, Pyf);
-                fy = cell_forward(s, dt, K.dx[1], K);
-                by = cell_backward<false>(s, dt, K.dx[1], K);
+                fy = cell_forward<true>(s, dt, K.dx[1], K, rdt, K.rdx[1]);
+                by = cell_backward<false, true>(s, dt, K.dx[1], K, rdt, K.rdx[1]);
             }
             {
                 const CellSide s = make_side<2>(A_cur, H, Pzb, Pzf);
-                fz = cell_forward(s, dt, K.dx[2], K);
-                bz = cell_backward<Quirk>(s, dt, K.dx[2], K);
+                fz = cell_forward<true>(s, dt, K.dx[2], K, rdt, K.rdx[2]);
+                bz = cell_backward<Quirk, true>(s, dt, K.dx[2], K, rdt, K.rdx[2]);
             }
             ox[0] = Pxb.t; ox[1] = Pxb.u2; ox[2] = Pxb.u3;
             oy[0] = Pyb.t; oy[1] = Pyb.u1; oy[2] = Pyb.u3;
@@ -339,8 +344,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         // ---- z-face p and the corrector of plane p-1 ------------------------
         Flux5 mz{};
         if (p >= k0) {
-            const FaceOut o = face_combine(fz_prev, bz, H_prev.u3, H.u3, H_prev.t, H.t, H_prev.u1, H.u1, H_prev.u2,
-                                           H.u2, oz[0], oz[1], oz[2], K);
+            const FaceOut o = face_combine<true>(fz_prev, bz, H_prev.u3, H.u3, H_prev.t, H.t, H_prev.u1, H.u1,
+                                                 H_prev.u2, H.u2, oz[0], oz[1], oz[2], K);
             mz = new_flux<2>(o);
             if (owned) {
                 if (p < kend || kend == g.N3 + 1) store_face<2>(A, at(g, i, j, p), o);
@@ -370,16 +375,16 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
             if (xface) {
                 const LdsCell& nb = lds[tid - 1].c;
                 const Fwd fwb{nb.fx[0], nb.fx[1], nb.fx[2], nb.fx[3]};
-                const FaceOut o = face_combine(fwb, bx, nb.h[1], H.u1, nb.h[4], H.t, nb.h[2], H.u2, nb.h[3], H.u3,
-                                               ox[0], ox[1], ox[2], K);
+                const FaceOut o = face_combine<true>(fwb, bx, nb.h[1], H.u1, nb.h[4], H.t, nb.h[2], H.u2, nb.h[3],
+                                                     H.u3, ox[0], ox[1], ox[2], K);
                 if (xface_store) store_face<0>(A, at(g, i, j, p), o);
                 mx = new_flux<0>(o);
             }
             if (yface) {
                 const LdsCell& nb = lds[tid - RX].c;
                 const Fwd fwb{nb.fy[0], nb.fy[1], nb.fy[2], nb.fy[3]};
-                const FaceOut o = face_combine(fwb, by, nb.h[2], H.u2, nb.h[4], H.t, nb.h[1], H.u1, nb.h[3], H.u3,
-                                               oy[0], oy[1], oy[2], K);
+                const FaceOut o = face_combine<true>(fwb, by, nb.h[2], H.u2, nb.h[4], H.t, nb.h[1], H.u1, nb.h[3],
+                                                     H.u3, oy[0], oy[1], oy[2], K);
                 if (yface_store) store_face<1>(A, at(g, i, j, p), o);
                 my = new_flux<1>(o);
             }
