@@ -601,6 +601,34 @@ ark_status ark_phase2(ark_ctx* c) {
     return s;
 }
 
+// npairs x two phase-path steps from one captured graph.  A step is only
+// kernel launches on c->stream, its dt in device memory and no memset or
+// host synchronisation; the host side of a step swaps the face buffers, so
+// two steps bring every buffer back to the role it had, and the host state
+// after the capture (which ran that host side once) is the state after any
+// number of pairs.  The graph lives for this call only.
+ark_status ark_phase_pairs(ark_ctx* c, int npairs) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ark_status s = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (s != ARK_OK) return s;
+    ark_status body = ARK_OK;
+    for (int k = 0; k < 2 && body == ARK_OK; ++k) {
+        body = ark_refresh_rho(c);
+        if (body == ARK_OK) body = ark_begin_step(c);
+        if (body == ARK_OK) body = ark_phase_body(c);
+    }
+    s = c->hip(hipStreamEndCapture(c->stream, &graph), "end capture");
+    if (body != ARK_OK) s = body;
+    if (s == ARK_OK) s = c->hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    for (int k = 0; k < npairs && s == ARK_OK; ++k) s = c->hip(hipGraphLaunch(exec, c->stream), "graph launch");
+    const ark_status sync = c->hip(hipStreamSynchronize(c->stream), "sync after graph");
+    if (s == ARK_OK) s = sync;
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    return s;
+}
+
 ark_status ark_step_phases(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);

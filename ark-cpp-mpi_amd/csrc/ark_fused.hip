@@ -498,6 +498,13 @@ ark_status ark_step_fused(ark_ctx* c, int nsteps) {
         s = fused_steps(c, nsteps - 1 - done);
         done = nsteps - 1;
     }
+    // a long run on the phase path (solid cells, physical boundaries) is
+    // launch-bound: its steps go in pairs through one captured graph
+    if (s == ARK_OK && nsteps - done >= 8 && !c->profiling) {
+        const int pairs = (nsteps - done) / 2;
+        s = ark_phase_pairs(c, pairs);
+        done += 2 * pairs;
+    }
     while (s == ARK_OK && done < nsteps) {
         s = ark_refresh_rho(c);
         if (s == ARK_OK) s = ark_begin_step(c);

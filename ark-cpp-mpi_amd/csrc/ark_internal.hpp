@@ -89,6 +89,7 @@ ark_status ark_cfl_finalize(ark_ctx* c, int mode);      // see k_cfl_finalize
 ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pending MIN or Con), TIME, nStep
 ark_status ark_refresh_rho(ark_ctx* c);                 // store face densities if they are derived
 ark_status ark_phase_body(ark_ctx* c);                  // Phase1 .. UseForces of one step, dt already set
+ark_status ark_phase_pairs(ark_ctx* c, int npairs);     // 2*npairs phase-path steps from one captured graph
 }
 
 // Forcing of a batch of contexts in two launches per step (ark_turb.hip).
