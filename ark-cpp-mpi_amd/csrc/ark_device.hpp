@@ -91,26 +91,17 @@ __device__ __forceinline__ void block_min_atomic(unsigned long long bits, unsign
     if (threadIdx.x == 0 && block_min != ~0ull) atomicMin(slot, block_min);
 }
 
-// Tile order of the z-marching kernels.  The launch is one-dimensional with
-// one workgroup per tile; the hardware hands workgroup b to die (XCD) b % 8,
-// and each die has its own L2.  Tile t(b) gives each die one contiguous run of
-// tile numbers, numbered x fastest, then y, then z-chunk, so the tiles that
-// run at the same time on one die are neighbours and the one-cell rings they
-// share come from that die's L2 instead of HBM.  t(b) is a bijection on
-// [0, T) for any T.
+// Work order of the z-marching kernels.  The launch is one-dimensional, and
+// the hardware hands workgroup b to die (XCD) b % 8, each with its own L2.
+// die_major(T) maps b to a unit in [0, T) so that each die takes one
+// contiguous run of units: units numbered with neighbouring tiles adjacent
+// are then in flight on the same die, and the one-cell rings they share come
+// from that die's L2 instead of HBM.  It is a bijection on [0, T) for any T.
 constexpr int kDies = 8;
-struct TileId { int x, y, z; };
-__device__ __forceinline__ TileId die_major_tile(int nbx, int nby, int nbz) {
-    const int T = nbx * nby * nbz;
+__device__ __forceinline__ int die_major(int T) {
     const int b = blockIdx.x;
     const int die = b % kDies, local = b / kDies;
-    const int q = T / kDies, r = T % kDies;
-    const int t = die * q + min(die, r) + local;
-    TileId o;
-    o.x = t % nbx;
-    o.y = (t / nbx) % nby;
-    o.z = t / (nbx * nby);
-    return o;
+    return die * (T / kDies) + min(die, T % kDies) + local;
 }
 
 // Numeric constants of a context.

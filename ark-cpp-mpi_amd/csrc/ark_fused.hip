@@ -203,9 +203,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
     const double dt = a.S->dt, rdt = 1.0 / dt;
     const int tid = threadIdx.x;
     const int tx = tid % RX, ty = tid / RX;
-    const int U = gridDim.x, blk = blockIdx.x;
-    const int die = blk % kDies, local = blk / kDies;
-    const int4 w = units[die * (U / kDies) + min(die, U % kDies) + local];
+    const int4 w = units[die_major(gridDim.x)];
     const int i0 = (w.x % nbx) * TX + 1, j0 = (w.x / nbx) * TY + 1, k0 = w.y;
     const int iend = min(i0 + TX, g.N1 + 1), jend = min(j0 + TY, g.N2 + 1), kend = w.z;
     // logical cell of this thread (clamped into the region so every access stays in bounds)

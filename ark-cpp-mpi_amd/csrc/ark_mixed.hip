@@ -580,10 +580,7 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
 // of chunks, so the columns in flight on one die are neighbours.
 template <bool Quirk, class S>
 __global__ void __launch_bounds__(S::NTHR) k_fused_mixed(Args a, const int4* __restrict__ units, int nbx) {
-    const int U = gridDim.x, b = blockIdx.x;
-    const int die = b % kDies, local = b / kDies;
-    const int q = U / kDies, r = U % kDies;
-    const int4 w = units[die * q + min(die, r) + local];
+    const int4 w = units[die_major(gridDim.x)];
     mixed_tile<Quirk, S>(a, w.x % nbx, w.x / nbx, w.y, w.z);
 }
 
@@ -595,10 +592,7 @@ __global__ void __launch_bounds__(S::NTHR) k_fused_mixed(Args a, const int4* __r
 template <bool Quirk, class S>
 __global__ void __launch_bounds__(S::NTHR) k_fused_mixed_batch(const Args* __restrict__ boxes,
                                                                const int* __restrict__ start, int nboxes) {
-    const int T = start[nboxes];
-    const int die = blockIdx.x % kDies, local = blockIdx.x / kDies;
-    const int q = T / kDies, r = T % kDies;
-    const int t = die * q + min(die, r) + local;
+    const int t = die_major(start[nboxes]);
     int lo = 0, hi = nboxes - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
