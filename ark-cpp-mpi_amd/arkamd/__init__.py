@@ -44,7 +44,7 @@ SCHEME_REFERENCE, SCHEME_CORRECTED = 0, 1
 
 EXPORTED_SYMBOLS = [
     "ark_create", "ark_destroy", "ark_last_error", "ark_upload", "ark_download", "ark_set_dt",
-    "ark_get_dt", "ark_time_step_size", "ark_phase1", "ark_stress_tensor", "ark_use_forces",
+    "ark_get_dt", "ark_next_step", "ark_half_time", "ark_time_step_size", "ark_phase1", "ark_stress_tensor", "ark_use_forces",
     "ark_phase2", "ark_step", "ark_advance", "ark_step_phases", "ark_diagnostics", "ark_init_tgv",
     "ark_synchronize", "ark_get_stream", "ark_profile_enable", "ark_profile_read",
     "ark_profile_reset", "ark_get_plane", "ark_set_plane", "ark_boundary_conditions", "ark_device_bytes",
@@ -116,7 +116,7 @@ def lib() -> ctypes.CDLL:
     L.ark_get_dt.argtypes = [vp, dp]
     L.ark_time_step_size.argtypes = [vp, dp]
     for fn in ("ark_phase1", "ark_stress_tensor", "ark_use_forces", "ark_phase2", "ark_synchronize",
-               "ark_profile_reset"):
+               "ark_profile_reset", "ark_next_step", "ark_half_time"):
         getattr(L, fn).argtypes = [vp]
     L.ark_step.argtypes = [vp, ip]
     L.ark_step_phases.argtypes = [vp, ip]
@@ -298,6 +298,14 @@ class Solver:
         {"TimeStepSize": self.time_step_size, "Phase1": self.phase1,
          "StressTensor": self.stress_tensor, "UseForces": self.use_forces,
          "Phase2": self.phase2, "Step": lambda: self.step(1)}[name]()
+
+    def next_step(self) -> None:
+        """nStep++ of the reference's loop (source/main.cpp:83)."""
+        self._check(self._L.ark_next_step(self._h))
+
+    def half_time(self) -> None:
+        """TIME += 0.5 * dt of the reference's loop (source/main.cpp:84, 91)."""
+        self._check(self._L.ark_half_time(self._h))
 
     def step(self, nsteps: int = 1) -> None:
         """nsteps of the driver loop (source/main.cpp:82-91), fused kernels."""

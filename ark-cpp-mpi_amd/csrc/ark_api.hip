@@ -74,11 +74,12 @@ ark_status ark_ctx::launch(const char* name, hipError_t (*fn)(void*), void* arg)
     if (!profiling) return hip(fn(arg), name);
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return fail(ARK_ERR_HIP, "hipEventCreate");
-    hipEventRecord(a, stream);
-    hipError_t e = fn(arg);
-    hipEventRecord(b, stream);
+    ark_status s = hip(hipEventRecord(a, stream), "hipEventRecord");
+    const hipError_t e = fn(arg);
+    if (s == ARK_OK) s = hip(hipEventRecord(b, stream), "hipEventRecord");
     prof_events.push_back({name, a, b});
-    return hip(e, name);
+    const ark_status sk = hip(e, name);
+    return sk != ARK_OK ? sk : s;
 }
 
 Fields ark_ctx::fields() const {
@@ -252,13 +253,14 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
 
 void ark_destroy(ark_ctx* c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    // best effort: nothing can be reported from here
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->prof_events) {
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
-    for (void* m : c->allocs) hipFree(m);
-    if (c->stream) hipStreamDestroy(c->stream);
+    for (void* m : c->allocs) (void)hipFree(m);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -343,6 +345,24 @@ ark_status ark_set_dt(ark_ctx* c, double dt) {
     ark_status s = c->hip(hipMemcpyAsync(&c->S->dt, &c->host_dt, sizeof(double), hipMemcpyHostToDevice, c->stream),
                           "set dt");
     return s == ARK_OK ? c->hip(hipStreamSynchronize(c->stream), "sync") : s;
+}
+
+struct ClockArgs { ark_ctx* c; int what; };
+hipError_t run_clock(void* p) {
+    ClockArgs* a = (ClockArgs*)p;
+    return launch_clock(a->c->stream, a->c->S, a->what);
+}
+
+ark_status ark_next_step(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    ClockArgs a{c, 0};
+    return c->launch("clock", run_clock, &a);
+}
+
+ark_status ark_half_time(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    ClockArgs a{c, 1};
+    return c->launch("clock", run_clock, &a);
 }
 
 ark_status ark_get_dt(ark_ctx* c, double* dt) {
@@ -840,8 +860,8 @@ ark_status ark_profile_reset(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
     ark_status s = c->hip(hipStreamSynchronize(c->stream), "sync");
     for (auto& e : c->prof_events) {
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
     c->prof_events.clear();
     return s;
@@ -857,7 +877,8 @@ ark_status ark_profile_read(ark_ctx* c, int max_entries, char (*names)[64], int6
     std::vector<double> tot;
     for (auto& e : c->prof_events) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e.a, e.b);
+        s = c->hip(hipEventElapsedTime(&ms, e.a, e.b), "hipEventElapsedTime");
+        if (s != ARK_OK) return s;
         size_t q = std::find(keys.begin(), keys.end(), e.name) - keys.begin();
         if (q == keys.size()) {
             keys.push_back(e.name);

@@ -56,6 +56,15 @@ __global__ void k_cfl_finalize(Scalars* S, int mode) {
     }
 }
 
+// The bookkeeping statements of the reference's loop, one at a time
+// (source/main.cpp:83-84, 91): what 0 is nStep++, what 1 is TIME += 0.5 * dt.
+__global__ void k_clock(Scalars* S, int what) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (what == 0) S->nstep += 1;
+        else S->time = S->time + 0.5 * S->dt;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Phase1 (source/main.cpp:417-575): Con <- nCon, then the flux-form half-step.
 __device__ __forceinline__ void phase1_cell(const Fields& F, int c, int px, int sxy, double dt, const Consts& K) {
@@ -641,6 +650,11 @@ hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, 
 
 hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance) {
     hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(64), 0, st, S, advance);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock(hipStream_t st, Scalars* S, int what) {
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, S, what);
     return hipGetLastError();
 }
 

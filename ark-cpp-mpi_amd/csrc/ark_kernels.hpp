@@ -31,6 +31,7 @@ struct TgvTables {
 hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, double c2, double c3, double sound,
                       Scalars* S);
 hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
+hipError_t launch_clock(hipStream_t st, Scalars* S, int what);   // 0: nStep++, 1: TIME += 0.5 dt
 hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan, int zero);   // zero: non-periodic exchange
 hipError_t launch_zero_gradient(hipStream_t st, const Geo& g, const GhostPlan& plan);

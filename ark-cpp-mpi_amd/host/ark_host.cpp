@@ -91,11 +91,21 @@ ark_params Solver::periodic_box(int N1, int N2, int N3, double L1, double L2, do
     return p;
 }
 
+ark_params Solver::channel(int n1_g, int n2_g, int n3_g, double x1_e, double x2_n, double x3_t, double CFL,
+                           double VIS, double sound) {
+    // node spacing dx = (x_e - x_w)/(n - 1) with x_w = 0 (source/main.cpp:291-293)
+    ark_params p = periodic_box(n1_g - 1, n2_g - 1, n3_g - 1, x1_e - 0., x2_n - 0., x3_t - 0., CFL, VIS, sound);
+    p.periodic[0] = p.periodic[1] = p.periodic[2] = 0;
+    return p;
+}
+
 void Solver::check(ark_status s, const char* what) const {
     if (s != ARK_OK) throw Error(s, std::string(what) + ": " + (_c ? ark_last_error(_c) : ""));
 }
 
 void Solver::TimeStepSize() { check(ark_time_step_size(_c, nullptr), "TimeStepSize"); }
+void Solver::NextStep() { check(ark_next_step(_c), "NextStep"); }
+void Solver::HalfTime() { check(ark_half_time(_c), "HalfTime"); }
 void Solver::Phase1() { check(ark_phase1(_c), "Phase1"); }
 void Solver::StressTensor() { check(ark_stress_tensor(_c), "StressTensor"); }
 void Solver::UseForces() { check(ark_use_forces(_c), "UseForces"); }
@@ -104,6 +114,14 @@ void Solver::Step(int n) { check(ark_step(_c, n), "Step"); }
 void Solver::Advance(int n) { check(ark_advance(_c, n), "Advance"); }
 void Solver::StepPhases(int n) { check(ark_step_phases(_c, n), "StepPhases"); }
 void Solver::InitTGV(double U0) { check(ark_init_tgv(_c, U0), "InitTGV"); }
+void Solver::InitChannel(double u10) { check(ark_init_channel(_c, u10), "InitChannel"); }
+void Solver::WriteDataParaView(const std::string& dir, bool need_swap) {
+    check(ark_write_paraview(_c, dir.c_str(), need_swap ? 1 : 0), "WriteDataParaView");
+}
+void Solver::AverageValues() { check(ark_average_values(_c), "AverageValues"); }
+void Solver::WriteAverageValues(const std::string& dir, double start_time, bool need_swap) {
+    check(ark_write_average_values(_c, dir.c_str(), start_time, need_swap ? 1 : 0), "WriteAverageValues");
+}
 void Solver::Synchronize() { check(ark_synchronize(_c), "Synchronize"); }
 void Solver::SetDt(double dt) { check(ark_set_dt(_c, dt), "SetDt"); }
 

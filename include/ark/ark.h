@@ -121,6 +121,12 @@ ark_status ark_download(ark_ctx* ctx, int field, double* host, int e1, int e2, i
 /* The global dt (headers/constants.h:20) and TIME/nStep bookkeeping. */
 ark_status ark_set_dt(ark_ctx* ctx, double dt);
 ark_status ark_get_dt(ark_ctx* ctx, double* dt);   /* synchronises */
+/* The bookkeeping statements of the reference's loop, for a caller that runs
+ * the phase functions itself: ark_next_step is nStep++ (source/main.cpp:83),
+ * ark_half_time is TIME += 0.5 * dt (84 and 91), on the device in stream
+ * order.  ark_step and ark_step_phases do both themselves. */
+ark_status ark_next_step(ark_ctx* ctx);
+ark_status ark_half_time(ark_ctx* ctx);
 
 /* TimeStepSize(), source/main.cpp:374-415: dt = min over interior cells of
  * the CFL bounds, on the device; the MPI MIN allreduce is the identity on one

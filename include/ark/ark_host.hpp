@@ -107,9 +107,17 @@ public:
      * the given physical constants (FP64, reference scheme unless set). */
     static ark_params periodic_box(int N1, int N2, int N3, double L1, double L2, double L3, double CFL,
                                    double VIS, double sound, double ro0 = 1.0, double t0 = 1.0);
+    /* The reference's own problem as its Input() sets it up
+     * (source/main.cpp:118-195): n1_g x n2_g x n3_g nodes on [0, x1_e] x
+     * [0, x2_n] x [0, x3_t], no periodic axis.  Fill it with InitChannel. */
+    static ark_params channel(int n1_g = 201, int n2_g = 61, int n3_g = 2, double x1_e = 20.0, double x2_n = 6.0,
+                              double x3_t = 0.1, double CFL = 0.2, double VIS = 1.0 / 1000.0, double sound = 10.0);
 
     /* The reference's phase functions (source/main.cpp:374-2495). */
     void TimeStepSize();
+    /* nStep++ and TIME += 0.5 * dt of the reference's loop (83-84, 91). */
+    void NextStep();
+    void HalfTime();
     void Phase1();
     void StressTensor();
     void UseForces();
@@ -123,6 +131,15 @@ public:
     void StepPhases(int n);
 
     void InitTGV(double U0 = 1.0);
+    /* InitializeData (source/main.cpp:280-372) of a channel() context. */
+    void InitChannel(double u10 = 1.0);
+
+    /* The output path (source/main.cpp:2577-2891): <dir>out_<nStep>.vtk,
+     * the running averages, and <dir>average.vtk; need_swap as the
+     * reference's needSwap (big-endian files). */
+    void WriteDataParaView(const std::string& dir, bool need_swap = true);
+    void AverageValues();
+    void WriteAverageValues(const std::string& dir, double start_time, bool need_swap = true);
     ark_diag Diagnostics();
     double dt();
     void SetDt(double dt);

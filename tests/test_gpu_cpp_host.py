@@ -1,7 +1,8 @@
 """GPU: the C++ host interface (include/ark/ark_host.hpp) through its two
 programs.  ark_driver runs the reference's time loop (source/main.cpp:80-104)
 and must reproduce the reference's 16^3 x 10 Taylor-Green digests bit for bit,
-phase by phase and fused; ark_host_check drives BoundaryConditions and Arr2d
+phase by phase and fused, and, with --channel, the reference's whole program
+on its own problem with its output files; ark_host_check drives BoundaryConditions and Arr2d
 exactly as a reference-side program would and must reproduce the reference
 BoundaryConditions class and halo packing loops."""
 import hashlib
@@ -38,6 +39,28 @@ def test_driver_reproduces_reference_digest(tmp_path, mode):
         if hashlib.sha256(data.tobytes()).hexdigest() != run["arrays"][name]["sha256"]:
             bad.append(name)
     assert not bad, f"{mode}: arrays differing from the reference: {bad}"
+
+
+@pytest.mark.parametrize("mode", [[], ["--phases"]])
+def test_driver_runs_the_reference_program(tmp_path, mode):
+    """ark_driver --channel is the reference's main() on its own problem: the
+    first output, 20 steps with WriteDataParaView every 10 from step 10 and
+    the running averages from step 5 -- the files the reference writes, byte
+    for byte, and its progress lines."""
+    files = json.load(open(os.path.join(GOLDEN, "channel_digests.json")))["vtk"]["files"]
+    out = tmp_path / "out"
+    res = subprocess.run([DRIVER, "--channel"] + mode + ["20", str(out), "10", "10", "5"], capture_output=True,
+                         text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    lines = res.stdout.strip().splitlines()
+    assert [l.split()[:2] for l in lines[:-1]] == [["step:", "10"], ["step:", "20"]]
+    rec = json.loads(lines[-1])
+    assert rec["nstep"] == 20 and rec["nonfinite"] == 0
+    assert (out / "out_0.vtk").exists()
+    for name in ("out_10.vtk", "out_20.vtk", "average.vtk"):
+        data = (out / name).read_bytes()
+        assert len(data) == files[name]["bytes"], name
+        assert hashlib.sha256(data).hexdigest() == files[name]["sha256"], name
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])

@@ -254,3 +254,25 @@ def test_corrected_scheme_fused_vs_oracle(N):
         s.init_tgv()
         s.step(6)
         assert_state_equal(s, ref, what=f"corrected fused {N}")
+
+
+def test_reference_loop_by_hand_equals_step_phases():
+    """The reference's loop written out through the C-ABI (source/main.cpp:
+    82-91: TimeStepSize, nStep++, TIME += dt/2, the phases, TIME += dt/2)
+    gives step_phases' arrays, TIME and nStep bit for bit."""
+    with arkamd.Solver(16, 12, 10) as a, arkamd.Solver(16, 12, 10) as b:
+        a.init_tgv()
+        b.init_tgv()
+        for _ in range(3):
+            a.time_step_size()
+            a.next_step()
+            a.half_time()
+            for ph in ("Phase1", "StressTensor", "UseForces", "Phase2", "Phase1", "UseForces"):
+                a.phase(ph)
+            a.half_time()
+        b.step_phases(3)
+        ga, gb = a.download_all(), b.download_all()
+        bad = [describe_mismatch(n, ga[n], gb[n]) for n in FIELDS if not bits_equal(ga[n], gb[n])]
+        assert not bad, "\n".join(bad)
+        da, db = a.diagnostics(), b.diagnostics()
+        assert da["time"] == db["time"] and da["nstep"] == db["nstep"] == 3 and da["dt"] == db["dt"]
