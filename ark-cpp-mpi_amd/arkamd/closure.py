@@ -104,13 +104,18 @@ def train(X: np.ndarray, T: np.ndarray, steps: int, batch: int, lr: float, seed:
 
 
 def les(ub0: np.ndarray, t_end: float, vis: float, eps: float, seed: int, mlp: Optional[MLP] = None,
-        delta: float = 0.0, tau: float = 0.01) -> Dict:
+        delta: float = 0.0, tau: float = 0.01, kcut: int = 0) -> Dict:
+    """A coarse run from the filtered field ub0 to t_end with the same
+    forcing; with mlp, the learned closure, applied on the shells >= kcut only
+    when kcut > 0 (the scale-selective variant)."""
     N3, N2, N1 = ub0.shape[1:]
     s = Solver(N1, N2, N3, VIS=vis, precision=PREC_MIXED)
     s.init_velocity(ub0)
     s.set_forcing(seed + 1, eps, tau)
     if mlp is not None:
         s.set_closure(mlp, delta)
+        if kcut > 0:
+            s.set_closure_cutoff(kcut)
     s.synchronize()
     t0 = time.perf_counter()
     n = run_to(s, t_end)

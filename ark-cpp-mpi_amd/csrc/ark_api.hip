@@ -255,6 +255,7 @@ void ark_destroy(ark_ctx* c) {
     if (!c) return;
     // best effort: nothing can be reported from here
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    ark_closure_hp_free(c);
     for (auto& e : c->prof_events) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);

@@ -51,6 +51,7 @@ struct ark_ctx {
     double closure_delta = 0.0;
     float* les_X = nullptr;
     float* les_Y = nullptr;
+    struct ark_closure_hp* closure_hp = nullptr;   // scale-selective application (ark_set_closure_cutoff)
     double* average = nullptr;         // AverageValues' three accumulators (cell fields, common index map)
     void* mixed_units = nullptr;       // chunk table of the mixed-precision step (int4 per chunk)
     int mixed_nunits = 0;
@@ -91,6 +92,9 @@ ark_status ark_refresh_rho(ark_ctx* c);                 // store face densities 
 ark_status ark_phase_body(ark_ctx* c);                  // Phase1 .. UseForces of one step, dt already set
 ark_status ark_phase_pairs(ark_ctx* c, int npairs);     // 2*npairs phase-path steps from one captured graph
 }
+
+// The closure's spectral projection state (ark_turb.hip); freed with the context.
+void ark_closure_hp_free(ark_ctx* c);
 
 // Forcing of a batch of contexts in two launches per step (ark_turb.hip).
 struct ark_force_batch;

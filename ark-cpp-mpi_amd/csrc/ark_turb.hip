@@ -453,6 +453,54 @@ __global__ void k_les_force(Geo g, const float* Y, float inv0, float inv1, float
     }
 }
 
+// The closure's acceleration -d_j tau_ij into three contiguous N1 x N2 x N3
+// arrays (i fastest), the layout of a 3-D real transform.
+__global__ void k_les_force_dense(Geo g, const float* Y, float inv0, float inv1, float inv2, float* dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.N1) return;
+    const int ip = i == g.N1 ? 1 : i + 1, im = i == 1 ? g.N1 : i - 1;
+    const int jp = j == g.N2 ? 1 : j + 1, jm = j == 1 ? g.N2 : j - 1;
+    const int kp = k == g.N3 ? 1 : k + 1, km = k == 1 ? g.N3 : k - 1;
+    auto tau = [&](int a, int b, int c, int q) { return Y[(((size_t)(c - 1) * g.N2 + (b - 1)) * g.N1 + (a - 1)) * 6 + q]; };
+    const int cmp[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+    const size_t n = (size_t)g.N1 * g.N2 * g.N3, o = ((size_t)(k - 1) * g.N2 + (j - 1)) * g.N1 + (i - 1);
+    for (int m = 0; m < 3; ++m) {
+        const float div = (tau(ip, j, k, cmp[m][0]) - tau(im, j, k, cmp[m][0])) * inv0 +
+                          (tau(i, jp, k, cmp[m][1]) - tau(i, jm, k, cmp[m][1])) * inv1 +
+                          (tau(i, j, kp, cmp[m][2]) - tau(i, j, km, cmp[m][2])) * inv2;
+        dst[m * n + o] = -div;
+    }
+}
+
+// Keeps the Fourier modes of shells round(|m|) >= kcut (m the integer
+// wavevector) and scales them by `scale` (the inverse transform's 1/n); the
+// others are zeroed.  Three half spectra of H1 x N2 x N3, H1 fastest.
+__global__ void k_keep_high_shells(float2* spec, int N1, int N2, int N3, int H1, int kcut, float scale) {
+    const size_t nc = (size_t)H1 * N2 * N3;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < 3 * nc; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = q % nc;
+        const int a = (int)(r % H1), b = (int)((r / H1) % N2), c = (int)(r / ((size_t)H1 * N2));
+        const int m1 = a, m2 = wave_index(b, N2), m3 = wave_index(c, N3);
+        const int shell = (int)lrint(sqrt((double)m1 * m1 + (double)m2 * m2 + (double)m3 * m3));
+        const float w = shell >= kcut ? scale : 0.f;
+        spec[q] = make_float2(spec[q].x * w, spec[q].y * w);
+    }
+    (void)N1;
+}
+
+// force (common index map) := force + src (add) or src, for the three
+// components of a dense field.
+__global__ void k_put_force(Geo g, const float* src, float* force, long long stride, int add) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.N1) return;
+    const size_t n = (size_t)g.N1 * g.N2 * g.N3, o = ((size_t)(k - 1) * g.N2 + (j - 1)) * g.N1 + (i - 1);
+    const int c = at(g, i, j, k);
+    for (int m = 0; m < 3; ++m) {
+        float* f = force + m * stride + c;
+        *f = (add ? *f : 0.f) + src[m * n + o];
+    }
+}
+
 // ---------------------------------------------------------------------------
 struct Fft {
     hipfftHandle plan_f = 0, plan_b = 0;
@@ -670,6 +718,26 @@ static ark_status ensure_force(ark_ctx* c) {
     return s;
 }
 
+// Scale-selective application: FP32 transforms of the closure's
+// acceleration, in the context's stream.
+struct ark_closure_hp {
+    hipfftHandle plan_f = 0, plan_b = 0;
+    float* real = nullptr;     // 3 x N1 x N2 x N3
+    float2* spec = nullptr;    // 3 x H1 x N2 x N3
+    int kcut = 0;
+};
+
+void ark_closure_hp_free(ark_ctx* c) {
+    ark_closure_hp* h = c->closure_hp;
+    if (!h) return;
+    if (h->plan_f) (void)hipfftDestroy(h->plan_f);
+    if (h->plan_b) (void)hipfftDestroy(h->plan_b);
+    if (h->real) (void)hipFree(h->real);
+    if (h->spec) (void)hipFree(h->spec);
+    delete h;
+    c->closure_hp = nullptr;
+}
+
 // The closure's contribution to this step's body force (after the forcing).
 ark_status ark_closure_step(ark_ctx* c, hipStream_t st) {
     if (!c->closure) return ARK_OK;
@@ -680,10 +748,33 @@ ark_status ark_closure_step(ark_ctx* c, hipStream_t st) {
     ark_status s = c->hip(hipGetLastError(), "closure inputs");
     if (s == ARK_OK) s = ark_mlp_forward_device(c->closure, c->les_X, c->les_Y, (long long)g.N1 * g.N2 * g.N3, st);
     if (s != ARK_OK) return s;
-    hipLaunchKernelGGL(k_les_force, interior_grid(g, 0), dim3(256), 0, st, g, c->les_Y, (float)(0.5 / c->prm.dx1),
-                       (float)(0.5 / c->prm.dx2), (float)(0.5 / c->prm.dx3), c->force, c->force_stride,
-                       c->force_eps > 0.0 ? 1 : 0);
-    return c->hip(hipGetLastError(), "closure force");
+    const int add = c->force_eps > 0.0 ? 1 : 0;
+    const float inv0 = (float)(0.5 / c->prm.dx1), inv1 = (float)(0.5 / c->prm.dx2), inv2 = (float)(0.5 / c->prm.dx3);
+    ark_closure_hp* h = c->closure_hp;
+    if (!h) {
+        hipLaunchKernelGGL(k_les_force, interior_grid(g, 0), dim3(256), 0, st, g, c->les_Y, inv0, inv1, inv2, c->force,
+                           c->force_stride, add);
+        return c->hip(hipGetLastError(), "closure force");
+    }
+    // only the shells >= kcut of the acceleration
+    const size_t n = (size_t)g.N1 * g.N2 * g.N3, nc = (size_t)(g.N1 / 2 + 1) * g.N2 * g.N3;
+    hipLaunchKernelGGL(k_les_force_dense, interior_grid(g, 0), dim3(256), 0, st, g, c->les_Y, inv0, inv1, inv2,
+                       h->real);
+    s = c->hip(hipGetLastError(), "closure force");
+    for (int m = 0; m < 3 && s == ARK_OK; ++m)
+        if (hipfftExecR2C(h->plan_f, h->real + m * n, (hipfftComplex*)(h->spec + m * nc)) != HIPFFT_SUCCESS)
+            s = c->fail(ARK_ERR_HIP, "closure R2C");
+    if (s == ARK_OK) {
+        hipLaunchKernelGGL(k_keep_high_shells, dim3(1024), dim3(256), 0, st, h->spec, g.N1, g.N2, g.N3, g.N1 / 2 + 1,
+                           h->kcut, (float)(1.0 / (double)n));
+        s = c->hip(hipGetLastError(), "closure projection");
+    }
+    for (int m = 0; m < 3 && s == ARK_OK; ++m)
+        if (hipfftExecC2R(h->plan_b, (hipfftComplex*)(h->spec + m * nc), h->real + m * n) != HIPFFT_SUCCESS)
+            s = c->fail(ARK_ERR_HIP, "closure C2R");
+    if (s != ARK_OK) return s;
+    hipLaunchKernelGGL(k_put_force, interior_grid(g, 0), dim3(256), 0, st, g, h->real, c->force, c->force_stride, add);
+    return c->hip(hipGetLastError(), "closure force add");
 }
 
 extern "C" {
@@ -710,6 +801,36 @@ ark_status ark_set_closure(ark_ctx* c, ark_mlp* mlp, double delta) {
         c->closure = mlp;
         c->closure_delta = delta;
     }
+    return s;
+}
+
+ark_status ark_set_closure_cutoff(ark_ctx* c, int kcut) {
+    if (!c || kcut < 0) return ARK_ERR_INVALID;
+    if (c->prm.precision != ARK_PREC_MIXED)
+        return c->fail(ARK_ERR_UNSUPPORTED, "closure: mixed-precision contexts only");
+    if (kcut == 0) {
+        ark_status s = c->hip(hipStreamSynchronize(c->stream), "sync");
+        ark_closure_hp_free(c);
+        return s;
+    }
+    if (c->closure_hp) {
+        c->closure_hp->kcut = kcut;
+        return ARK_OK;
+    }
+    const Geo& g = c->geo;
+    ark_closure_hp* h = new ark_closure_hp;
+    h->kcut = kcut;
+    c->closure_hp = h;
+    const size_t n = (size_t)g.N1 * g.N2 * g.N3, nc = (size_t)(g.N1 / 2 + 1) * g.N2 * g.N3;
+    ark_status s = c->hip(hipMalloc(&h->real, 3 * n * sizeof(float)), "hipMalloc closure projection");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&h->spec, 3 * nc * sizeof(float2)), "hipMalloc closure projection");
+    if (s == ARK_OK && (hipfftPlan3d(&h->plan_f, g.N3, g.N2, g.N1, HIPFFT_R2C) != HIPFFT_SUCCESS ||
+                        hipfftPlan3d(&h->plan_b, g.N3, g.N2, g.N1, HIPFFT_C2R) != HIPFFT_SUCCESS))
+        s = c->fail(ARK_ERR_HIP, "hipfftPlan3d for the closure projection failed");
+    if (s == ARK_OK && (hipfftSetStream(h->plan_f, c->stream) != HIPFFT_SUCCESS ||
+                        hipfftSetStream(h->plan_b, c->stream) != HIPFFT_SUCCESS))
+        s = c->fail(ARK_ERR_HIP, "hipfftSetStream");
+    if (s != ARK_OK) ark_closure_hp_free(c);
     return s;
 }
 

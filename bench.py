@@ -216,10 +216,20 @@ def closure_experiment(N, steps, width, train_steps, batch):
     d = C.dns(N, steps=steps, snapshots=6, width=width, vis=vis, eps=eps, seed=seed, series_every=100)
     tr = C.train(d["X"], d["T"], steps=train_steps, batch=batch, lr=1e-3, seed=1)
     Eref = C.spectrum_of(d["ub_end"], vis)
+    nc = N // width
     plain = C.les(d["ub0"], d["t_end"], vis, eps, seed)
     model = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=tr["mlp"], delta=width * d["dx"])
+    # the scale-selective closure (variational multiscale): the network's
+    # acceleration on the shells >= 3/8 of the coarse Nyquist wavenumber only
+    kc = 3 * (nc // 2) // 8
+    vms = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=tr["mlp"], delta=width * d["dx"], kcut=kc)
+    sens = {}
+    for k in sorted({nc // 8, nc // 6, nc // 4 - nc // 16, nc // 4} - {kc}):
+        r = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=tr["mlp"], delta=width * d["dx"], kcut=k)
+        sens[str(k)] = {"spectrum_err_k<=%d" % (nc // 4): round(C.spectrum_error(r["E"], Eref, nc // 4), 4),
+                        "spectrum_err_k<=%d" % (nc // 2): round(C.spectrum_error(r["E"], Eref, nc // 2), 4),
+                        "nonfinite": r["nonfinite"]}
     tr["mlp"].close()
-    nc = N // width
     out = {"fine": {"grid": N, "steps": steps, "time": d["t_end"], "wall_s": round(d["wall"], 3),
                     "Mcell_steps_per_s": round(N ** 3 * steps / d["wall"] / 1e6, 1), "nonfinite": d["nonfinite"],
                     "dissipation_end": d["eps_end"],
@@ -228,15 +238,18 @@ def closure_experiment(N, steps, width, train_steps, batch):
            "training": {"pairs": int(d["X"].shape[0]), "steps": train_steps, "batch": batch,
                         "loss0": tr["loss0"], "loss_end": tr["loss_end"], "wall_s": round(tr["wall"], 3)},
            "coarse_grid": nc, "filter_width_cells": width}
-    for name, r in (("plain", plain), ("closure", model)):
+    for name, r in (("plain", plain), ("closure", model), ("closure_scale_selective", vms)):
         out[name] = {"steps": r["steps"], "wall_s": round(r["wall"], 4), "nonfinite": r["nonfinite"],
                      "spectrum_err_k<=%d" % (nc // 4): round(C.spectrum_error(r["E"], Eref, nc // 4), 4),
                      "spectrum_err_k<=%d" % (nc // 2): round(C.spectrum_error(r["E"], Eref, nc // 2), 4),
                      "dissipation_resolved": r["eps_resolved"],
                      "wall_ratio_fine_over_coarse": round(d["wall"] / r["wall"], 1)}
+    out["closure_scale_selective"]["kcut"] = kc
+    out["closure_kcut_sensitivity"] = sens
     out["spectrum_filtered_fine"] = [float("%.5g" % v) for v in Eref[: nc // 2 + 1]]
     out["spectrum_plain"] = [float("%.5g" % v) for v in plain["E"][: nc // 2 + 1]]
     out["spectrum_closure"] = [float("%.5g" % v) for v in model["E"][: nc // 2 + 1]]
+    out["spectrum_closure_scale_selective"] = [float("%.5g" % v) for v in vms["E"][: nc // 2 + 1]]
     return out
 
 

@@ -313,6 +313,13 @@ ark_status ark_mlp_predict(ark_mlp* mlp, const float* X, float* Y, long long n);
  * (with the forcing, if any).  NULL detaches.  The network must outlive the
  * attachment; a context with a closure cannot join a batch. */
 ark_status ark_set_closure(ark_ctx* ctx, ark_mlp* mlp, double delta);
+/* Scale-selective closure (variational multiscale, Hughes, Mazzei & Jansen,
+ * Comput. Visual. Sci. 3 (2000) 47-59): with kcut > 0 only the Fourier modes
+ * of the closure's acceleration in shells round(|m|) >= kcut (m the integer
+ * wavevector) are added to the body force, so the network acts on the small
+ * resolved scales only; kcut = 0 (the default) adds all of it.  FP32
+ * transforms on the context's stream, three each way per step. */
+ark_status ark_set_closure_cutoff(ark_ctx* ctx, int kcut);
 /* Initial state from host cell velocities u (3 x N1*N2*N3 interior values,
  * component-major, x fastest): the cells take them, the faces the same
  * Fourier series half a cell down (as ark_init_turbulence); uniform rho0 and

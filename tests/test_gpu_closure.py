@@ -59,3 +59,39 @@ def test_les_with_and_without_closure(pipeline):
     Eref = C.spectrum_of(d["ub_end"], VIS)
     for r in (plain, model):
         assert np.isfinite(C.spectrum_error(r["E"], Eref, 8))
+
+
+def _shell_energy(du):
+    """Energy per shell round(|m|) of a (3, N, N, N) periodic field."""
+    N = du.shape[-1]
+    m = np.fft.fftfreq(N, 1.0 / N)
+    kk = np.sqrt(m[:, None, None] ** 2 + m[None, :, None] ** 2 + m[None, None, :] ** 2)
+    shell = np.rint(kk).astype(int)
+    e = sum(np.abs(np.fft.fftn(c)) ** 2 for c in du)
+    return np.bincount(shell.ravel(), weights=e.ravel())
+
+
+def test_scale_selective_closure_acts_on_high_shells_only(pipeline):
+    """With a cutoff, the closure's acceleration keeps only the shells >= kcut:
+    over one step from the same state, the velocity change it causes has (to
+    second order in dt) no energy below kcut, while the whole closure's does,
+    and above kcut the two agree."""
+    d, tr = pipeline
+    kc = 6
+
+    def one_step(closure, kcut=0):
+        with arkamd.Solver(32, 32, 32, VIS=VIS, precision=arkamd.PREC_MIXED) as s:
+            s.init_velocity(d["ub0"])
+            if closure:
+                s.set_closure(tr["mlp"], 2 * d["dx"])
+                if kcut:
+                    s.set_closure_cutoff(kcut)
+            s.step(1)
+            return np.stack([s.download(n)[1:-1, 1:-1, 1:-1] for n in ("u1nCon", "u2nCon", "u3nCon")])
+
+    base = one_step(False)
+    e_full = _shell_energy(one_step(True) - base)
+    e_cut = _shell_energy(one_step(True, kc) - base)
+    assert e_cut[:kc].sum() < 1e-3 * e_cut[kc:].sum(), (e_cut[:kc].sum(), e_cut[kc:].sum())
+    assert e_full[:kc].sum() > 100 * e_cut[:kc].sum()
+    assert abs(e_cut[kc:].sum() / e_full[kc:].sum() - 1) < 0.05
