@@ -131,8 +131,11 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
     const long long pq = p->precision == ARK_PREC_MIXED ? 4 : 16;
     long long px = ((long long)p->N1 + 3 + pq - 1) / pq * pq;
     long long elems = px * (p->N2 + 3) * (long long)(p->N3 + 3);
-    if (elems + kOriginPad >= (1ll << 31)) {
-        c->err = "grid too large for 32-bit indexing";
+    // The fused kernels address a field with a 32-bit byte offset from its
+    // base (8 bytes per element): at most 2^29 elements, about 800^3 cells.
+    if ((elems + kOriginPad) * (long long)sizeof(double) >= (1ll << 32)) {
+        c->err = "grid too large: a field's byte offsets must fit in 32 bits (at most 2^29 elements per field, "
+                 "about 800^3 cells)";
         *out = c;
         return ARK_ERR_UNSUPPORTED;
     }

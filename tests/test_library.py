@@ -53,3 +53,14 @@ def test_cpp_host_layer_built():
     for prog in ("ark_driver", "ark_host_check"):
         res = subprocess.run([os.path.join(bindir, prog)], capture_output=True, text=True)
         assert res.returncode == 2 and "usage" in res.stderr
+
+
+def test_create_refuses_boxes_beyond_32_bit_offsets():
+    """The fused kernels address a field with 32-bit byte offsets, so a box
+    with more than 2^29 elements per field is refused at creation (before
+    any device work, so this runs without a GPU)."""
+    for N in ((1024, 1024, 1024), (810, 810, 810), (4096, 512, 256)):
+        with pytest.raises(arkamd.ArkError) as e:
+            arkamd.Solver(*N)
+        assert e.value.status == arkamd.ARK_ERR_UNSUPPORTED
+        assert "32 bits" in str(e.value)
