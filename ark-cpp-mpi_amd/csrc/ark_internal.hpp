@@ -37,6 +37,8 @@ struct ark_ctx {
     double* diag_part = nullptr;
     double* staging = nullptr;         // one FP64 field (mixed precision conversions)
     double* plane_buf = nullptr;       // one interior plane (ark_get_plane / ark_set_plane)
+    double* dense = nullptr;           // one host array in its dense layout (upload / download)
+    size_t dense_n = 0;
     // body force of the mixed-precision step (ark_set_forcing): acceleration
     // components at force + m*force_stride in the field index map, FP32
     float* force = nullptr;

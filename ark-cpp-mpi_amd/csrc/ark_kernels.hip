@@ -820,6 +820,25 @@ hipError_t launch_boundary(hipStream_t st, const Geo& g, const BcFields& F, int 
     return hipGetLastError();
 }
 
+// Rows of e1 elements between two 3-D layouts: row (j, k) starts at
+// src + k*s_plane + j*s_row and at dst + k*d_plane + j*d_row.  One workgroup
+// per row: a host array in the reference's dense layout and a field in the
+// padded device layout trade places in one contiguous transfer.
+__global__ void k_copy_rows(const double* __restrict__ src, long long s_row, long long s_plane,
+                            double* __restrict__ dst, long long d_row, long long d_plane, int e1, int e2) {
+    const int r = blockIdx.x, j = r % e2, k = r / e2;
+    const double* s = src + k * s_plane + j * s_row;
+    double* d = dst + k * d_plane + j * d_row;
+    for (int i = threadIdx.x; i < e1; i += blockDim.x) d[i] = s[i];
+}
+
+hipError_t launch_copy_rows(hipStream_t st, const double* src, long long s_row, long long s_plane, double* dst,
+                            long long d_row, long long d_plane, int e1, int e2, int e3) {
+    hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)e2 * (unsigned)e3), dim3(256), 0, st, src, s_row, s_plane, dst,
+                       d_row, d_plane, e1, e2);
+    return hipGetLastError();
+}
+
 hipError_t launch_plane(hipStream_t st, const Geo& g, double* field, int axis, int q, double* buf, int to_buf) {
     const int Na = axis == 0 ? g.N2 : g.N1, Nb = axis == 2 ? g.N2 : g.N3;
     hipLaunchKernelGGL(k_plane, dim3((Na + 255) / 256, Nb), dim3(256), 0, st, g, field, axis, q, buf, to_buf);

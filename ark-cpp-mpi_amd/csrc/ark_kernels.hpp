@@ -42,6 +42,8 @@ hipError_t launch_average(hipStream_t st, const Geo& g, const Fields& F, const S
 struct BcFields { double* f[5]; };   // u1, u2, u3, ro, t
 hipError_t launch_boundary(hipStream_t st, const Geo& g, const BcFields& F, int axis, int periodic, double t0);
 hipError_t launch_plane(hipStream_t st, const Geo& g, double* field, int axis, int q, double* buf, int to_buf);
+hipError_t launch_copy_rows(hipStream_t st, const double* src, long long s_row, long long s_plane, double* dst,
+                            long long d_row, long long d_plane, int e1, int e2, int e3);
 hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 // Zero narr arrays of n elements each; dev_ptrs is a device array of pointers.
 hipError_t launch_zero(hipStream_t st, double* const* dev_ptrs, int narr, long long n);

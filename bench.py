@@ -15,6 +15,9 @@ Printed: ONE JSON line.  Beyond the common fields it carries
                 timed with HIP events on the library's stream
   accuracy      mixed vs FP64 after the same steps, per field (corrected
                 scheme: the reference scheme goes unstable on this vortex)
+  pcie_inclusive  the mixed run with its 20 state arrays downloaded and
+                uploaded through the host boundary (pageable FP64 arrays),
+                the transfers timed apart from the steps
   batched       64 independent boxes of uneven size and Reynolds number in one
                 launch per step (graph-replayed, tile shape autotuned), half
                 Taylor-Green, half forced turbulence, filtered into training pairs
@@ -253,6 +256,43 @@ def closure_experiment(N, steps, width, train_steps, batch):
     return out
 
 
+STATE_FIELDS = ("ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon",
+                "u11", "u21", "u31", "p1", "t1", "u12", "u22", "u32", "p2", "t2", "u13", "u23", "u33", "p3", "t3")
+
+
+def pcie_inclusive(N, steps):
+    """The mixed step with its state crossing the host boundary: the 20
+    persistent arrays (5 cell, 15 face; FP64 host arrays with the reference's
+    extents) downloaded and uploaded through ark_download / ark_upload from
+    pageable host memory, timed apart from the steps.  Reported beside the
+    headline, never as it: inputs resident in HBM are the headline's premise."""
+    import time
+    import arkamd
+    with arkamd.Solver(N, N, N, precision=arkamd.PREC_MIXED) as s:
+        s.init_tgv(1.0)
+        s.step(2)
+        s.synchronize()
+        t0 = time.perf_counter()
+        host = {n: s.download(n) for n in STATE_FIELDS}
+        t_down = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for n in STATE_FIELDS:
+            s.upload(n, host[n])
+        s.synchronize()
+        t_up = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        t_steps = time.perf_counter() - t0
+    nbytes = sum(a.nbytes for a in host.values())
+    cells = N ** 3
+    return {"state_bytes": nbytes, "download_s": round(t_down, 4), "upload_s": round(t_up, 4),
+            "download_GB_per_s": round(nbytes / t_down / 1e9, 2), "upload_GB_per_s": round(nbytes / t_up / 1e9, 2),
+            "steps": steps, "steps_s": round(t_steps, 4),
+            "Mcell_steps_per_s_state_in_and_out_once": round(cells * steps / (t_up + t_steps + t_down) / 1e6, 1),
+            "Mcell_steps_per_s_state_in_and_out_every_step": round(cells / (t_up + t_down + t_steps / steps) / 1e6, 2)}
+
+
 def accuracy(N, steps):
     import numpy as np
     import arkamd
@@ -377,6 +417,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--phase-steps", type=int, default=3,
                     help="steps of the phase-by-phase FP64 path timed per kernel (0: skip)")
+    ap.add_argument("--pcie-steps", type=int, default=500,
+                    help="steps of the run whose state crosses the host boundary (0: skip)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-ranks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -444,6 +486,8 @@ def main():
                                                                args.closure_train_steps, 65536)
     if args.accuracy_steps > 0 and not args.only:
         line["accuracy"] = accuracy(N, args.accuracy_steps)
+    if args.pcie_steps > 0 and not args.only:
+        line["pcie_inclusive"] = pcie_inclusive(N, args.pcie_steps)
     base = None
     if not args.no_cpu_baseline:
         ranks = args.cpu_ranks or min(16, os.cpu_count() or 1)
