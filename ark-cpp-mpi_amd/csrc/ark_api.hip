@@ -502,21 +502,27 @@ hipError_t run_phase2_ghost(void* p) {
     const int z_faces[5] = {face_u(2, 0), face_u(2, 1), face_u(2, 2), face_ro(2), face_p(2)};
     return exchange(c, cells, 7, faces, 15, x_ends, 5, z_faces, 5);
 }
-struct P2Args { ark_ctx* c; int axis; };
-hipError_t run_phase2_axis(void* p) {
-    P2Args* a = (P2Args*)p;
-    ark_ctx* c = a->c;
-    int d = a->axis;
-    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-    NextFaces nx;
-    nx.p[0] = c->alt[face_u(d, d)];
-    nx.p[1] = c->alt[face_u(d, ta)];
-    nx.p[2] = c->alt[face_u(d, tb)];
-    nx.p[3] = c->alt[face_p(d)];
-    nx.p[4] = c->alt[face_ro(d)];
-    nx.p[5] = c->alt[face_t(d)];
-    return launch_phase2(c->stream, d, c->prm.scheme, c->geo, c->fields(), nx, c->K, c->S,
-                         d == 0 && !c->prm.periodic[0]);
+hipError_t run_phase2_xyz(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    NextFaces nx[3];
+    for (int d = 0; d < 3; ++d) {
+        const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+        nx[d].p[0] = c->alt[face_u(d, d)];
+        nx[d].p[1] = c->alt[face_u(d, ta)];
+        nx[d].p[2] = c->alt[face_u(d, tb)];
+        nx[d].p[3] = c->alt[face_p(d)];
+        nx[d].p[4] = c->alt[face_ro(d)];
+        nx[d].p[5] = c->alt[face_t(d)];
+    }
+    return launch_phase2_xyz(c->stream, c->prm.scheme, c->geo, c->fields(), nx, c->K, c->S, !c->prm.periodic[0]);
+}
+hipError_t run_forces_use_forces(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_forces_use_forces(c->stream, c->geo, c->fields(), c->K, c->S);
+}
+hipError_t run_phase1_use_forces(void* p) {
+    ark_ctx* c = (ark_ctx*)p;
+    return launch_phase1_use_forces(c->stream, c->geo, c->fields(), c->K, c->S);
 }
 
 }  // namespace
@@ -575,13 +581,18 @@ ark_status ark_refresh_rho(ark_ctx* c) {
     return s;
 }
 
+// One step after TimeStepSize, in the reference's order (source/main.cpp:
+// 85-101): Phase1, StressTensor, UseForces, Phase2, Phase1, UseForces.  The
+// force and the first UseForces, and the second Phase1 and UseForces, each
+// run as one launch: those phases use only the same cell's values of the
+// phase before.
+static ark_status stress_tensor(ark_ctx* c, bool use_forces);
+static ark_status phase1(ark_ctx* c, bool use_forces);
 ark_status ark_phase_body(ark_ctx* c) {
-    ark_status s = ark_phase1(c);
-    if (s == ARK_OK) s = ark_stress_tensor(c);
-    if (s == ARK_OK) s = ark_use_forces(c);
+    ark_status s = phase1(c, false);
+    if (s == ARK_OK) s = stress_tensor(c, true);
     if (s == ARK_OK) s = ark_phase2(c);
-    if (s == ARK_OK) s = ark_phase1(c);
-    if (s == ARK_OK) s = ark_use_forces(c);
+    if (s == ARK_OK) s = phase1(c, true);
     return s;
 }
 
@@ -601,16 +612,24 @@ ark_status ark_time_step_size(ark_ctx* c, double* dt_out) {
     return s;
 }
 
-ark_status ark_phase1(ark_ctx* c) {
-    if (!c) return ARK_ERR_INVALID;
+// Phase1; with use_forces, also the UseForces that follows the second
+// Phase1 of a step, in the same launch.
+static ark_status phase1(ark_ctx* c, bool use_forces) {
     ark_status s = require_fluid(c);
     if (s == ARK_OK) s = drop_pending(c);
     if (s == ARK_OK) s = ark_refresh_rho(c);
-    return s == ARK_OK ? c->launch("phase1", run_phase1, c) : s;
+    if (s != ARK_OK) return s;
+    return use_forces ? c->launch("phase1_use_forces", run_phase1_use_forces, c) : c->launch("phase1", run_phase1, c);
 }
 
-ark_status ark_stress_tensor(ark_ctx* c) {
+ark_status ark_phase1(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
+    return phase1(c, false);
+}
+
+// StressTensor; with use_forces, also the UseForces that follows it in a
+// step, in the force's launch.
+static ark_status stress_tensor(ark_ctx* c, bool use_forces) {
     ark_status s = require_fluid(c);
     if (s == ARK_OK) s = drop_pending(c);
     if (s == ARK_OK && c->sigma_dirty) {
@@ -619,8 +638,14 @@ ark_status ark_stress_tensor(ark_ctx* c) {
     }
     if (s == ARK_OK) s = c->launch("ghost_stress", run_stress_ghost, c);
     if (s == ARK_OK) s = c->launch("stress_faces", run_stress_faces, c);
-    if (s == ARK_OK) s = c->launch("forces", run_forces, c);
+    if (s == ARK_OK)
+        s = use_forces ? c->launch("forces_use_forces", run_forces_use_forces, c) : c->launch("forces", run_forces, c);
     return s;
+}
+
+ark_status ark_stress_tensor(ark_ctx* c) {
+    if (!c) return ARK_ERR_INVALID;
+    return stress_tensor(c, false);
 }
 
 ark_status ark_use_forces(ark_ctx* c) {
@@ -635,11 +660,7 @@ ark_status ark_phase2(ark_ctx* c) {
     ark_status s = require_fluid(c);
     if (s == ARK_OK) s = drop_pending(c);
     if (s == ARK_OK) s = c->launch("ghost_phase2", run_phase2_ghost, c);
-    static const char* names[3] = {"phase2_x", "phase2_y", "phase2_z"};
-    for (int d = 0; d < 3 && s == ARK_OK; ++d) {
-        P2Args a{c, d};
-        s = c->launch(names[d], run_phase2_axis, &a);
-    }
+    if (s == ARK_OK) s = c->launch("phase2", run_phase2_xyz, c);
     if (s == ARK_OK) {
         for (int g = 1; g < 4; ++g) ark_swap_group(c, g);
         std::swap(c->cur[F_ro1], c->alt[F_ro1]);

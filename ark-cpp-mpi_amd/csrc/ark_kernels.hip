@@ -254,14 +254,9 @@ __global__ void __launch_bounds__(256) k_zero(double* const* arrs, int narr, lon
 }
 
 // Friction force (source/main.cpp:995-1023).
-__global__ void __launch_bounds__(256) k_forces(Geo g, Fields F, Consts K) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    int k = blockIdx.z + 1;
-    if (i > g.N1 || j > g.N2 || k > g.N3) return;
+__device__ __forceinline__ void forces_cell(const Fields& F, int c, int px, int sxy, const Consts& K) {
     double* const* a = F.a;
-    int c = at(g, i, j, k);
-    const int sx = 1, sy = g.px, sz = g.sxy;
+    const int sx = 1, sy = px, sz = sxy;
     if (a[F_condition][c] == 0) {
         double ro_c = a[F_roCon][c];
         a[F_f1][c] = ((a[F_sigm11][c + sx] - a[F_sigm11][c]) * K.ds1 +
@@ -280,14 +275,17 @@ __global__ void __launch_bounds__(256) k_forces(Geo g, Fields F, Consts K) {
     }
 }
 
-// UseForces (source/main.cpp:1026-1045).
-__global__ void __launch_bounds__(256) k_use_forces(Geo g, Fields F, Consts K, const Scalars* S) {
+__global__ void __launch_bounds__(256) k_forces(Geo g, Fields F, Consts K) {
     int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     int k = blockIdx.z + 1;
     if (i > g.N1 || j > g.N2 || k > g.N3) return;
+    forces_cell(F, at(g, i, j, k), g.px, g.sxy, K);
+}
+
+// UseForces (source/main.cpp:1026-1045).
+__device__ __forceinline__ void use_forces_cell(const Fields& F, int c, const Consts& K, const Scalars* S) {
     double* const* a = F.a;
-    int c = at(g, i, j, k);
     if (a[F_condition][c] == 0) {
         const double dt = S->dt, dvc = K.dvc;
         double ro_cn = a[F_ronCon][c];
@@ -295,6 +293,38 @@ __global__ void __launch_bounds__(256) k_use_forces(Geo g, Fields F, Consts K, c
         a[F_u2nCon][c] = (ro_cn * dvc * a[F_u2nCon][c] + 0.5 * dt * a[F_f2][c]) / (dvc * ro_cn);
         a[F_u3nCon][c] = (ro_cn * dvc * a[F_u3nCon][c] + 0.5 * dt * a[F_f3][c]) / (dvc * ro_cn);
     }
+}
+
+__global__ void __launch_bounds__(256) k_use_forces(Geo g, Fields F, Consts K, const Scalars* S) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 || j > g.N2 || k > g.N3) return;
+    use_forces_cell(F, at(g, i, j, k), K, S);
+}
+
+// The step's own sequences, one launch each where a phase only needs the
+// previous phase's values of the same cell: the force and UseForces after
+// the stresses, and Phase1 followed by UseForces (the forces of the step's
+// StressTensor).  Each cell's arithmetic is the separate kernels'.
+__global__ void __launch_bounds__(256) k_forces_use_forces(Geo g, Fields F, Consts K, const Scalars* S) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 || j > g.N2 || k > g.N3) return;
+    const int c = at(g, i, j, k);
+    forces_cell(F, c, g.px, g.sxy, K);
+    use_forces_cell(F, c, K, S);
+}
+
+__global__ void __launch_bounds__(256) k_phase1_use_forces(Geo g, Fields F, Consts K, const Scalars* S) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int k = blockIdx.z + 1;
+    if (i > g.N1 || j > g.N2 || k > g.N3) return;
+    const int c = at(g, i, j, k);
+    phase1_cell(F, c, g.px, g.sxy, S->dt, K);
+    use_forces_cell(F, c, K, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,11 +362,8 @@ __device__ __forceinline__ double x2_wall_t(const Geo& g, const Fields& F, int i
 }
 
 template <int D, bool GqUsesR>
-__global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, Consts K, const Scalars* S,
-                                                int inflow) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    int k = blockIdx.z + 1;
+__device__ __forceinline__ void phase2_face(const Geo& g, const Fields& F, const NextFaces& nx, const Consts& K,
+                                            const Scalars* S, int inflow, int i, int j, int k) {
     const int E1 = g.N1 + (D == 0), E2 = g.N2 + (D == 1), E3 = g.N3 + (D == 2);
     if (i > E1 || j > E2 || k > E3) return;
     constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
@@ -456,6 +483,19 @@ __global__ void __launch_bounds__(256) k_phase2(Geo g, Fields F, NextFaces nx, C
         nx.p[0][cf] = Un[cf]; nx.p[1][cf] = Ua[cf]; nx.p[2][cf] = Ub[cf];
         nx.p[3][cf] = P[cf]; nx.p[4][cf] = a[face_ro(D)][cf];
     }
+}
+
+// The three sweeps in one launch.  They are independent: each reads the
+// current cell and face arrays only and writes the next faces of its own
+// axis.  z-blocks [d (N3+1), (d+1)(N3+1)) run axis d.
+template <bool Quirk>
+__global__ void __launch_bounds__(256) k_phase2_xyz(Geo g, Fields F, NextFaces nx0, NextFaces nx1, NextFaces nx2,
+                                                    Consts K, const Scalars* S, int inflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int nz = g.N3 + 1, d = blockIdx.z / nz, k = blockIdx.z % nz + 1;
+    if (d == 0) phase2_face<0, Quirk>(g, F, nx0, K, S, inflow, i, j, k);
+    else if (d == 1) phase2_face<1, false>(g, F, nx1, K, S, 0, i, j, k);
+    else phase2_face<2, Quirk>(g, F, nx2, K, S, 0, i, j, k);
 }
 
 
@@ -702,25 +742,25 @@ hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, cons
     return hipGetLastError();
 }
 
-hipError_t launch_phase2(hipStream_t st, int axis, int scheme, const Geo& g, const Fields& F, const NextFaces& nx,
-                         const Consts& K, const Scalars* S, int inflow) {
-    // reference scheme: gq from R on the X1 and X3 sweeps; corrected: from Q everywhere
-    const bool quirk = scheme == 0;
-    if (axis == 0) {
-        if (quirk)
-            hipLaunchKernelGGL((k_phase2<0, true>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S,
-                               inflow);
-        else
-            hipLaunchKernelGGL((k_phase2<0, false>), cell_grid(g.N1 + 1, g.N2, g.N3), kBlock, 0, st, g, F, nx, K, S,
-                               inflow);
-    } else if (axis == 1) {
-        hipLaunchKernelGGL((k_phase2<1, false>), cell_grid(g.N1, g.N2 + 1, g.N3), kBlock, 0, st, g, F, nx, K, S, 0);
-    } else {
-        if (quirk)
-            hipLaunchKernelGGL((k_phase2<2, true>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S, 0);
-        else
-            hipLaunchKernelGGL((k_phase2<2, false>), cell_grid(g.N1, g.N2, g.N3 + 1), kBlock, 0, st, g, F, nx, K, S, 0);
-    }
+hipError_t launch_forces_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K,
+                                   const Scalars* S) {
+    hipLaunchKernelGGL(k_forces_use_forces, cell_grid(g.N1, g.N2, g.N3), kBlock, 0, st, g, F, K, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase1_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K,
+                                    const Scalars* S) {
+    hipLaunchKernelGGL(k_phase1_use_forces, cell_grid(g.N1, g.N2, g.N3), kBlock, 0, st, g, F, K, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase2_xyz(hipStream_t st, int scheme, const Geo& g, const Fields& F, const NextFaces nx[3],
+                             const Consts& K, const Scalars* S, int inflow) {
+    dim3 grid = cell_grid(g.N1 + 1, g.N2 + 1, 3 * (g.N3 + 1));
+    if (scheme == 0)
+        hipLaunchKernelGGL(k_phase2_xyz<true>, grid, kBlock, 0, st, g, F, nx[0], nx[1], nx[2], K, S, inflow);
+    else
+        hipLaunchKernelGGL(k_phase2_xyz<false>, grid, kBlock, 0, st, g, F, nx[0], nx[1], nx[2], K, S, inflow);
     return hipGetLastError();
 }
 

@@ -49,8 +49,15 @@ hipError_t launch_stress_faces(hipStream_t st, const Geo& g, const Fields& F, co
 hipError_t launch_zero(hipStream_t st, double* const* dev_ptrs, int narr, long long n);
 hipError_t launch_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
-hipError_t launch_phase2(hipStream_t st, int axis, int scheme, const Geo& g, const Fields& F, const NextFaces& nx,
-                         const Consts& K, const Scalars* S, int inflow);   // inflow: non-periodic X1 ends
+// The three Phase2 sweeps in one launch (nx[d]: the next faces of axis d).
+hipError_t launch_phase2_xyz(hipStream_t st, int scheme, const Geo& g, const Fields& F, const NextFaces nx[3],
+                             const Consts& K, const Scalars* S, int inflow);
+// Force then UseForces, and Phase1 then UseForces, per cell in one launch.
+hipError_t launch_forces_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K,
+                                   const Scalars* S);
+hipError_t launch_phase1_use_forces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K,
+                                    const Scalars* S);
+   // inflow: non-periodic X1 ends
 hipError_t launch_rho_faces(hipStream_t st, const Geo& g, const Fields& F, const Consts& K);
 hipError_t launch_diag(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, double* part, int nblocks,
                        double* out);

@@ -60,10 +60,13 @@ FUSED_KERNEL = {"fp64": "fused_step", "mixed": "fused_mixed"}
 # p, rho, t) and writes Con and nCon; stress_faces reads u1..3 and the
 # condition and writes 9 stresses; forces reads condition, rho and 9 stresses
 # and writes f1..3; use_forces reads condition, rho, u1..3, f1..3 and writes
-# u1..3; phase2_d reads 5 faces and 10 cell fields and writes 6 faces;
-# rho_faces reads and writes 3 face fields.
+# u1..3; phase2 (the three sweeps in one launch) reads 5 faces and 10 cell
+# fields and writes 6 faces per axis; rho_faces reads and writes 3 face fields.
+# A step's fused pairs: forces_use_forces reads condition, rho, rho_n, u1..3
+# and 9 stresses and writes f1..3 and u1..3; phase1_use_forces is phase1 plus
+# the reads of f1..3.
 PHASE_BYTES_PER_CELL = {"cfl": 24, "phase1": 264, "stress_faces": 104, "forces": 112, "use_forces": 88,
-                        "phase2_x": 168, "phase2_y": 168, "phase2_z": 168, "rho_faces": 48}
+                        "phase2": 3 * 168, "rho_faces": 48, "forces_use_forces": 168, "phase1_use_forces": 288}
 # Ghost fills move planes: 3 cell fields x 3 axes x 2 sides before the stress;
 # 7 cell fields and 5 face fields x 3 axes x 2 sides before Phase2 (read and
 # write, 16 B per element).
