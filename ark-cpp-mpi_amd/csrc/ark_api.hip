@@ -430,38 +430,47 @@ int periodic_mask(const ark_ctx* c) {
     return (c->prm.periodic[0] ? 1 : 0) | (c->prm.periodic[1] ? 2 : 0) | (c->prm.periodic[2] ? 4 : 0);
 }
 
-void add_plan(GhostPlan& plan, double* p, int is_face, int mask) {
-    if (mask == 0) return;
-    plan.p[plan.n] = p;
-    plan.is_face[plan.n] = is_face;
-    plan.axis_mask[plan.n] = mask;
-    plan.n++;
+// The entry of array p in the plan (added on first use).  The exchanges
+// below name at most 22 distinct arrays; a full plan reuses its last entry
+// and is refused by exchange().
+constexpr int kPlanMax = (int)(sizeof(ExchangePlan::p) / sizeof(double*));
+static int plan_entry(ExchangePlan& plan, double* p, int is_face) {
+    for (int q = 0; q < plan.n; ++q)
+        if (plan.p[q] == p) return q;
+    if (plan.n == kPlanMax) return kPlanMax - 1;
+    const int q = plan.n++;
+    plan.p[q] = p;
+    plan.is_face[q] = is_face;
+    plan.copy_mask[q] = plan.zero_mask[q] = plan.grad_mask[q] = 0;
+    return q;
 }
 
 // The exchange of the given cell arrays (every axis) and face arrays (pairs
 // of field and axis: along their own axis), then, on non-periodic X1/X3, the
 // zero-gradient ends of the arrays the reference sets there (x_ends on X1;
-// the cell arrays and z_faces on X3).
+// the cell arrays and z_faces on X3), as one launch.  A zero-gradient end
+// replaces the zero fill of the same planes.
 hipError_t exchange(ark_ctx* c, const int* cells, int ncells, const int* faces, int nfaces, const int* x_ends,
                     int nx_ends, const int* z_faces, int nz_faces) {
     const int per = periodic_mask(c), non = 7 & ~per;
-    GhostPlan copy{}, zero{}, grad{};
+    ExchangePlan plan{};
     for (int q = 0; q < ncells; ++q) {
-        add_plan(copy, c->cur[cells[q]], 0, per);
-        add_plan(zero, c->cur[cells[q]], 0, non);
+        const int e = plan_entry(plan, c->cur[cells[q]], 0);
+        plan.copy_mask[e] |= per;
+        plan.zero_mask[e] |= non;
     }
     for (int q = 0; q < nfaces; ++q) {
         const int d = faces[2 * q + 1];
-        add_plan(copy, c->cur[faces[2 * q]], 1, per & (1 << d));
-        add_plan(zero, c->cur[faces[2 * q]], 1, non & (1 << d));
+        const int e = plan_entry(plan, c->cur[faces[2 * q]], 1);
+        plan.copy_mask[e] |= per & (1 << d);
+        plan.zero_mask[e] |= non & (1 << d);
     }
-    for (int q = 0; q < nx_ends; ++q) add_plan(grad, c->cur[x_ends[q]], 0, non & 1);
-    for (int q = 0; q < ncells; ++q) add_plan(grad, c->cur[cells[q]], 0, non & 4);
-    for (int q = 0; q < nz_faces; ++q) add_plan(grad, c->cur[z_faces[q]], 1, non & 4);
-    hipError_t e = launch_ghost(c->stream, c->geo, copy, 0);
-    if (e == hipSuccess) e = launch_ghost(c->stream, c->geo, zero, 1);
-    if (e == hipSuccess) e = launch_zero_gradient(c->stream, c->geo, grad);
-    return e;
+    for (int q = 0; q < nx_ends; ++q) plan.grad_mask[plan_entry(plan, c->cur[x_ends[q]], 0)] |= non & 1;
+    for (int q = 0; q < ncells; ++q) plan.grad_mask[plan_entry(plan, c->cur[cells[q]], 0)] |= non & 4;
+    for (int q = 0; q < nz_faces; ++q) plan.grad_mask[plan_entry(plan, c->cur[z_faces[q]], 1)] |= non & 4;
+    if (ncells + nfaces > kPlanMax) return hipErrorInvalidValue;
+    for (int q = 0; q < plan.n; ++q) plan.zero_mask[q] &= ~plan.grad_mask[q];
+    return launch_exchange(c->stream, c->geo, plan);
 }
 
 hipError_t run_stress_ghost(void* p) {

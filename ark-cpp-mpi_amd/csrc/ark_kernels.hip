@@ -132,65 +132,57 @@ __global__ void __launch_bounds__(256) k_phase1(Geo g, Fields F, Consts K, const
 }
 
 // ---------------------------------------------------------------------------
-// Periodic ghost fill: the single-GPU image of the reference's MPI halo
-// exchange with periods={1,1,1} (source/main.cpp:619-653, 688-719, 754-785,
-// 1101-1189, 1296-1384, 1491-1579).  For a cell array along axis d:
-// plane 0 <- plane N_d, plane N_d+1 <- plane 1; for a face array along its own
-// axis: face 0 <- face N_d, face N_d+2 <- face 2.  Only the interior of the
-// two other axes is written (no edges or corners), and no written element is
-// a source of another: the whole plan runs as one launch.
+// Ghost fill of one exchange, one launch.  Per array and axis d, one of:
 //
-// With zero set, the same planes are set to 0 instead: the exchange along a
-// non-periodic axis of one rank, whose neighbour is MPI_PROC_NULL, copies a
-// receive buffer that nothing has written (the reference's buffers are
-// uninitialised, Arr2d.cpp:7-12; zero-filled buffers define the result).
-__global__ void __launch_bounds__(256) k_ghost(Geo g, GhostPlan plan, int zero) {
-    int a = blockIdx.y, d = blockIdx.z;
-    if (a >= plan.n || !((plan.axis_mask[a] >> d) & 1)) return;
+// copy: the single-GPU image of the reference's MPI halo exchange with
+// periods={1,1,1} (source/main.cpp:619-653, 688-719, 754-785, 1101-1189,
+// 1296-1384, 1491-1579).  For a cell array: plane 0 <- plane N_d, plane
+// N_d+1 <- plane 1; for a face array along its own axis: face 0 <- face N_d,
+// face N_d+2 <- face 2.
+//
+// zero: the same planes set to 0.  The exchange along a non-periodic axis
+// of one rank, whose neighbour is MPI_PROC_NULL, copies a receive buffer that
+// nothing has written (the reference's buffers are uninitialised,
+// Arr2d.cpp:7-12; zero-filled buffers define the result).
+//
+// grad: the zero-gradient ends that the reference's physical-boundary blocks
+// write after that exchange (source/main.cpp:803-845 and 1633-1698): cell
+// planes 0 <- 1 and N+1 <- N, face planes 0 <- 1 and N+2 <- N+1.  They
+// overwrite exactly the planes the zero fill writes, so the host drops the
+// zero fill wherever they apply.
+//
+// Only the interior of the two other axes is written (no edges or corners),
+// and every read is of an interior plane, which no mode writes: the modes
+// and arrays are independent and run together.
+__global__ void __launch_bounds__(256) k_exchange(Geo g, ExchangePlan plan) {
+    const int a = blockIdx.y, d = blockIdx.z;
+    if (a >= plan.n) return;
+    const int bit = 1 << d;
+    const int mode = (plan.copy_mask[a] & bit) ? 0 : (plan.zero_mask[a] & bit) ? 1 : (plan.grad_mask[a] & bit) ? 2 : -1;
+    if (mode < 0) return;
     const int N[3] = {g.N1, g.N2, g.N3};
     const int st[3] = {1, g.px, g.sxy};
-    int o1 = d == 0 ? 1 : 0;           // x-fastest enumeration of the plane
-    int o2 = d == 2 ? 1 : 2;
-    int plane = N[o1] * N[o2];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int o1 = d == 0 ? 1 : 0;   // x-fastest enumeration of the plane
+    const int o2 = d == 2 ? 1 : 2;
+    const int plane = N[o1] * N[o2];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * plane) return;
-    int side = t >= plane;
-    int r = t - side * plane;
-    int q = r % N[o1] + 1, p = r / N[o1] + 1;
+    const int side = t >= plane;
+    const int r = t - side * plane;
+    const int q = r % N[o1] + 1, p = r / N[o1] + 1;
     double* A = plan.p[a];
-    int base = q * st[o1] + p * st[o2];
-    if (zero) {
-        A[base + (side == 0 ? 0 : (N[d] + 1 + plan.is_face[a]) * st[d])] = 0.0;
-    } else if (!plan.is_face[a]) {
+    const int f = plan.is_face[a];
+    const int base = q * st[o1] + p * st[o2];
+    const int hi = (N[d] + 1 + f) * st[d];   // the far ghost plane
+    if (mode == 1) {
+        A[base + (side == 0 ? 0 : hi)] = 0.0;
+    } else if (mode == 0) {
         if (side == 0) A[base] = A[base + N[d] * st[d]];
-        else A[base + (N[d] + 1) * st[d]] = A[base + st[d]];
+        else A[base + hi] = A[base + (1 + f) * st[d]];
     } else {
-        if (side == 0) A[base] = A[base + N[d] * st[d]];
-        else A[base + (N[d] + 2) * st[d]] = A[base + 2 * st[d]];
+        if (side == 0) A[base] = A[base + st[d]];
+        else A[base + hi] = A[base + (N[d] + f) * st[d]];
     }
-}
-
-// Zero-gradient ends of a non-periodic axis (the physical-boundary blocks of
-// source/main.cpp:803-845 and 1633-1698): cell planes 0 <- 1 and N+1 <- N, face
-// planes 0 <- 1 and N+2 <- N+1, over the interior of the two other axes.
-__global__ void __launch_bounds__(256) k_zero_gradient(Geo g, GhostPlan plan) {
-    int a = blockIdx.y, d = blockIdx.z;
-    if (a >= plan.n || !((plan.axis_mask[a] >> d) & 1)) return;
-    const int N[3] = {g.N1, g.N2, g.N3};
-    const int st[3] = {1, g.px, g.sxy};
-    int o1 = d == 0 ? 1 : 0;
-    int o2 = d == 2 ? 1 : 2;
-    int plane = N[o1] * N[o2];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * plane) return;
-    int side = t >= plane;
-    int r = t - side * plane;
-    int q = r % N[o1] + 1, p = r / N[o1] + 1;
-    double* A = plan.p[a];
-    int base = q * st[o1] + p * st[o2];
-    const int last = N[d] + plan.is_face[a];     // last interior plane: cell N, face N+1
-    if (side == 0) A[base] = A[base + st[d]];
-    else A[base + (last + 1) * st[d]] = A[base + last * st[d]];
 }
 
 // ---------------------------------------------------------------------------
@@ -710,15 +702,9 @@ static dim3 plane_grid(const Geo& g, int n) {
     return dim3((2 * m + 255) / 256, n, 3);
 }
 
-hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan, int zero) {
+hipError_t launch_exchange(hipStream_t st, const Geo& g, const ExchangePlan& plan) {
     if (plan.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ghost, plane_grid(g, plan.n), dim3(256), 0, st, g, plan, zero);
-    return hipGetLastError();
-}
-
-hipError_t launch_zero_gradient(hipStream_t st, const Geo& g, const GhostPlan& plan) {
-    if (plan.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zero_gradient, plane_grid(g, plan.n), dim3(256), 0, st, g, plan);
+    hipLaunchKernelGGL(k_exchange, plane_grid(g, plan.n), dim3(256), 0, st, g, plan);
     return hipGetLastError();
 }
 

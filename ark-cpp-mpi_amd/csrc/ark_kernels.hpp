@@ -6,10 +6,13 @@
 namespace arkd {
 
 // A list of arrays to ghost-fill: axis_mask bit d set = fill along axis d.
-struct GhostPlan {
+// The arrays of one ghost fill and, per array, the axes (bit d) that are
+// copied periodically, set to zero, or given zero-gradient ends.  An axis is
+// in at most one of the three masks.
+struct ExchangePlan {
     double* p[24];
     int is_face[24];
-    int axis_mask[24];
+    int copy_mask[24], zero_mask[24], grad_mask[24];
     int n;
 };
 
@@ -33,8 +36,7 @@ hipError_t launch_cfl(hipStream_t st, const Geo& g, const Fields& F, double c1, 
 hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
 hipError_t launch_clock(hipStream_t st, Scalars* S, int what);   // 0: nStep++, 1: TIME += 0.5 dt
 hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
-hipError_t launch_ghost(hipStream_t st, const Geo& g, const GhostPlan& plan, int zero);   // zero: non-periodic exchange
-hipError_t launch_zero_gradient(hipStream_t st, const Geo& g, const GhostPlan& plan);
+hipError_t launch_exchange(hipStream_t st, const Geo& g, const ExchangePlan& plan);
 hipError_t launch_vorticity(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, bool float_faces,
                             double* o1, double* o2, double* o3);
 hipError_t launch_average(hipStream_t st, const Geo& g, const Fields& F, const Scalars* S, double* a1, double* a2,
