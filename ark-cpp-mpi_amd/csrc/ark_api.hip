@@ -320,9 +320,14 @@ static ark_status copy_planes(ark_ctx* c, double* dev, const double* host_src, d
     return s == ARK_OK ? c->hip(hipStreamSynchronize(c->stream), "sync after copy") : s;
 }
 
+static ark_status drop_pending(ark_ctx* c);
+
 ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2, int e3) {
     if (!c || !host) return ARK_ERR_INVALID;
     ark_status s = check_extents(c, field, e1, e2, e3);
+    if (s != ARK_OK) return s;
+    // TimeStepSize reads these: a dt left pending by fused steps no longer holds
+    if (field == F_u1Con || field == F_u2Con || field == F_u3Con) s = drop_pending(c);
     if (s != ARK_OK) return s;
     if (field == F_condition) {
         long long n = (long long)e1 * e2 * e3;
@@ -882,6 +887,7 @@ ark_status ark_finish_init(ark_ctx* c) {
     c->rho_consistent = true;
     c->rho_stored = true;
     c->sigma_dirty = false;
+    c->dt_pending = false;   // the reset below discards a MIN left by fused steps
     if (s == ARK_OK) {
         // fresh run: TIME = 0, nStep = 0, running MIN reset
         // (ordered on the context's stream, after the kernels that set dt_init)
