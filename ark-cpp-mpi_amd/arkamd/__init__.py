@@ -474,7 +474,7 @@ class Solver:
         return out
 
 
-BATCH_SHAPES = ["30x6", "30x14", "62x6"]
+BATCH_SHAPES = ["30x6", "30x14", "62x6", "32x13"]
 BATCH_LZ = [16, 32, 64]
 
 
@@ -504,9 +504,10 @@ class Batch:
         self._check(self._L.ark_batch_synchronize(self._h))
 
     def autotune(self, reps: int = 3) -> Dict[tuple, float]:
-        ms = (ctypes.c_double * 9)()
-        self._check(self._L.ark_batch_autotune(self._h, int(reps), ms, 9))
-        return {(BATCH_SHAPES[i // 3], BATCH_LZ[i % 3]): ms[i] for i in range(9)}
+        n = 3 * len(BATCH_SHAPES)
+        ms = (ctypes.c_double * n)()
+        self._check(self._L.ark_batch_autotune(self._h, int(reps), ms, n))
+        return {(BATCH_SHAPES[i // 3], BATCH_LZ[i % 3]): ms[i] for i in range(n)}
 
     def set_config(self, shape: int, lz: int) -> None:
         self._check(self._L.ark_batch_set_config(self._h, int(shape), int(lz)))

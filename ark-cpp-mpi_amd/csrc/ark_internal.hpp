@@ -55,8 +55,11 @@ struct ark_ctx {
     float* les_Y = nullptr;
     struct ark_closure_hp* closure_hp = nullptr;   // scale-selective application (ark_set_closure_cutoff)
     double* average = nullptr;         // AverageValues' three accumulators (cell fields, common index map)
-    void* mixed_units = nullptr;       // chunk table of the mixed-precision step (int4 per chunk)
-    int mixed_nunits = 0;
+    // chunk tables of the mixed-precision step, one per single-box tile shape
+    // (int4 per chunk), and the shape in use (-1: not chosen yet)
+    void* mixed_units[2] = {nullptr, nullptr};
+    int mixed_nunits[2] = {0, 0};
+    int mixed_shape = -1;
     void* fused_units = nullptr;       // chunk table of the FP64 fused step
     int fused_nunits = 0;
     int diag_blocks = 0;

@@ -58,6 +58,7 @@ struct Shape {
 using Shape30x6 = Shape<30, 6>;
 using Shape30x14 = Shape<30, 14>;
 using Shape62x6 = Shape<62, 6>;
+using Shape32x13 = Shape<32, 13>;   // 34 x 15 = 510 threads: whole 32-cell rows owned
 
 struct Args {
     Geo g;
@@ -643,10 +644,10 @@ using namespace arkd;
 
 namespace {
 
-// Tile shapes a launch can use (mixed::Shape): owned 30x6, 30x14, 62x6.
-constexpr int kShapes = 3;
-constexpr int kShapeTX[kShapes] = {30, 30, 62}, kShapeTY[kShapes] = {6, 14, 6};
-constexpr int kShapeThreads[kShapes] = {256, 512, 512};
+// Tile shapes a launch can use (mixed::Shape): owned 30x6, 30x14, 62x6, 32x13.
+constexpr int kShapes = 4;
+constexpr int kShapeTX[kShapes] = {30, 30, 62, 32}, kShapeTY[kShapes] = {6, 14, 6, 13};
+constexpr int kShapeThreads[kShapes] = {256, 512, 512, 510};
 constexpr int kDefaultShape = 1;
 
 int tiles_of(const Geo& g, int shape, int LZ) {
@@ -726,26 +727,81 @@ hipError_t launch_shape_batch(int shape, bool quirk, dim3 grid, hipStream_t st, 
                               const int* start, int n) {
     if (shape == 0) launch_batch<mixed::Shape30x6>(quirk, grid, st, boxes, start, n);
     else if (shape == 1) launch_batch<mixed::Shape30x14>(quirk, grid, st, boxes, start, n);
+    else if (shape == 3) launch_batch<mixed::Shape32x13>(quirk, grid, st, boxes, start, n);
     else launch_batch<mixed::Shape62x6>(quirk, grid, st, boxes, start, n);
     return hipGetLastError();
 }
 
+// The single-box step's two tile shapes: 30x14 owned (32 x 16 threads) and
+// 32x13 owned (34 x 15).  Which is faster depends on the box: at 256^3 the
+// 32-wide tiles cover each row with 8 tiles instead of 9 (1.06 against
+// 1.12 ms), at 320^3 they are slower (2.09 against 1.88 ms).
 using SingleShape = mixed::Shape30x14;
+using SingleWide = mixed::Shape32x13;
+
+void launch_single_shape(const ark_ctx* c, int shape, const mixed::Args& a) {
+    const bool quirk = c->prm.scheme == ARK_SCHEME_REFERENCE;
+    if (shape == 1)
+        launch_single<SingleWide>(quirk, c->mixed_nunits[1], (const int4*)c->mixed_units[1], c->stream, a);
+    else
+        launch_single<SingleShape>(quirk, c->mixed_nunits[0], (const int4*)c->mixed_units[0], c->stream, a);
+}
 
 hipError_t run_mixed(void* p) {
     ark_ctx* c = (ark_ctx*)p;
-    const mixed::Args a = make_args(c, 0, &c->S->dt_bits);
-    launch_single<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE, c->mixed_nunits, (const int4*)c->mixed_units,
-                               c->stream, a);
+    launch_single_shape(c, c->mixed_shape, make_args(c, 0, &c->S->dt_bits));
     return hipGetLastError();
 }
 
-// The chunk table of the single-box step, built once per context.
+// The chunk tables of the single-box step, built once per context.
 ark_status ensure_units(ark_ctx* c) {
-    if (c->mixed_units) return ARK_OK;
-    const std::vector<int4> u =
-        mixed_units<SingleShape>(c->geo, resident_blocks<SingleShape>(c->prm.scheme == ARK_SCHEME_REFERENCE));
-    return ark_upload_chunks(c, u, &c->mixed_units, &c->mixed_nunits);
+    if (c->mixed_units[0]) return ARK_OK;
+    const bool quirk = c->prm.scheme == ARK_SCHEME_REFERENCE;
+    ark_status s = ark_upload_chunks(c, mixed_units<SingleShape>(c->geo, resident_blocks<SingleShape>(quirk)),
+                                     &c->mixed_units[0], &c->mixed_nunits[0]);
+    if (s == ARK_OK)
+        s = ark_upload_chunks(c, mixed_units<SingleWide>(c->geo, resident_blocks<SingleWide>(quirk)),
+                              &c->mixed_units[1], &c->mixed_nunits[1]);
+    return s;
+}
+
+// Times both single-box shapes on this context (one launch, then kReps
+// launches each) and keeps the faster.  The timing launches write only the
+// buffers the step is about to overwrite and send their dt MIN to a scratch
+// slot, so the state is untouched; every cell's arithmetic is the same in
+// either tiling, so the choice changes only the speed.
+ark_status tune_single_shape(ark_ctx* c) {
+    constexpr int kReps = 3;
+    unsigned long long* scratch = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ark_status s = c->hip(hipMalloc(&scratch, sizeof(unsigned long long)), "hipMalloc tuning slot");
+    if (s == ARK_OK) s = c->hip(hipEventCreate(&e0), "event");
+    if (s == ARK_OK) s = c->hip(hipEventCreate(&e1), "event");
+    float best = 0.f;
+    int shape = 0;
+    const mixed::Args a = make_args(c, 0, scratch);
+    for (int sh = 0; sh < 2 && s == ARK_OK; ++sh) {
+        launch_single_shape(c, sh, a);
+        s = c->hip(hipGetLastError(), "tuning launch");
+        if (s == ARK_OK) s = c->hip(hipEventRecord(e0, c->stream), "event");
+        for (int r = 0; r < kReps && s == ARK_OK; ++r) {
+            launch_single_shape(c, sh, a);
+            s = c->hip(hipGetLastError(), "tuning launch");
+        }
+        if (s == ARK_OK) s = c->hip(hipEventRecord(e1, c->stream), "event");
+        if (s == ARK_OK) s = c->hip(hipEventSynchronize(e1), "event sync");
+        float ms = 0.f;
+        if (s == ARK_OK) s = c->hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (s == ARK_OK && (sh == 0 || ms < best)) {
+            best = ms;
+            shape = sh;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (scratch) (void)hipFree(scratch);
+    if (s == ARK_OK) c->mixed_shape = shape;
+    return s;
 }
 
 int face_mode(int f) { return (f == F_t1 || f == F_t2 || f == F_t3) ? 1 : 0; }
@@ -829,6 +885,8 @@ ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     s = ensure_units(c);
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
+        // the first step of a context picks its tile shape, with this step's dt
+        if (s == ARK_OK && c->mixed_shape < 0) s = tune_single_shape(c);
         if (s == ARK_OK) s = ark_turb_force_step(c, c->stream);
         if (s == ARK_OK) s = ark_closure_step(c, c->stream);
         if (s == ARK_OK) s = c->launch("fused_mixed", run_mixed, c);

@@ -147,7 +147,9 @@ ark_status ark_phase2(ark_ctx* ctx);
  * 82-91: TimeStepSize, Phase1, StressTensor, UseForces, Phase2, Phase1,
  * UseForces) through the fused kernels.  Results are bitwise equal to the
  * phase-by-phase calls in FP64 mode.  Nothing synchronises: dt stays on the
- * device between steps. */
+ * device between steps.  In mixed precision the first step of a context also
+ * times its two tile shapes (a few launches that leave the state unchanged)
+ * and keeps the faster; results do not depend on the shape. */
 ark_status ark_step(ark_ctx* ctx, int nsteps);
 
 /* nsteps through the fused kernels only, maintaining the persistent state:
@@ -234,8 +236,8 @@ ark_status ark_batch_step(ark_batch* batch, int nsteps);
 ark_status ark_batch_synchronize(ark_batch* batch);
 /* Times every (tile shape, chunk length) candidate on the batch's own boxes,
  * reps launches each, and keeps the fastest.  Tile shapes: 0 = 30x6 owned
- * (256 threads), 1 = 30x14 (512), 2 = 62x6 (512); chunk lengths 16, 32, 64
- * planes.  ms_out[shape*3 + lz_index] receives each candidate's ms per step
+ * (256 threads), 1 = 30x14 (512), 2 = 62x6 (512), 3 = 32x13 (510); chunk
+ * lengths 16, 32, 64 planes.  ms_out[shape*3 + lz_index] receives each candidate's ms per step
  * (max_out entries).  The boxes' state is not changed. */
 ark_status ark_batch_autotune(ark_batch* batch, int reps, double* ms_out, int max_out);
 ark_status ark_batch_set_config(ark_batch* batch, int shape, int lz);

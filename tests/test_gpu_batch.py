@@ -43,7 +43,7 @@ def test_batch_equals_standalone():
     assert_same(ref, bat, "batch vs standalone")
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 @pytest.mark.parametrize("lz", [16, 64])
 def test_batch_result_independent_of_tiling(shape, lz):
     ref, bat = make(BOXES[:2]), make(BOXES[:2])
@@ -79,7 +79,7 @@ def test_batch_autotune_leaves_state_unchanged():
     with arkamd.Batch(bat) as b:
         b.step(1)
         ms = b.autotune(reps=2)
-        assert len(ms) == 9 and all(v > 0 for v in ms.values())
+        assert len(ms) == 3 * len(arkamd.BATCH_SHAPES) and all(v > 0 for v in ms.values())
         best = min(ms, key=ms.get)
         cfg = b.config()
         assert (cfg["shape"], cfg["lz"]) == best
