@@ -33,11 +33,13 @@
 
 namespace arkd {
 
-// 30x14 owned columns plus the ring: 32x16 = 512 threads, eight full
-// wavefronts, two per SIMD (256 VGPRs).  A multiple of four wavefronts keeps
-// the four SIMDs of a CU equally loaded.  30x14 covers a 256^2 plane with
-// 1.34 threads per owned column (62x6: 1.68).
-constexpr int TX = 30, TY = 14;
+// 32x13 owned columns plus the ring: 34x15 = 510 threads, eight wavefronts
+// (the last one 62 lanes), two per SIMD (256 VGPRs).  Eight wavefronts keep
+// the four SIMDs of a CU equally loaded.  32x13 covers a 256^2 plane with 160
+// tiles, 1.25 threads per owned column (30x14: 171 tiles, 1.34; 62x6: 1.68),
+// and the step at 256^3 in 2.21 ms (30x14: 2.27), at 320^3 in 4.18 ms (4.28);
+// at 384^3 the two are within 0.5%.
+constexpr int TX = 32, TY = 13;
 constexpr int RX = TX + 2, RY = TY + 2;
 constexpr int NTHR = RX * RY;
 
