@@ -44,7 +44,7 @@ SCHEME_REFERENCE, SCHEME_CORRECTED = 0, 1
 
 EXPORTED_SYMBOLS = [
     "ark_create", "ark_destroy", "ark_last_error", "ark_upload", "ark_download", "ark_set_dt",
-    "ark_get_dt", "ark_next_step", "ark_half_time", "ark_set_closure_cutoff", "ark_time_step_size", "ark_phase1", "ark_stress_tensor", "ark_use_forces",
+    "ark_get_dt", "ark_next_step", "ark_half_time", "ark_set_closure_cutoff", "ark_set_tile_shape", "ark_get_tile_shape", "ark_time_step_size", "ark_phase1", "ark_stress_tensor", "ark_use_forces",
     "ark_phase2", "ark_step", "ark_advance", "ark_step_phases", "ark_diagnostics", "ark_init_tgv",
     "ark_synchronize", "ark_get_stream", "ark_profile_enable", "ark_profile_read",
     "ark_profile_reset", "ark_get_plane", "ark_set_plane", "ark_boundary_conditions", "ark_device_bytes",
@@ -164,6 +164,8 @@ def lib() -> ctypes.CDLL:
     L.ark_mlp_predict.argtypes = [vp, fp, fp, ll]
     L.ark_set_closure.argtypes = [vp, vp, ctypes.c_double]
     L.ark_set_closure_cutoff.argtypes = [vp, ip]
+    L.ark_set_tile_shape.argtypes = [vp, ip]
+    L.ark_get_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.ark_write_paraview.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     L.ark_average_values.argtypes = [vp]
     L.ark_init_channel.argtypes = [vp, ctypes.c_double]
@@ -393,6 +395,18 @@ class Solver:
         """Attach a trained MLP (or None) as the subgrid closure."""
         self._closure = mlp
         self._check(self._L.ark_set_closure(self._h, mlp._h if mlp is not None else None, float(delta)))
+
+    @property
+    def tile_shape(self) -> int:
+        """Mixed-precision tile shape: -1 (chosen by timing at the next step),
+        0 (30x14 owned) or 1 (32x13); results are the same for every shape."""
+        v = ctypes.c_int()
+        self._check(self._L.ark_get_tile_shape(self._h, ctypes.byref(v)))
+        return v.value
+
+    @tile_shape.setter
+    def tile_shape(self, shape: int) -> None:
+        self._check(self._L.ark_set_tile_shape(self._h, int(shape)))
 
     def set_closure_cutoff(self, kcut: int) -> None:
         """Apply only the shells round(|m|) >= kcut of the closure's

@@ -878,6 +878,19 @@ ark_status ark_upload_chunks(ark_ctx* c, const std::vector<int4>& u, void** dst,
     return ARK_OK;
 }
 
+ark_status ark_set_tile_shape(ark_ctx* c, int shape) {
+    if (!c) return ARK_ERR_INVALID;
+    if (shape < -1 || shape > 1) return c->fail(ARK_ERR_INVALID, "tile shape must be -1 (timed), 0 (30x14) or 1 (32x13)");
+    c->mixed_shape = shape;
+    return ARK_OK;
+}
+
+ark_status ark_get_tile_shape(const ark_ctx* c, int* shape) {
+    if (!c || !shape) return ARK_ERR_INVALID;
+    *shape = c->mixed_shape;
+    return ARK_OK;
+}
+
 ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)
         return c->fail(ARK_ERR_UNSUPPORTED, "mixed precision needs N1, N2, N3 >= 2");

@@ -93,8 +93,9 @@ def time_steps(N, steps, warmup, precision, scheme, profile_steps):
         prof = s.profile_read()
         s.profile(False)
     dev_bytes = s.device_bytes()
+    tile = ("30x14", "32x13")[s.tile_shape] if prec == arkamd.PREC_MIXED else "32x13"
     s.close()
-    return elapsed, diag, prof, dev_bytes
+    return elapsed, diag, prof, dev_bytes, tile
 
 
 def roofline_of(prof, precision, cells):
@@ -446,9 +447,9 @@ def main():
     for prec in ("mixed", "fp64"):
         if args.only and prec != args.only:
             continue
-        elapsed, diag, prof, dev_bytes = time_steps(N, args.steps, args.warmup, prec, arkamd.SCHEME_REFERENCE,
-                                                    args.profile_steps)
-        runs[prec] = {"elapsed": elapsed, "diag": diag, "prof": prof, "device_bytes": dev_bytes,
+        elapsed, diag, prof, dev_bytes, tile = time_steps(N, args.steps, args.warmup, prec,
+                                                          arkamd.SCHEME_REFERENCE, args.profile_steps)
+        runs[prec] = {"elapsed": elapsed, "diag": diag, "prof": prof, "device_bytes": dev_bytes, "tile": tile,
                       "value": cells * args.steps / elapsed / 1e6, "ms_per_step": elapsed * 1e3 / args.steps,
                       "roofline": roofline_of(prof, prec, cells)}
 
@@ -463,7 +464,7 @@ def main():
         "data": "synthetic (Taylor-Green vortex initial condition, generated on device)",
         "config": {"workload": f"Taylor-Green vortex {N}^3 periodic, Re=1600, c=10, CFL 0.3, "
                                f"{args.steps} steps, {head} precision, reference scheme",
-                   "grid": [N, N, N], "precision": head},
+                   "grid": [N, N, N], "precision": head, "tile_owned_columns": H["tile"]},
         "roofline": H["roofline"],
     }
     if "fp64" in runs and head != "fp64":

@@ -152,6 +152,14 @@ ark_status ark_phase2(ark_ctx* ctx);
  * and keeps the faster; results do not depend on the shape. */
 ark_status ark_step(ark_ctx* ctx, int nsteps);
 
+/* Tile shape of the mixed-precision step: -1 = chosen by timing at the next
+ * step (the default), 0 = 30x14 owned columns, 1 = 32x13.  Every shape gives
+ * the same results bit for bit; only the speed differs.  No counterpart in
+ * the reference. */
+ark_status ark_set_tile_shape(ark_ctx* ctx, int shape);
+/* The tile shape in use (-1 before the first mixed-precision step chose it). */
+ark_status ark_get_tile_shape(const ark_ctx* ctx, int* shape);
+
 /* nsteps through the fused kernels only, maintaining the persistent state:
  * the level-n cell fields (ronCon, u1nCon, u2nCon, u3nCon, tnCon) on the
  * interior cells, the face fields u*, p*, t* on the faces 1..N_d+1 along

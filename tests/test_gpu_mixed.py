@@ -5,6 +5,7 @@ reference.  Tolerances are written per test.
 import numpy as np
 import pytest
 
+from conftest import bits_equal
 import arkamd
 
 pytestmark = pytest.mark.gpu
@@ -142,3 +143,26 @@ def test_mixed_reference_scheme_before_instability():
     f64, dmix = r[arkamd.PREC_FP64], r[arkamd.PREC_MIXED]
     for n in ("ronCon", "u1nCon", "u2nCon", "u11", "u22"):
         assert rel_l2(dmix[0][n], f64[0][n]) < 1e-5, n
+
+
+@pytest.mark.parametrize("kind", ["tgv", "forced"])
+def test_mixed_tile_shapes_bitwise_equal(kind):
+    """The single-box step's tile shapes (30x14 and 32x13 owned columns)
+    compute every cell the same way: forced shapes give the same state bit
+    for bit, and a context left to time them takes one of the two."""
+    N = (70, 46, 34)   # even sides (the turbulence initialiser's transforms); ragged against both tilings
+    outs = []
+    for shape in (0, 1, -1):
+        with arkamd.Solver(*N, VIS=1 / 800, precision=arkamd.PREC_MIXED, scheme=arkamd.SCHEME_CORRECTED) as s:
+            if kind == "tgv":
+                s.init_tgv(1.0)
+            else:
+                s.init_turbulence(7, 3.0, 1.0)
+                s.set_forcing(8, 0.1, 0.0)
+            s.tile_shape = shape
+            s.step(6)
+            assert s.tile_shape in (0, 1) and (shape < 0 or s.tile_shape == shape)
+            outs.append({n: s.download(n) for n in ("ronCon", "u1nCon", "u3nCon", "tnCon", "p1", "u22", "t3")})
+    for n in outs[0]:
+        assert bits_equal(outs[0][n], outs[1][n]), n
+        assert bits_equal(outs[0][n], outs[2][n]), n
