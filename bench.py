@@ -427,8 +427,8 @@ def main():
                     help="steps of the CPU baseline's 256^3 sample (about 12 s on 16 host cores)")
     ap.add_argument("--cpu-ranks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
-                    help="time one precision only (profiling runs)")
+    ap.add_argument("--only", choices=["fp64", "mixed", "channel"], default=None,
+                    help="time one precision only, or only the reference's channel (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
 
@@ -441,6 +441,9 @@ def main():
         return
 
     import arkamd
+    if args.only == "channel":
+        print(json.dumps({"reference_channel": reference_channel(args.channel_steps, 0)}))
+        return
     N = args.size
     cells = N ** 3
     runs = {}
