@@ -55,6 +55,7 @@ EXPORTED_SYMBOLS = [
     "ark_mlp_get_params", "ark_mlp_set_normalization", "ark_mlp_set_data", "ark_mlp_gradient", "ark_mlp_train",
     "ark_mlp_predict", "ark_set_closure", "ark_init_velocity",
     "ark_write_paraview", "ark_average_values", "ark_write_average_values", "ark_init_channel",
+    "ark_get_clock", "ark_restart",
 ]
 
 
@@ -83,6 +84,19 @@ class Diag(ctypes.Structure):
         ("nstep", ctypes.c_int64),
     ]
 
+
+class Clock(ctypes.Structure):
+    _fields_ = [
+        ("time", ctypes.c_double), ("nstep", ctypes.c_int64), ("dt", ctypes.c_double),
+        ("dt_next", ctypes.c_double), ("dt_next_known", ctypes.c_int),
+    ]
+
+
+# The persistent state of a fluid, periodic box: what a checkpoint holds
+# besides the clock (the face densities are derived from p on restart).
+PERSISTENT = ["ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon",
+              "u11", "u21", "u31", "p1", "t1", "u12", "u22", "u32", "p2", "t2",
+              "u13", "u23", "u33", "p3", "t3"]
 
 _lib = None
 
@@ -171,6 +185,8 @@ def lib() -> ctypes.CDLL:
     L.ark_init_channel.argtypes = [vp, ctypes.c_double]
     L.ark_write_average_values.argtypes = [vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int]
     L.ark_init_velocity.argtypes = [vp, dp]
+    L.ark_get_clock.argtypes = [vp, ctypes.POINTER(Clock)]
+    L.ark_restart.argtypes = [vp, ctypes.POINTER(Clock)]
     for fn in EXPORTED_SYMBOLS:
         if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error",
                       "ark_mlp_destroy", "ark_mlp_last_error"):
@@ -370,6 +386,32 @@ class Solver:
 
     def synchronize(self) -> None:
         self._check(self._L.ark_synchronize(self._h))
+
+    # ---- checkpoint and restart ---------------------------------------------
+    def clock(self) -> Dict[str, float]:
+        """TIME, nStep, dt and the next step's dt if already reduced."""
+        k = Clock()
+        self._check(self._L.ark_get_clock(self._h, ctypes.byref(k)))
+        return {n: getattr(k, n) for n, _ in Clock._fields_}
+
+    def restart(self, clock: Dict[str, float]) -> None:
+        """Continue from uploaded arrays with the given clock (ark_restart)."""
+        k = Clock()
+        for n, _ in Clock._fields_:
+            setattr(k, n, clock[n])
+        self._check(self._L.ark_restart(self._h, ctypes.byref(k)))
+
+    def checkpoint(self) -> Dict[str, object]:
+        """The persistent state and the clock, enough to continue bit for bit
+        (plus u1Con..u3Con when the next step's dt is still to be computed
+        from them)."""
+        clock = self.clock()
+        names = PERSISTENT if clock["dt_next_known"] else PERSISTENT + ["u1Con", "u2Con", "u3Con"]
+        return {"arrays": self.download_all(names), "clock": clock}
+
+    def resume(self, ckpt: Dict[str, object]) -> None:
+        self.upload_all(ckpt["arrays"], list(ckpt["arrays"]))
+        self.restart(ckpt["clock"])
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

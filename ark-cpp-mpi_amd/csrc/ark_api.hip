@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -246,10 +247,13 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
         h.nstep = 0;
         memcpy(&h.dt_init, &d0, 8);
         h.dt_bits = h.dt_init;
-        h.nonfinite = 0;
+        h.nf[0] = 0;
+        h.nf[1] = LLONG_MAX;
         s = c->hip(hipMemcpyAsync(c->S, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "upload scalars");
         if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     }
+    if (s == ARK_OK) s = c->hip(hipHostMalloc((void**)&c->nf_host, 2 * sizeof(long long)), "hipHostMalloc");
+    if (s == ARK_OK) s = c->hip(hipEventCreateWithFlags(&c->nf_event, hipEventDisableTiming), "hipEventCreate");
     *out = c;
     return s;
 }
@@ -265,6 +269,8 @@ void ark_destroy(ark_ctx* c) {
     }
     for (void* m : c->allocs) (void)hipFree(m);
     if (c->dense) (void)hipFree(c->dense);
+    if (c->nf_host) (void)hipHostFree(c->nf_host);
+    if (c->nf_event) (void)hipEventDestroy(c->nf_event);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -345,8 +351,12 @@ ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2,
     }
     if (field >= F_sigm11) c->sigma_dirty = true;
     if (field == F_ro1 || field == F_ro2 || field == F_ro3 || field == F_p1 || field == F_p2 || field == F_p3) {
+        // face densities left derived by fused steps are stored first, from
+        // the pressures they belong to, so that the arrays not uploaded keep
+        // the reference's values
+        s = ark_refresh_rho(c);
+        if (s != ARK_OK) return s;
         c->rho_consistent = false;
-        c->rho_stored = true;
     }
     s = copy_planes(c, c->cur[field], host, nullptr, e1, e2, e3);
     // the second buffer from the dense copy still on the device
@@ -380,6 +390,51 @@ ark_status ark_set_dt(ark_ctx* c, double dt) {
     ark_status s = c->hip(hipMemcpyAsync(&c->S->dt, &c->host_dt, sizeof(double), hipMemcpyHostToDevice, c->stream),
                           "set dt");
     return s == ARK_OK ? c->hip(hipStreamSynchronize(c->stream), "sync") : s;
+}
+
+ark_status ark_get_clock(ark_ctx* c, ark_clock* out) {
+    if (!c || !out) return ARK_ERR_INVALID;
+    Scalars sc;
+    ark_status s = c->hip(hipMemcpyAsync(&sc, c->S, sizeof sc, hipMemcpyDeviceToHost, c->stream), "scalars");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s != ARK_OK) return s;
+    out->time = sc.time;
+    out->nstep = sc.nstep;
+    out->dt = sc.dt;
+    out->dt_next_known = c->dt_pending ? 1 : 0;
+    out->dt_next = 0.0;
+    if (c->dt_pending) memcpy(&out->dt_next, &sc.dt_bits, sizeof(double));
+    return ARK_OK;
+}
+
+ark_status ark_restart(ark_ctx* c, const ark_clock* k) {
+    if (!c || !k) return ARK_ERR_INVALID;
+    if (k->dt_next_known && !(k->dt_next > 0.0 && std::isfinite(k->dt_next)))
+        return c->fail(ARK_ERR_INVALID, "restart: dt_next must be positive and finite");
+    Scalars hs;
+    ark_status s = c->hip(hipMemcpyAsync(&hs, c->S, sizeof hs, hipMemcpyDeviceToHost, c->stream), "scalars");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s != ARK_OK) return s;
+    hs.time = k->time;
+    hs.nstep = k->nstep;
+    hs.dt = k->dt;
+    if (k->dt_next_known) memcpy(&hs.dt_bits, &k->dt_next, sizeof(double));
+    else hs.dt_bits = hs.dt_init;
+    hs.nf[0] = 0;
+    hs.nf[1] = LLONG_MAX;
+    s = c->hip(hipMemcpyAsync(c->S, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream), "scalars");
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    if (s != ARK_OK) return s;
+    c->host_dt = k->dt;
+    c->dt_pending = k->dt_next_known != 0;
+    c->nf_seen = false;
+    c->nf_posted = false;
+    const bool periodic = c->prm.periodic[0] && c->prm.periodic[1] && c->prm.periodic[2];
+    if (c->prm.precision == ARK_PREC_FP64 && periodic && !c->has_solid) {
+        c->rho_consistent = true;
+        c->rho_stored = false;   // derived from p when next needed
+    }
+    return ARK_OK;
 }
 
 struct ClockArgs { ark_ctx* c; int what; };
@@ -615,9 +670,10 @@ ark_status ark_time_step_size(ark_ctx* c, double* dt_out) {
     ark_status s = c->has_solid ? require_fluid(c) : ARK_OK;
     if (s == ARK_OK) {
         if (c->dt_pending) {
-            // after fused steps Con is not stored; their MIN is TimeStepSize's result
-            c->dt_pending = false;
-            s = ark_cfl_finalize(c, 0);
+            // after fused steps Con is not stored; their MIN is TimeStepSize's
+            // result, and it stays pending: the next step takes the same dt,
+            // as a second TimeStepSize call in the reference would
+            s = ark_cfl_finalize(c, 3);
         } else {
             s = ark_cfl_step(c, 0);
         }
@@ -717,23 +773,29 @@ ark_status ark_phase_pairs(ark_ctx* c, int npairs) {
 ark_status ark_step_phases(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
     ark_status s = require_fluid(c);
+    if (s == ARK_OK) s = ark_nf_check(c, 0);
     for (int n = 0; n < nsteps && s == ARK_OK; ++n) {
         s = ark_begin_step(c);
         if (s == ARK_OK) s = ark_phase_body(c);
     }
-    return s;
+    return s == ARK_OK ? ark_nf_post(c) : s;
 }
 
+// A stepping call first looks at the non-finite count its predecessor posted
+// (without waiting for it), and posts its own behind its launches.
 ark_status ark_step(ark_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return ARK_ERR_INVALID;
+    ark_status s = ark_nf_check(c, 0);
+    if (s != ARK_OK) return s;
     if (c->prm.precision == ARK_PREC_MIXED) {
         if (c->has_solid)
             return c->fail(ARK_ERR_UNSUPPORTED, "solid cells run on the FP64 phase path, not in mixed precision");
-        return ark_mixed_steps(c, nsteps);
+        s = ark_mixed_steps(c, nsteps);
+    } else {
+        s = require_fluid(c);
+        if (s == ARK_OK) s = ark_step_fused(c, nsteps);
     }
-    ark_status s = require_fluid(c);
-    if (s != ARK_OK) return s;
-    return ark_step_fused(c, nsteps);
+    return s == ARK_OK ? ark_nf_post(c) : s;
 }
 
 ark_status ark_diagnostics(ark_ctx* c, ark_diag* out) {
@@ -768,6 +830,8 @@ ark_status ark_diagnostics(ark_ctx* c, ark_diag* out) {
 // Con := nCon; every other element 0.
 ark_status ark_init_channel(ark_ctx* c, double u10) {
     if (!c) return ARK_ERR_INVALID;
+    if (c->prm.precision == ARK_PREC_MIXED)
+        return c->fail(ARK_ERR_UNSUPPORTED, "the channel has solid cells and physical boundaries: FP64 only");
     const int n1 = c->geo.N1 + 1, n2 = c->geo.N2 + 1, n3 = c->geo.N3 + 1;
     const double dx1 = c->prm.dx1, dx2 = c->prm.dx2;
     std::vector<double> x1(n1 + 2), x2(n2 + 2);
@@ -888,6 +952,9 @@ ark_status ark_finish_init(ark_ctx* c) {
     c->rho_stored = true;
     c->sigma_dirty = false;
     c->dt_pending = false;   // the reset below discards a MIN left by fused steps
+    c->nf_seen = false;
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    c->nf_posted = false;
     if (s == ARK_OK) {
         // fresh run: TIME = 0, nStep = 0, running MIN reset
         // (ordered on the context's stream, after the kernels that set dt_init)
@@ -898,7 +965,8 @@ ark_status ark_finish_init(ark_ctx* c) {
         hs.nstep = 0;
         hs.dt = 0.0;
         hs.dt_bits = hs.dt_init;
-        hs.nonfinite = 0;
+        hs.nf[0] = 0;
+        hs.nf[1] = LLONG_MAX;
         if (s == ARK_OK) s = c->hip(hipMemcpyAsync(c->S, &hs, sizeof hs, hipMemcpyHostToDevice, c->stream), "scalars");
         if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     }
@@ -907,7 +975,40 @@ ark_status ark_finish_init(ark_ctx* c) {
 
 ark_status ark_synchronize(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
-    return c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    ark_status s = ark_nf_post(c);
+    if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return s == ARK_OK ? ark_nf_check(c, 1) : s;
+}
+
+ark_status ark_nf_post(ark_ctx* c) {
+    ark_status s = c->hip(hipMemcpyAsync(c->nf_host, c->S->nf, 2 * sizeof(long long), hipMemcpyDeviceToHost,
+                                         c->stream), "non-finite readback");
+    if (s == ARK_OK) s = c->hip(hipEventRecord(c->nf_event, c->stream), "hipEventRecord");
+    if (s == ARK_OK) c->nf_posted = true;
+    return s;
+}
+
+ark_status ark_nf_check(ark_ctx* c, int wait) {
+    if (c->nf_posted) {
+        const hipError_t e = wait ? hipEventSynchronize(c->nf_event) : hipEventQuery(c->nf_event);
+        if (e == hipSuccess) {
+            c->nf_posted = false;
+            if (c->nf_host[0] > 0 && !c->nf_seen) {
+                c->nf_seen = true;
+                c->nf_count = c->nf_host[0];
+                c->nf_step = c->nf_host[1];
+            }
+        } else if (e != hipErrorNotReady) {
+            return c->hip(e, "non-finite readback");
+        }
+    }
+    if (!c->nf_seen) return ARK_OK;
+    char buf[200];
+    snprintf(buf, sizeof buf,
+             "non-finite velocity or density: %lld cell(s) by the time it was read, first seen in step %lld; "
+             "stepping is refused until a new initial state (ark_init_*, ark_restart)",
+             c->nf_count, c->nf_step);
+    return c->fail(ARK_ERR_NONFINITE, buf);
 }
 
 ark_status ark_get_stream(ark_ctx* c, void** stream) {
@@ -1003,8 +1104,9 @@ ark_status ark_set_plane(ark_ctx* c, int field, int axis, int q, const double* h
     if (s != ARK_OK) return s;
     if (field >= F_sigm11) c->sigma_dirty = true;
     if (field == F_ro1 || field == F_ro2 || field == F_ro3 || field == F_p1 || field == F_p2 || field == F_p3) {
+        s = ark_refresh_rho(c);   // as in ark_upload
+        if (s != ARK_OK) return s;
         c->rho_consistent = false;
-        c->rho_stored = true;
     }
     if (field == F_condition) {
         for (long long k = 0; k < (long long)na * nb; ++k)

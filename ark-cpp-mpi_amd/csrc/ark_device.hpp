@@ -33,7 +33,11 @@ struct Scalars {
     long long nstep;              // nStep
     unsigned long long dt_bits;   // running MIN of the CFL bound, as the bits of a positive double
     unsigned long long dt_init;   // bits of min(1e8, dtv): the value dt_bits is reset to
-    long long nonfinite;          // cells seen with a non-finite velocity by the CFL scan
+    // Non-finite detection, folded into the CFL reduction of every step
+    // (TimeStepSize, source/main.cpp:374-415, has none): nf[0] counts cells
+    // seen with a non-finite velocity (or, in the fused steps, density), nf[1]
+    // is the first step number nStep at which one was seen (LLONG_MAX: none).
+    long long nf[2];
 };
 
 // Field slots, same order as ark_field in include/ark/ark.h.
@@ -89,6 +93,31 @@ __device__ __forceinline__ void block_min_atomic(unsigned long long bits, unsign
     if (bits != ~0ull) atomicMin(&block_min, bits);
     __syncthreads();
     if (threadIdx.x == 0 && block_min != ~0ull) atomicMin(slot, block_min);
+}
+
+// The same MIN, plus the count of the block's non-finite cells (`bad` per
+// thread) added to nf[0], with nf[1] lowered to `nstep` when there are any.
+// The count costs one LDS atomic per thread that saw a bad cell, which a
+// finite state never does.
+__device__ __forceinline__ void block_min_count(unsigned long long bits, unsigned bad, unsigned long long* slot,
+                                                long long* nf, long long nstep) {
+    __shared__ unsigned long long block_min;
+    __shared__ unsigned block_bad;
+    if (threadIdx.x == 0) {
+        block_min = ~0ull;
+        block_bad = 0u;
+    }
+    __syncthreads();
+    if (bits != ~0ull) atomicMin(&block_min, bits);
+    if (bad) atomicAdd(&block_bad, bad);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (block_min != ~0ull) atomicMin(slot, block_min);
+        if (block_bad) {
+            atomicAdd((unsigned long long*)&nf[0], (unsigned long long)block_bad);
+            atomicMin(&nf[1], nstep);
+        }
+    }
 }
 
 // Work order of the z-marching kernels.  The launch is one-dimensional, and

@@ -55,8 +55,9 @@ struct FusedArgs {
     const char* F[3];          // old faces normal to d: u1, u2, u3, p, t
     char* Fo[3];               // new faces
     unsigned long long fb;     // bytes between the fields of a group
-    const Scalars* S;          // S->dt is this step's dt
+    const Scalars* S;          // S->dt is this step's dt, S->nstep its number
     unsigned long long* dt_next;
+    long long* nf;             // non-finite count and first step (Scalars::nf)
 };
 
 __device__ __forceinline__ double gld(const char* grp, unsigned long long fb, int v, int idx) {
@@ -239,6 +240,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
     double S_prev[5] = {0, 0, 0, 0, 0};
     Flux5 mz_prev{};
     unsigned long long dtbits = ~0ull;
+    unsigned bad = 0;
 
     for (int p = k0 - 1; p <= kend; ++p) {
         // keep the per-field addresses (group + v*stride) from being hoisted
@@ -339,6 +341,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
             double d3 = A.cfl_dx[2] / (K.sound + fabs(H.u3));
             unsigned long long b = (unsigned long long)__double_as_longlong(min3d(d1, d2, d3));
             dtbits = b < dtbits ? b : dtbits;
+            // a NaN or Inf anywhere in the half level makes this sum non-finite
+            bad += !isfinite(H.ro + H.u1 + H.u2 + H.u3);
         }
 
         // ---- z-face p and the corrector of plane p-1 ------------------------
@@ -407,7 +411,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         mz_prev = mz;
         A_cur = A_next;
     }
-    block_min_atomic(dtbits, A.dt_next);
+    block_min_count(dtbits, bad, A.dt_next, A.nf, A.S->nstep);
 }
 
 }  // namespace arkd
@@ -434,6 +438,7 @@ hipError_t run_fused(void* p) {
     a.fb = c->field_bytes;
     a.S = c->S;
     a.dt_next = &c->S->dt_bits;
+    a.nf = c->S->nf;
     const dim3 grid(c->fused_nunits);
     const int4* units = (const int4*)c->fused_units;
     const int nbx = (c->geo.N1 + TX - 1) / TX;
@@ -521,10 +526,16 @@ extern "C" ark_status ark_advance(ark_ctx* c, int nsteps) {
         return c->fail(ARK_ERR_UNSUPPORTED,
                        "the fused steps need a fluid, triply periodic box; solid cells and non-periodic axes run on "
                        "the phase path (ark_step)");
-    if (c->prm.precision == ARK_PREC_MIXED) return ark_mixed_steps(c, nsteps);
-    if (!ark_fused_eligible(c))
-        return c->fail(ARK_ERR_STATE,
-                       "ark_advance needs face densities equal to ro0 + p/c^2 (after ark_init_tgv or any step) and "
-                       "N1, N2, N3 >= 2 in FP64 mode; call ark_step first");
-    return fused_steps(c, nsteps);
+    ark_status s = ark_nf_check(c, 0);
+    if (s != ARK_OK) return s;
+    if (c->prm.precision == ARK_PREC_MIXED) {
+        s = ark_mixed_steps(c, nsteps);
+    } else {
+        if (!ark_fused_eligible(c))
+            return c->fail(ARK_ERR_STATE,
+                           "ark_advance needs face densities equal to ro0 + p/c^2 (after ark_init_tgv or any step) "
+                           "and N1, N2, N3 >= 2 in FP64 mode; call ark_step first");
+        s = fused_steps(c, nsteps);
+    }
+    return s == ARK_OK ? ark_nf_post(c) : s;
 }

@@ -70,6 +70,15 @@ struct ark_ctx {
     bool dt_pending = false;           // S->dt_bits holds the next step's dt from a fused step
     bool sigma_dirty = false;          // an upload left non-zero values in a stress array
     double** sigma_ptrs = nullptr;     // device array: allocation bases of the nine stress arrays
+    // Non-finite detection: the stepping calls copy Scalars::nf into nf_host
+    // (pinned) behind their launches and record nf_event; the next stepping
+    // call reads it if the copy has landed, a synchronising call always.
+    // nf_seen is sticky until a new initial state.
+    long long* nf_host = nullptr;
+    hipEvent_t nf_event = nullptr;
+    bool nf_posted = false;
+    bool nf_seen = false;
+    long long nf_count = 0, nf_step = 0;
     bool profiling = false;
     std::vector<ark_prof_event> prof_events;
     std::vector<void*> allocs;
@@ -96,6 +105,8 @@ ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pen
 ark_status ark_refresh_rho(ark_ctx* c);                 // store face densities if they are derived
 ark_status ark_phase_body(ark_ctx* c);                  // Phase1 .. UseForces of one step, dt already set
 ark_status ark_phase_pairs(ark_ctx* c, int npairs);     // 2*npairs phase-path steps from one captured graph
+ark_status ark_nf_post(ark_ctx* c);                     // copy the non-finite count behind the queued work
+ark_status ark_nf_check(ark_ctx* c, int wait);          // ARK_ERR_NONFINITE once a posted count is non-zero
 }
 
 // The closure's spectral projection state (ark_turb.hip); freed with the context.

@@ -34,17 +34,21 @@ __global__ void __launch_bounds__(256) k_cfl(Geo g, const double* __restrict__ u
         bits = umin64(bits, (unsigned long long)__double_as_longlong(dtu));
         bad += !(isfinite(a1) && isfinite(a2) && isfinite(a3));
     }
-    block_min_atomic(bits, &S->dt_bits);
-    for (int off = 32; off > 0; off >>= 1) bad += __shfl_xor(bad, off, 64);
-    if ((threadIdx.x & 63) == 0 && bad) atomicAdd((unsigned long long*)&S->nonfinite, (unsigned long long)bad);
+    block_min_count(bits, (unsigned)bad, &S->dt_bits, S->nf, S->nstep);
 }
 
 // mode 0: dt := MIN and reset the running MIN; mode 1: the same plus the
 // driver's bookkeeping TIME += dt/2 twice and nStep++ (source/main.cpp:83-84,
-// 91); mode 2: only reset the running MIN (discard it).
+// 91); mode 2: only reset the running MIN (discard it); mode 3: dt := MIN and
+// keep the running MIN (a TimeStepSize between fused steps, whose next step
+// finalizes the same MIN again).
 __global__ void k_cfl_finalize(Scalars* S, int mode) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double dt = __longlong_as_double((long long)S->dt_bits);
+        if (mode == 3) {
+            S->dt = dt;
+            return;
+        }
         S->dt_bits = S->dt_init;
         if (mode == 2) return;
         S->dt = dt;

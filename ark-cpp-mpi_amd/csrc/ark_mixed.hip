@@ -75,6 +75,7 @@ struct Args {
     unsigned long long fb;
     const Scalars* S;
     unsigned long long* dt_next;
+    long long* nf;      // non-finite count and first step (Scalars::nf)
     int LZ;
     // body acceleration (forcing, FP32, components force + m*fstride in the
     // field index map) or null; the cell force is rho dvc F
@@ -162,9 +163,15 @@ __device__ __forceinline__ float max3f(float x, float y, float z) { return fmaxf
 __device__ __forceinline__ float min3f(float x, float y, float z) { return fminf(fminf(x, y), z); }
 
 // Limited extrapolations (CABARET min-max limiter, source/main.cpp:1777-1873).
+// The Riemann invariants r = u + p/(rho0 c) and q = u - p/(rho0 c) are held
+// relative to the cell's half-level normal velocity un_cn: every operation on
+// them (differences, 2 r_cn - r_b, min, max, clamp) commutes with a common
+// shift, and the shifted values are of the size of the velocity differences
+// and p/(rho0 c), so FP32 keeps the digits that the face pressure
+// (r - q) rho0 c / 2 is made of.  combine() adds the shifts back.
 __device__ __forceinline__ FwdF forward(const Side& s, float dt, float inv_dt2, float inv_dx, const Args& a) {
-    const float r_f = s.un_f + s.p_f * a.inv_rc, r_b = s.un_b + s.p_b * a.inv_rc;
-    const float r_cn = s.un_cn + s.p_cn * a.inv_rc, r_c = s.un_c + s.p_c * a.inv_rc;
+    const float r_f = (s.un_f - s.un_cn) + s.p_f * a.inv_rc, r_b = (s.un_b - s.un_cn) + s.p_b * a.inv_rc;
+    const float r_cn = s.p_cn * a.inv_rc, r_c = (s.un_c - s.un_cn) + s.p_c * a.inv_rc;
     const float gr = (r_cn - r_c) * inv_dt2 + (s.un_cn + a.c) * (r_f - r_b) * inv_dx;
     const float gt = (s.t_cn - s.t_c) * inv_dt2 + s.un_cn * (s.t_f - s.t_b) * inv_dx;
     const float ga = (s.ua_cn - s.ua_c) * inv_dt2 + s.un_cn * (s.ua_f - s.ua_b) * inv_dx;
@@ -183,11 +190,11 @@ __device__ __forceinline__ FwdF forward(const Side& s, float dt, float inv_dt2, 
 // (source/main.cpp:1804, 2321), the X2 sweep from q (2083).
 template <bool GqUsesR>
 __device__ __forceinline__ BwdF backward(const Side& s, float dt, float inv_dt2, float inv_dx, const Args& a) {
-    const float q_f = s.un_f - s.p_f * a.inv_rc, q_b = s.un_b - s.p_b * a.inv_rc;
-    const float q_cn = s.un_cn - s.p_cn * a.inv_rc, q_c = s.un_c - s.p_c * a.inv_rc;
+    const float q_f = (s.un_f - s.un_cn) - s.p_f * a.inv_rc, q_b = (s.un_b - s.un_cn) - s.p_b * a.inv_rc;
+    const float q_cn = -(s.p_cn * a.inv_rc), q_c = (s.un_c - s.un_cn) - s.p_c * a.inv_rc;
     float dq;
     if (GqUsesR) {
-        const float r_cn = s.un_cn + s.p_cn * a.inv_rc, r_c = s.un_c + s.p_c * a.inv_rc;
+        const float r_cn = s.p_cn * a.inv_rc, r_c = (s.un_c - s.un_cn) + s.p_c * a.inv_rc;
         dq = (r_cn - r_c) * inv_dt2;
     } else {
         dq = (q_cn - q_c) * inv_dt2;
@@ -210,13 +217,14 @@ struct FaceOutF { float un, p, t, a, b; };
 
 // New face from the cell behind (forward half fw, half-level hb: un, t, a, b)
 // and the cell ahead (bw, hf); upwinding of the passive quantities as in
-// source/main.cpp:1932-1961.
+// source/main.cpp:1932-1961.  fw.r and bw.q are relative to the half-level
+// normal velocities ucb and ucf of their cells.
 __device__ __forceinline__ FaceOutF combine(const FwdF& fw, const BwdF& bw, float ucb, float ucf, float tb_,
                                             float tf_, float ab_, float af_, float bb_, float bf_, float t_old,
                                             float a_old, float b_old, const Args& a) {
     FaceOutF o;
-    o.un = (fw.r + bw.q) * 0.5f;
-    o.p = (fw.r - bw.q) * a.half_rc;
+    o.un = ((fw.r + bw.q) + (ucb + ucf)) * 0.5f;
+    o.p = ((fw.r - bw.q) + (ucb - ucf)) * a.half_rc;
     if (ucf >= 0.f && ucb >= 0.f) {
         o.t = fw.t; o.a = fw.a; o.b = fw.b;
     } else if (ucf <= 0.f && ucb <= 0.f) {
@@ -397,6 +405,7 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     CellF Hprev{};
     FwdF fz_prev{};
     unsigned long long dtbits = ~0ull;
+    unsigned bad = 0;
 
     for (int p = k0 - 1; p <= kend; ++p) {
         asm volatile("" : "+s"(A.fb));
@@ -486,6 +495,8 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
             const double dmin = (double)fminf(fminf(d1, d2), d3);
             const unsigned long long b = (unsigned long long)__double_as_longlong(dmin);
             dtbits = b < dtbits ? b : dtbits;
+            // a NaN or Inf anywhere in the half level makes this sum non-finite
+            bad += !isfinite(H.dr + H.u[0] + H.u[1] + H.u[2]);
         }
 
         // z-face p, then the FP64 closing update of plane p-1
@@ -568,7 +579,7 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         Acur = Anext;
         Anext = Anext2;
     }
-    block_min_atomic(dtbits, A.dt_next);
+    block_min_count(dtbits, bad, A.dt_next, A.nf, A.S->nstep);
 }
 
 // One box, one workgroup per chunk of a tile column.  units[u] = (tile
@@ -657,7 +668,9 @@ int tiles_of(const Geo& g, int shape, int LZ) {
 
 // Kernel arguments of one mixed step of context c: read the current buffers,
 // write the other ones; dt_next is where the next step's dt MIN goes.
-mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, int flip = 0) {
+// nf: where the non-finite count goes (Scalars::nf of the box, or a scratch
+// pair for the timing launches).
+mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, long long* nf, int flip = 0) {
     const ark_params& P = c->prm;
     mixed::Args a;
     a.g = c->geo;
@@ -688,6 +701,7 @@ mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, int
     a.fb = c->field_bytes;
     a.S = c->S;
     a.dt_next = dt_next;
+    a.nf = nf;
     a.LZ = LZ;
     a.force = (c->force_eps > 0.0 || c->closure) ? c->force : nullptr;
     a.fstride = c->force_stride;
@@ -749,7 +763,7 @@ void launch_single_shape(const ark_ctx* c, int shape, const mixed::Args& a) {
 
 hipError_t run_mixed(void* p) {
     ark_ctx* c = (ark_ctx*)p;
-    launch_single_shape(c, c->mixed_shape, make_args(c, 0, &c->S->dt_bits));
+    launch_single_shape(c, c->mixed_shape, make_args(c, 0, &c->S->dt_bits, c->S->nf));
     return hipGetLastError();
 }
 
@@ -774,12 +788,13 @@ ark_status tune_single_shape(ark_ctx* c) {
     constexpr int kReps = 3;
     unsigned long long* scratch = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    ark_status s = c->hip(hipMalloc(&scratch, sizeof(unsigned long long)), "hipMalloc tuning slot");
+    // three slots: the dt MIN, and the non-finite count and first step
+    ark_status s = c->hip(hipMalloc(&scratch, 3 * sizeof(unsigned long long)), "hipMalloc tuning slot");
     if (s == ARK_OK) s = c->hip(hipEventCreate(&e0), "event");
     if (s == ARK_OK) s = c->hip(hipEventCreate(&e1), "event");
     float best = 0.f;
     int shape = 0;
-    const mixed::Args a = make_args(c, 0, scratch);
+    const mixed::Args a = make_args(c, 0, scratch, (long long*)(scratch + 1));
     for (int sh = 0; sh < 2 && s == ARK_OK; ++sh) {
         launch_single_shape(c, sh, a);
         s = c->hip(hipGetLastError(), "tuning launch");
@@ -956,7 +971,8 @@ struct ark_batch {
     mixed::Args* args[2] = {nullptr, nullptr};   // device: parity 0 (current -> other), parity 1
     int* start = nullptr;                         // device: first tile of each box, total last
     Scalars** sptr = nullptr;                     // device: each box's scalars
-    unsigned long long* scratch_dt = nullptr;     // device: dt MIN slots of the tuning launches
+    unsigned long long* scratch_dt = nullptr;     // device: dt MIN and non-finite slots of the tuning launches,
+                                                  // three per box
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     ark_force_batch* forcing = nullptr;           // the forced boxes' force kernels, two launches per step
@@ -1003,7 +1019,8 @@ ark_status build_tables(ark_batch* b, bool tuning) {
         const int lz = box_lz(b, c);
         start[i + 1] = start[i] + tiles_of(c->geo, b->shape, lz);
         for (int p = 0; p < 2; ++p)
-            h[p][i] = make_args(c, lz, tuning ? b->scratch_dt + i : &c->S->dt_bits, p);
+            h[p][i] = tuning ? make_args(c, lz, b->scratch_dt + 3 * i, (long long*)(b->scratch_dt + 3 * i + 1), p)
+                             : make_args(c, lz, &c->S->dt_bits, c->S->nf, p);
     }
     b->tiles = start[n];
     ark_status s = ARK_OK;
@@ -1113,7 +1130,7 @@ ark_status ark_batch_create(ark_ctx* const* boxes, int nboxes, ark_batch** out) 
         s = b->hip(hipMalloc(&b->args[p], n * sizeof(mixed::Args)), "hipMalloc batch args");
     if (s == ARK_OK) s = b->hip(hipMalloc(&b->start, (n + 1) * sizeof(int)), "hipMalloc batch tiles");
     if (s == ARK_OK) s = b->hip(hipMalloc(&b->sptr, n * sizeof(Scalars*)), "hipMalloc batch scalars");
-    if (s == ARK_OK) s = b->hip(hipMalloc(&b->scratch_dt, n * sizeof(unsigned long long)), "hipMalloc batch dt");
+    if (s == ARK_OK) s = b->hip(hipMalloc(&b->scratch_dt, 3 * n * sizeof(unsigned long long)), "hipMalloc batch dt");
     if (s == ARK_OK) {
         std::vector<Scalars*> sp;
         for (ark_ctx* c : b->box) sp.push_back(c->S);

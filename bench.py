@@ -423,8 +423,8 @@ def main():
                     help="steps of the phase-by-phase FP64 path timed per kernel (0: skip)")
     ap.add_argument("--pcie-steps", type=int, default=500,
                     help="steps of the run whose state crosses the host boundary (0: skip)")
-    ap.add_argument("--cpu-steps", type=int, default=15,
-                    help="steps of the CPU baseline's 256^3 sample (about 12 s on 16 host cores)")
+    ap.add_argument("--cpu-steps", type=int, default=50,
+                    help="steps of the CPU baseline's 256^3 sample (about 45 s on 16 host cores)")
     ap.add_argument("--cpu-ranks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed", "channel"], default=None,

@@ -128,6 +128,32 @@ ark_status ark_get_dt(ark_ctx* ctx, double* dt);   /* synchronises */
 ark_status ark_next_step(ark_ctx* ctx);
 ark_status ark_half_time(ark_ctx* ctx);
 
+/* The step clock of a checkpoint: the reference's TIME and nStep
+ * (source/main.cpp:83-84, 91) and dt (headers/constants.h:20), plus the next
+ * step's dt when a fused step has already reduced it (then the next step does
+ * not read Con).  A checkpoint of a fluid, periodic box is the persistent
+ * state -- the level-n cells ronCon..tnCon and, per axis, the faces u1*, u2*,
+ * u3*, p*, t* (downloaded) -- and this clock; a restart uploads them into a
+ * context of the same parameters and calls ark_restart.  No counterpart in
+ * the reference, which cannot restart (its VTK output holds no faces). */
+typedef struct {
+    double time;          /* TIME */
+    int64_t nstep;        /* nStep */
+    double dt;            /* dt of the last step taken */
+    double dt_next;       /* the next step's dt, when dt_next_known */
+    int dt_next_known;    /* 1: reduced by a fused step; 0: the next step's
+                             TimeStepSize scans u1Con..u3Con */
+} ark_clock;
+/* Synchronises. */
+ark_status ark_get_clock(ark_ctx* ctx, ark_clock* out);
+/* Continue from uploaded arrays: sets the clock; on a fluid, triply periodic
+ * FP64 box the face densities become ro0 + p/c^2 of the uploaded pressures
+ * (source/main.cpp:1930: what every step leaves there), elsewhere they stay
+ * as uploaded.  Clears the non-finite record.  With dt_next_known = 0 the
+ * next step computes dt from u1Con..u3Con, which must be uploaded too.
+ * Synchronises. */
+ark_status ark_restart(ark_ctx* ctx, const ark_clock* clock);
+
 /* TimeStepSize(), source/main.cpp:374-415: dt = min over interior cells of
  * the CFL bounds, on the device; the MPI MIN allreduce is the identity on one
  * GPU.  dt_out may be NULL (then nothing synchronises). */
@@ -147,7 +173,11 @@ ark_status ark_phase2(ark_ctx* ctx);
  * 82-91: TimeStepSize, Phase1, StressTensor, UseForces, Phase2, Phase1,
  * UseForces) through the fused kernels.  Results are bitwise equal to the
  * phase-by-phase calls in FP64 mode.  Nothing synchronises: dt stays on the
- * device between steps.  In mixed precision the first step of a context also
+ * device between steps.  Non-finite values: every step counts the cells whose
+ * half-level velocity or density is NaN or Inf in its CFL reduction (the
+ * reference checks nothing); ark_synchronize, and any stepping call that
+ * finds the count of an earlier call already copied back, then return
+ * ARK_ERR_NONFINITE, and stepping is refused until a new initial state.  In mixed precision the first step of a context also
  * times its two tile shapes (a few launches that leave the state unchanged)
  * and keeps the faster; results do not depend on the shape. */
 ark_status ark_step(ark_ctx* ctx, int nsteps);
@@ -194,7 +224,8 @@ ark_status ark_init_tgv(ark_ctx* ctx, double U0);
  * is 0; Con := nCon; TIME and nStep 0. */
 ark_status ark_init_channel(ark_ctx* ctx, double u10);
 
-/* Block until all work on the context's stream is done. */
+/* Block until all work on the context's stream is done.  Returns
+ * ARK_ERR_NONFINITE when a step has met a non-finite value (see ark_step). */
 ark_status ark_synchronize(ark_ctx* ctx);
 /* The context's hipStream_t, for callers that order their own work on it. */
 ark_status ark_get_stream(ark_ctx* ctx, void** stream);

@@ -238,7 +238,60 @@ def channel_fixture(path: str, work: str) -> None:
         json.dump(out, fh, indent=1)
 
 
+def tgv256_fixture(path: str, work: str, steps: int = 5) -> None:
+    """The 256^3 Taylor-Green case of the GPU workload, `steps` steps of the
+    unmodified reference: one rank (all 41 arrays with their ghosts, so the
+    GPU's full state can be compared) and eight ranks (dims 4x2x1, z unsplit;
+    the interior cell fields must be the same bits).  Adds the run to
+    tgv_digests.json as "256^3x<steps>".  The eight-rank gather sums each
+    cell with zeros from the other ranks, which turns -0.0 into +0.0, so its
+    interior digests are taken after adding +0.0 on both sides."""
+    dig = json.load(open(path))
+    N = 256
+    d = tgv_run(N, steps, work, [steps])
+    sd = os.path.join(d, f"s{steps}")
+    st = OracleState.from_dir(sd)
+    meta = read_meta(sd)
+    run1 = {
+        "N": N, "steps": steps, "VIS": float(st.VIS).hex(), "CFL": float(st.CFL).hex(),
+        "sound": float(st.sound).hex(), "final_dt": float(meta["dt"]).hex(),
+        "series": read_series(d),
+        "arrays": {n: digest(st[n]) for n in FIELDS},
+    }
+    interior = {n: np.ascontiguousarray(st[n][1:-1, 1:-1, 1:-1]) + 0.0
+                for n in ("ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon")}
+    del st
+    shutil.rmtree(d, ignore_errors=True)
+    h1 = {n: hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+          for n, a in interior.items()}
+    d8 = tgv_run(N, steps, work, [steps], ranks=8)
+    h8 = {}
+    for n in interior:
+        a = np.fromfile(os.path.join(d8, f"s{steps}", n + ".interior.f64"), dtype="<f8").reshape(N, N, N) + 0.0
+        h8[n] = hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    ser8 = read_series(d8)
+    shutil.rmtree(d8, ignore_errors=True)
+    phys = [n for n in h1 if n != "tnCon"]
+    if any(h1[n] != h8[n] for n in phys) or ser8["dt"] != run1["series"]["dt"]:
+        raise SystemExit("256^3: eight ranks differ from one rank")
+    run1["interior_sha256_plus_zero"] = h1
+    run1["interior_sha256_plus_zero_8ranks"] = h8
+    dig["runs"][f"{N}^3x{steps}"] = run1
+    with open(path, "w") as fh:
+        json.dump(dig, fh, indent=1)
+
+
 def main() -> None:
+    if len(sys.argv) > 1 and sys.argv[1] == "tgv256":
+        if not os.path.exists(HARNESS):
+            sys.exit("reference harness not built: make -C oracle ref")
+        work = tempfile.mkdtemp(prefix="ark_golden_")
+        try:
+            tgv256_fixture(os.path.join(HERE, "tgv_digests.json"), work)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        print("256^3 digests added to", os.path.join(HERE, "tgv_digests.json"))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "channel":
         if not os.path.exists(HARNESS):
             sys.exit("reference harness not built: make -C oracle ref")

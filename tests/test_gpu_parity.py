@@ -145,6 +145,41 @@ def test_tgv_digest(N, steps):
         assert d["nstep"] == steps
 
 
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("how", ["step", "advance"])
+def test_tgv_digest_256(how):
+    """The 256^3 workload against the unmodified reference (5 steps, one rank;
+    its eight-rank run gives the same interior cells).  `step`: four steps
+    through the production fused kernel (32x13 tiles, the chip-sized chunk
+    table, 32-bit byte offsets up to the top of a field group) and the last
+    through the phase kernels, then all 41 arrays, ghosts included, by
+    SHA-256.  `advance`: five fused steps, then the interior of the level-n
+    cell fields (after adding +0.0, as the reference's gathered digest) and
+    the final dt."""
+    run = json.load(open(os.path.join(GOLDEN, "tgv_digests.json")))["runs"]["256^3x5"]
+    want_dt = [float.fromhex(v) for v in run["series"]["dt"][1:]]
+    N = 256
+    with arkamd.Solver(N, N, N) as s:
+        s.init_tgv()
+        if how == "step":
+            s.step(5)
+            assert s.dt == want_dt[-1]
+            bad = [n for n in FIELDS if sha(s.download(n)) != run["arrays"][n]["sha256"]]
+            assert not bad, f"arrays differing from the reference digest: {bad}"
+        else:
+            s.advance(4)
+            assert s.time_step_size() == want_dt[4]
+            s.advance(1)
+            assert s.dt == want_dt[-1]
+            for n, h in run["interior_sha256_plus_zero"].items():
+                got = s.download(n)[1:-1, 1:-1, 1:-1] + 0.0
+                assert sha(got) == h, n
+        assert s.diagnostics()["nstep"] == 5
+
+
 @pytest.mark.parametrize("N", [(32, 32, 32), (20, 13, 7), (70, 40, 90)])
 def test_fused_equals_phases(N):
     with arkamd.Solver(*N) as a, arkamd.Solver(*N) as b:
