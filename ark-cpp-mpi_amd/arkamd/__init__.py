@@ -55,7 +55,7 @@ EXPORTED_SYMBOLS = [
     "ark_mlp_get_params", "ark_mlp_set_normalization", "ark_mlp_set_data", "ark_mlp_gradient", "ark_mlp_train",
     "ark_mlp_predict", "ark_set_closure", "ark_init_velocity",
     "ark_write_paraview", "ark_average_values", "ark_write_average_values", "ark_init_channel",
-    "ark_get_clock", "ark_restart",
+    "ark_get_clock", "ark_restart", "ark_diagnostics_noalloc",
 ]
 
 
@@ -187,6 +187,8 @@ def lib() -> ctypes.CDLL:
     L.ark_init_velocity.argtypes = [vp, dp]
     L.ark_get_clock.argtypes = [vp, ctypes.POINTER(Clock)]
     L.ark_restart.argtypes = [vp, ctypes.POINTER(Clock)]
+    L.ark_diagnostics_noalloc.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.c_void_p, ctypes.c_void_p]
     for fn in EXPORTED_SYMBOLS:
         if fn not in ("ark_destroy", "ark_last_error", "ark_batch_destroy", "ark_batch_last_error",
                       "ark_mlp_destroy", "ark_mlp_last_error"):

@@ -209,6 +209,22 @@ ark_status ark_step_phases(ark_ctx* ctx, int nsteps);
  * mass and a non-finite scan.  Synchronises. */
 ark_status ark_diagnostics(ark_ctx* ctx, ark_diag* out);
 
+/* ark_diagnostics without a context and without allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (work under HIP graph capture, tight serving loops) and hold the level-n
+ * cell fields themselves.  cells = device pointers to ronCon, u1nCon,
+ * u2nCon, u3nCon, tnCon in the reference's Arr3d layout (extents
+ * (N1+2)(N2+2)(N3+2), x fastest); from params only N1..N3, dx1..dx3 and
+ * device are read.  out (device, 5 doubles) receives ke_reference,
+ * ke_physical, mass, max_abs_u and the non-finite count, the same sums bit
+ * for bit as ark_diagnostics on the same fields.  Two kernel launches on
+ * `stream` (a hipStream_t, or NULL), nothing synchronises, no hipMalloc: the
+ * partial sums use a 40 KiB device array of the library, one per GPU, shared
+ * by every call there, so calls that overlap on one GPU race.  Grids with
+ * 2^31 or more elements per array are refused (ARK_ERR_UNSUPPORTED). */
+ark_status ark_diagnostics_noalloc(const ark_params* params, const double* const cells[5], double* out,
+                                   void* stream);
+
 /* Taylor-Green vortex initial state built on the device from 1-D trig tables
  * that are evaluated on the host with the C library's sin/cos, so the fields
  * are bitwise equal to a host evaluation of the same formula.  Replaces the
