@@ -55,7 +55,7 @@ EXPORTED_SYMBOLS = [
     "ark_mlp_get_params", "ark_mlp_set_normalization", "ark_mlp_set_data", "ark_mlp_gradient", "ark_mlp_train",
     "ark_mlp_predict", "ark_set_closure", "ark_init_velocity",
     "ark_write_paraview", "ark_average_values", "ark_write_average_values", "ark_init_channel",
-    "ark_get_clock", "ark_restart", "ark_diagnostics_noalloc",
+    "ark_get_clock", "ark_restart", "ark_diagnostics_noalloc", "ark_init_state", "ark_filter_density",
 ]
 
 
@@ -185,6 +185,8 @@ def lib() -> ctypes.CDLL:
     L.ark_init_channel.argtypes = [vp, ctypes.c_double]
     L.ark_write_average_values.argtypes = [vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int]
     L.ark_init_velocity.argtypes = [vp, dp]
+    L.ark_init_state.argtypes = [vp, dp, dp]
+    L.ark_filter_density.argtypes = [vp, ip, dp]
     L.ark_get_clock.argtypes = [vp, ctypes.POINTER(Clock)]
     L.ark_restart.argtypes = [vp, ctypes.POINTER(Clock)]
     L.ark_diagnostics_noalloc.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p),
@@ -464,6 +466,24 @@ class Solver:
         if a.shape != (3, N3, N2, N1):
             raise ValueError(f"velocity shape {a.shape} != {(3, N3, N2, N1)}")
         self._check(self._L.ark_init_velocity(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def init_state(self, u, rho) -> None:
+        """init_velocity plus the density rho (N3, N2, N1) on the cells and,
+        by the same shift, on the faces with their pressures."""
+        N1, N2, N3 = self.N
+        a = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        if a.shape != (3, N3, N2, N1) or r.shape != (N3, N2, N1):
+            raise ValueError(f"state shapes {a.shape}, {r.shape} != {(3, N3, N2, N1)}, {(N3, N2, N1)}")
+        P = ctypes.POINTER(ctypes.c_double)
+        self._check(self._L.ark_init_state(self._h, a.ctypes.data_as(P), r.ctypes.data_as(P)))
+
+    def filter_density(self, width: int) -> np.ndarray:
+        """Block means of the cell density over width^3 cells, (Nc3, Nc2, Nc1)."""
+        N1, N2, N3 = self.N
+        out = np.zeros((N3 // width, N2 // width, N1 // width))
+        self._check(self._L.ark_filter_density(self._h, int(width), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
 
     def set_forcing(self, seed: int, eps: float, tau: float = 0.0) -> None:
         self._check(self._L.ark_set_forcing(self._h, int(seed), float(eps), float(tau)))

@@ -128,6 +128,132 @@ def les(ub0: np.ndarray, t_end: float, vis: float, eps: float, seed: int, mlp: O
     return out
 
 
+def stress_from_force(F: np.ndarray, dx: float) -> np.ndarray:
+    """A symmetric stress field tau (6, N3, N2, N1; 11 22 33 12 13 23) whose
+    acceleration as the closure applies it, -sum_j D_j tau_ij with D_j the
+    periodic central difference (tau(i+1) - tau(i-1)) / (2 dx), is the given
+    acceleration F (3, N3, N2, N1), except for the modes that operator cannot
+    produce (every k_j dx in {0, pi}), which are dropped.  With
+    tau_ij = D_j psi_i + D_i psi_j, and D_j of symbol i s_j,
+    s_j = sin(k_j dx) / dx, the condition reads (|s|^2 I + s s^T) psi = F,
+    which Sherman-Morrison inverts: psi = (F - s (s.F) / (2|s|^2)) / |s|^2."""
+    shp = F.shape[1:]
+    ks = [np.fft.fftfreq(n, d=1.0 / n) for n in shp]      # integer wavenumbers per array axis (z, y, x)
+    kz, ky, kx = np.meshgrid(*ks, indexing="ij")
+    L = [n * dx for n in shp]
+    s = [np.sin(2 * np.pi * kx / L[2] * dx) / dx, np.sin(2 * np.pi * ky / L[1] * dx) / dx,
+         np.sin(2 * np.pi * kz / L[0] * dx) / dx]          # component order x, y, z
+    s2 = s[0] ** 2 + s[1] ** 2 + s[2] ** 2
+    ok = s2 > 1e-12 * s2.max()
+    Fh = [np.fft.fftn(F[m]) for m in range(3)]
+    sF = s[0] * Fh[0] + s[1] * Fh[1] + s[2] * Fh[2]
+    inv = np.where(ok, 1.0 / np.where(ok, s2, 1.0), 0.0)
+    psi = [(Fh[m] - s[m] * sF * 0.5 * inv) * inv for m in range(3)]
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    return np.stack([np.real(np.fft.ifftn(1j * (s[b] * psi[a] + s[a] * psi[b]))) for a, b in pairs])
+
+
+def taylor_reynolds(ke_physical: float, eps: float, vis: float, volume: float) -> float:
+    """Re_lambda = u' lambda / nu with u' the per-component rms velocity
+    (u'^2 = 2 KE / (3 V)) and the Taylor microscale lambda = sqrt(15 nu u'^2 / eps)."""
+    up2 = 2.0 * ke_physical / (3.0 * volume)
+    return float(up2 * np.sqrt(15.0 / (vis * eps)))
+
+
+def correction_pairs(N: int, steps: int, pairs: int, width: int, vis: float, eps: float, seed: int,
+                     k0: float = 4.0, urms: float = 1.0, tau: float = 0.01, horizon: int = 8,
+                     scheme: Optional[int] = None, series_every: int = 0) -> Dict:
+    """Training pairs for a closure of the coarse solver itself (a posteriori
+    targets), from one forced fine run.  `pairs` times the fine state is
+    box-filtered (width w) to the coarse grid, velocity and density, and a
+    coarse context started from that filtered state at the same TIME takes
+    `horizon` steps with the same forcing.  The fine run continues to the
+    coarse run's final time (the filtered velocities of the two fine steps
+    around it, interpolated in time).  The acceleration the coarse run lacked,
+        F = (ubar_fine(t_c) - u_coarse(t_c)) / (t_c - t_n),
+    holds its subgrid stress and its own discretisation error together; the
+    stress whose closure force is F (stress_from_force) is the target "T",
+    the filtered velocity gradient and the filter width at t_n the inputs "X".
+    "T_sgs" holds the exact subgrid stresses of the same states (the a-priori
+    targets).  "wall" is the fine run's own stepping time; "series" its
+    kinetic energy, dissipation and Re_lambda every `series_every` steps."""
+    from . import SCHEME_REFERENCE
+    scheme = SCHEME_REFERENCE if scheme is None else scheme
+    Nc = N // width
+    s = Solver(N, N, N, VIS=vis, precision=PREC_MIXED, scheme=scheme)
+    s.init_turbulence(seed, k0, urms)
+    s.set_forcing(seed + 1, eps, tau)
+    c = Solver(Nc, Nc, Nc, VIS=vis, precision=PREC_MIXED, scheme=scheme)
+    c.set_forcing(seed + 1, eps, tau)
+    volume = s.dx[0] * s.dx[1] * s.dx[2] * N ** 3
+    dxc = width * s.dx[0]
+    ub0 = s.filter_box(width)[0]
+    at = [int(round(steps * (q + 1) / (pairs + 1))) for q in range(pairs)]
+    Xs, Ts, Ss, info, series = [], [], [], [], []
+    state = {"done": 0, "wall": 0.0}
+
+    def fine(k):
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(k)
+        s.synchronize()
+        state["wall"] += time.perf_counter() - t0
+        for q in range(state["done"] + 1, state["done"] + k + 1):
+            if series_every and q % series_every == 0 and q == state["done"] + k:
+                d = s.diagnostics(allow_nonfinite=True)
+                e = s.dissipation()
+                series.append({"step": q, "time": d["time"], "ke": d["ke_physical"], "eps": e,
+                               "Re_lambda": taylor_reynolds(d["ke_physical"], e, vis, volume)})
+        state["done"] += k
+
+    def fine_to(target):
+        while state["done"] < target:
+            nxt = target
+            if series_every:
+                nxt = min(target, (state["done"] // series_every + 1) * series_every)
+            fine(nxt - state["done"])
+
+    for target in at:
+        fine_to(target)
+        ub, tsgs, grad = s.filter_box(width)
+        rho = s.filter_density(width)
+        t_n = s.clock()["time"]
+        c.init_state(ub, rho)
+        c.restart({"time": t_n, "nstep": 0, "dt": 0.0, "dt_next": 0.0, "dt_next_known": 0})
+        c.step(horizon)
+        t_c = c.clock()["time"]
+        uc = np.stack([c.download(n)[1:-1, 1:-1, 1:-1] for n in ("u1nCon", "u2nCon", "u3nCon")])
+        while True:                          # fine steps until the next one passes t_c
+            k = s.clock()
+            if k["time"] + k["dt_next"] >= t_c:
+                break
+            fine(1)
+        ta, ua = s.clock()["time"], s.filter_box(width)[0]
+        fine(1)
+        tb, ubb = s.clock()["time"], s.filter_box(width)[0]
+        a = (t_c - ta) / (tb - ta)
+        F = ((1.0 - a) * ua + a * ubb - uc) / (t_c - t_n)
+        T = stress_from_force(F, dxc)
+        n = ub[0].size
+        X = np.empty((n, 10), dtype=np.float32)
+        X[:, :9] = grad.reshape(9, n).T
+        X[:, 9] = dxc
+        Xs.append(X)
+        Ts.append(T.reshape(6, n).T.astype(np.float32))
+        Ss.append(tsgs.reshape(6, n).T.astype(np.float32))
+        info.append({"step": state["done"], "t_n": t_n, "t_c": t_c, "F_rms": float(np.sqrt((F ** 2).mean()))})
+    fine_to(steps)
+    d = s.diagnostics(allow_nonfinite=True)
+    e_end = s.dissipation()
+    out = {"X": np.concatenate(Xs), "T": np.concatenate(Ts), "T_sgs": np.concatenate(Ss), "pairs": info,
+           "t_end": d["time"], "ub0": ub0, "ub_end": s.filter_box(width)[0], "dx": s.dx[0], "eps_end": e_end,
+           "ke_end": d["ke_physical"], "Re_lambda_end": taylor_reynolds(d["ke_physical"], e_end, vis, volume),
+           "nonfinite": d["nonfinite"], "wall": state["wall"], "steps": state["done"], "series": series}
+    c.close()
+    s.close()
+    return out
+
+
 def spectrum_error(E: np.ndarray, Eref: np.ndarray, kmax: int) -> float:
     """Mean over shells 1..kmax of |E/Eref - 1|."""
     k = np.arange(1, kmax + 1)

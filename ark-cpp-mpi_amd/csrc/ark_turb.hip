@@ -389,6 +389,18 @@ __global__ void k_box_filter(Geo g, const double* u1, const double* u2, const do
     for (int q = 0; q < 6; ++q) tau[q * nc + o] = p[q] * inv - m[pi[q]] * m[pj[q]];
 }
 
+// Block means of one cell field over w^3 cells (the box filter of a scalar).
+__global__ void k_box_mean(Geo g, const double* f, int w, double* out) {
+    const int Nc1 = g.N1 / w, Nc2 = g.N2 / w;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y, K = blockIdx.z;
+    if (I >= Nc1) return;
+    double s = 0.0;
+    for (int c = 0; c < w; ++c)
+        for (int b = 0; b < w; ++b)
+            for (int a = 0; a < w; ++a) s += f[at(g, I * w + a + 1, J * w + b + 1, K * w + c + 1)];
+    out[((long long)K * Nc2 + J) * Nc1 + I] = s * (1.0 / ((double)w * w * w));
+}
+
 // Gradient d_j ubar_i of the filtered velocity on the coarse grid (central
 // differences, periodic), component 3*i + j.
 __global__ void k_coarse_gradient(int Nc1, int Nc2, int Nc3, const double* ubar, double h0, double h1, double h2,
@@ -544,10 +556,13 @@ dim3 interior_grid(const Geo& g, int ext) {
 // A velocity field given by its half spectrum U (three components, the
 // unnormalised forward transforms of the cell values) times `scale`: every
 // array zeroed, then the velocities on the cells (level n, both buffers, and
-// Con) and, by the half-cell shift, on the three face families; uniform rho0
-// and t0, zero pressure.  W (one component) and R (one real field) are work.
+// Con) and, by the half-cell shift, on the three face families; t0.  The
+// density is rho0 with zero pressure, or, given the half spectrum P of
+// rho - rho0 (same scaling), rho on the cells and, by the same shift, on the
+// faces with their pressures c^2 (rho - rho0).  W (one component) and R (one
+// real field) are work.
 ark_status install_velocity(ark_ctx* c, Fft& f, const hipfftDoubleComplex* U, hipfftDoubleComplex* W, double* R,
-                            double scale) {
+                            double scale, const hipfftDoubleComplex* P = nullptr) {
     const Geo& g = c->geo;
     const long long nc = (long long)f.s.H1 * g.N2 * g.N3;
     ark_status s = ARK_OK;
@@ -578,10 +593,36 @@ ark_status install_velocity(ark_ctx* c, Fft& f, const hipfftDoubleComplex* U, hi
             }
         }
     }
-    // uniform density rho0 and t0, zero pressure
+    // t0; density rho0 and zero pressure, or the given density perturbation
+    const double ro0 = c->prm.ro0, c2 = c->prm.sound * c->prm.sound;
+    if (P && s == ARK_OK) {
+        s = c->hip(hipMemcpyAsync(W, P, nc * sizeof(hipfftDoubleComplex), hipMemcpyDeviceToDevice, c->stream),
+                   "copy spectrum");
+        if (s == ARK_OK && hipfftExecZ2D(f.plan_b, W, R) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "Z2D");
+        for (int q : {F_ronCon, F_roCon})
+            for (double* dst : {c->cur[q], c->alt[q]}) {
+                if (s != ARK_OK || !dst) continue;
+                hipLaunchKernelGGL(k_scatter, interior_grid(g, 2), dim3(256), 0, c->stream, g, dst, R, scale, ro0, 2);
+                s = c->hip(hipGetLastError(), "scatter");
+            }
+        for (int d = 0; d < 3 && s == ARK_OK; ++d) {
+            hipLaunchKernelGGL(k_shift, dim3(4096), dim3(256), 0, c->stream, f.s, P, W, d);
+            s = c->hip(hipGetLastError(), "shift");
+            if (s == ARK_OK && hipfftExecZ2D(f.plan_b, W, R) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "Z2D");
+            for (int b = 0; b < 2 && s == ARK_OK; ++b) {
+                double* rf = b ? c->alt[face_ro(d)] : c->cur[face_ro(d)];
+                double* pf = b ? c->alt[face_p(d)] : c->cur[face_p(d)];
+                if (rf) hipLaunchKernelGGL(k_scatter, interior_grid(g, 3), dim3(256), 0, c->stream, g, rf, R, scale,
+                                           ro0, 3);
+                if (pf) hipLaunchKernelGGL(k_scatter, interior_grid(g, 3), dim3(256), 0, c->stream, g, pf, R,
+                                           scale * c2, 0.0, 3);
+                s = c->hip(hipGetLastError(), "scatter");
+            }
+        }
+    }
     const int cellc[4] = {F_ronCon, F_roCon, F_tnCon, F_tCon};
-    const double cellv[4] = {c->prm.ro0, c->prm.ro0, c->prm.t0, c->prm.t0};
-    for (int q = 0; q < 4 && s == ARK_OK; ++q)
+    const double cellv[4] = {ro0, ro0, c->prm.t0, c->prm.t0};
+    for (int q = P ? 2 : 0; q < 4 && s == ARK_OK; ++q)
         for (double* dst : {c->cur[cellc[q]], c->alt[cellc[q]]}) {
             if (s != ARK_OK || !dst) continue;
             hipLaunchKernelGGL(k_scatter, interior_grid(g, 2), dim3(256), 0, c->stream, g, dst, (const double*)nullptr,
@@ -590,8 +631,8 @@ ark_status install_velocity(ark_ctx* c, Fft& f, const hipfftDoubleComplex* U, hi
         }
     for (int d = 0; d < 3 && s == ARK_OK; ++d) {
         const int fq[2] = {face_ro(d), face_t(d)};
-        const double fv[2] = {c->prm.ro0, c->prm.t0};
-        for (int q = 0; q < 2; ++q)
+        const double fv[2] = {ro0, c->prm.t0};
+        for (int q = P ? 1 : 0; q < 2; ++q)
             for (double* dst : {c->cur[fq[q]], c->alt[fq[q]]}) {
                 if (s != ARK_OK || !dst) continue;
                 hipLaunchKernelGGL(k_scatter, interior_grid(g, 3), dim3(256), 0, c->stream, g, dst,
@@ -834,7 +875,7 @@ ark_status ark_set_closure_cutoff(ark_ctx* c, int kcut) {
     return s;
 }
 
-ark_status ark_init_velocity(ark_ctx* c, const double* u) {
+ark_status ark_init_state(ark_ctx* c, const double* u, const double* rho) {
     if (!c || !u) return ARK_ERR_INVALID;
     const Geo& g = c->geo;
     if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "init_velocity: N must be even");
@@ -842,17 +883,48 @@ ark_status ark_init_velocity(ark_ctx* c, const double* u) {
     ark_status s = make_fft(c, f, true);
     if (s != ARK_OK) return s;
     const long long n = (long long)g.N1 * g.N2 * g.N3, nc = (long long)f.s.H1 * g.N2 * g.N3;
+    const int nf = rho ? 4 : 3;
     DevBuf real, spec, work;
-    s = c->hip(hipMalloc(&real.p, 3 * n * sizeof(double)), "hipMalloc");
-    if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, 3 * nc * sizeof(hipfftDoubleComplex)), "hipMalloc");
+    s = c->hip(hipMalloc(&real.p, nf * n * sizeof(double)), "hipMalloc");
+    if (s == ARK_OK) s = c->hip(hipMalloc(&spec.p, nf * nc * sizeof(hipfftDoubleComplex)), "hipMalloc");
     if (s == ARK_OK) s = c->hip(hipMalloc(&work.p, nc * sizeof(hipfftDoubleComplex)), "hipMalloc");
     if (s == ARK_OK) s = c->hip(hipMemcpyAsync(real.p, u, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
     double* R = (double*)real.p;
+    if (s == ARK_OK && rho) {
+        // the density perturbation rho - rho0, formed on the host in FP64
+        std::vector<double> dr(rho, rho + n);
+        for (double& v : dr) v -= c->prm.ro0;
+        s = c->hip(hipMemcpyAsync(R + 3 * n, dr.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream), "copy");
+        if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
+    }
     hipfftDoubleComplex* U = (hipfftDoubleComplex*)spec.p;
-    for (int d = 0; d < 3 && s == ARK_OK; ++d)
+    for (int d = 0; d < nf && s == ARK_OK; ++d)
         if (hipfftExecD2Z(f.plan_f, R + d * n, U + d * nc) != HIPFFT_SUCCESS) s = c->fail(ARK_ERR_HIP, "D2Z");
     if (s != ARK_OK) return s;
-    return install_velocity(c, f, U, (hipfftDoubleComplex*)work.p, R, 1.0 / (double)n);
+    s = install_velocity(c, f, U, (hipfftDoubleComplex*)work.p, R, 1.0 / (double)n, rho ? U + 3 * nc : nullptr);
+    // face densities are ro0 + p/c^2 of the installed pressures, bit for bit
+    if (s == ARK_OK && rho) c->rho_stored = false;
+    return s;
+}
+
+ark_status ark_init_velocity(ark_ctx* c, const double* u) { return ark_init_state(c, u, nullptr); }
+
+ark_status ark_filter_density(ark_ctx* c, int width, double* rhobar) {
+    if (!c || width < 1 || !rhobar) return ARK_ERR_INVALID;
+    const Geo& g = c->geo;
+    if (g.N1 % width || g.N2 % width || g.N3 % width)
+        return c->fail(ARK_ERR_INVALID, "filter: the width must divide N1, N2, N3");
+    const int Nc1 = g.N1 / width, Nc2 = g.N2 / width, Nc3 = g.N3 / width;
+    const long long nc = (long long)Nc1 * Nc2 * Nc3;
+    DevBuf db;
+    ark_status s = c->hip(hipMalloc(&db.p, nc * sizeof(double)), "hipMalloc filter");
+    if (s != ARK_OK) return s;
+    hipLaunchKernelGGL(k_box_mean, dim3((Nc1 + 255) / 256, Nc2, Nc3), dim3(256), 0, c->stream, g, c->cur[F_ronCon],
+                       width, (double*)db.p);
+    s = c->hip(hipGetLastError(), "box mean");
+    if (s == ARK_OK)
+        s = c->hip(hipMemcpyAsync(rhobar, db.p, nc * sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy");
+    return s == ARK_OK ? c->hip(hipStreamSynchronize(c->stream), "sync") : s;
 }
 
 ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, double urms) {

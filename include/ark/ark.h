@@ -382,6 +382,14 @@ ark_status ark_set_closure_cutoff(ark_ctx* ctx, int kcut);
  * Fourier series half a cell down (as ark_init_turbulence); uniform rho0 and
  * t0, zero pressure.  Needs even N.  TIME and nStep restart at 0. */
 ark_status ark_init_velocity(ark_ctx* ctx, const double* u);
+/* ark_init_velocity with a density: rho (N1*N2*N3 interior values, x
+ * fastest) on the cells and, by the same half-cell shift of rho - rho0, on
+ * the faces, with the face pressures c^2 (rho - rho0) (the equation of state,
+ * source/main.cpp:1774-1775, 1930).  NULL rho is ark_init_velocity. */
+ark_status ark_init_state(ark_ctx* ctx, const double* u, const double* rho);
+/* The box filter of width w (dividing N) of the level-n cell density:
+ * (N1/w)(N2/w)(N3/w) block means, x fastest.  Synchronises. */
+ark_status ark_filter_density(ark_ctx* ctx, int width, double* rhobar);
 
 /* The reference's output path (source/main.cpp:2577-2891), for one rank.
  * ark_write_paraview replaces WriteDataParaView (2577-2745): the file

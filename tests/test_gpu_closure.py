@@ -95,3 +95,29 @@ def test_scale_selective_closure_acts_on_high_shells_only(pipeline):
     assert e_cut[:kc].sum() < 1e-3 * e_cut[kc:].sum(), (e_cut[:kc].sum(), e_cut[kc:].sum())
     assert e_full[:kc].sum() > 100 * e_cut[:kc].sum()
     assert abs(e_cut[kc:].sum() / e_full[kc:].sum() - 1) < 0.05
+
+
+def test_a_posteriori_closure_beats_plain_coarse_run():
+    """The kept closure: a network trained on the coarse solver's own lag
+    behind the filtered fine run (closure.correction_pairs) halves the
+    spectrum error of the 128^3 coarse run against the filtered 512^3 run.
+    Forced turbulence at nu = 1/360 (Re_lambda about 100), 1000 fine steps,
+    six 8-step correction windows, 8000 training steps; the plain coarse run
+    is the fallback.  Measured on three seeds: 14.2 / 15.2 / 14.8 % without,
+    6.9 / 7.7 / 6.8 % with the closure (shells 1..32)."""
+    vis, eps, seed = 1.0 / 360.0, 0.1, 37
+    d = C.correction_pairs(512, 1000, 6, 4, vis, eps, seed, tau=0.01, horizon=8)
+    assert d["nonfinite"] == 0 and len(d["pairs"]) == 6
+    assert d["X"].shape == (6 * 128 ** 3, 10) and d["T"].shape == (6 * 128 ** 3, 6)
+    tr = C.train(d["X"], d["T"], steps=8000, batch=65536, lr=1e-3, seed=1)
+    try:
+        Eref = C.spectrum_of(d["ub_end"], vis)
+        plain = C.les(d["ub0"], d["t_end"], vis, eps, seed, tau=0.01)
+        model = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=tr["mlp"], delta=4 * d["dx"], tau=0.01)
+    finally:
+        tr["mlp"].close()
+    assert plain["nonfinite"] == 0 and model["nonfinite"] == 0
+    e_plain = C.spectrum_error(plain["E"], Eref, 32)
+    e_model = C.spectrum_error(model["E"], Eref, 32)
+    print("spectrum error, shells 1..32: plain %.4f, closure %.4f" % (e_plain, e_model))
+    assert e_model < 0.7 * e_plain, (e_plain, e_model)

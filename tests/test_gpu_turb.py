@@ -143,3 +143,33 @@ def test_forced_turbulence_runs():
         d = s.diagnostics()
         assert d["nonfinite"] == 0 and d["nstep"] == 200
         assert s.dissipation() > 0
+
+
+@pytest.mark.parametrize("prec", [arkamd.PREC_FP64, arkamd.PREC_MIXED])
+def test_init_state_and_filter_density(prec):
+    """ark_init_state puts a given density on the cells (and, shifted half a
+    cell, on the faces, whose pressures follow the equation of state); the
+    velocities are those of ark_init_velocity; ark_filter_density returns the
+    block means of the cell density."""
+    N = 16
+    with arkamd.Solver(N, N, N, precision=prec) as a, arkamd.Solver(N, N, N, precision=prec) as b:
+        a.init_turbulence(5, 3.0, 1.0)
+        u = np.stack([a.download(n)[1:-1, 1:-1, 1:-1] for n in ("u1nCon", "u2nCon", "u3nCon")])
+        x = (np.arange(N) + 0.5) * 2 * np.pi / N
+        rho = 1.0 + 1e-3 * (np.sin(x)[None, None, :] * np.cos(2 * x)[:, None, None] + 0.5 * np.cos(x)[None, :, None])
+        b.init_state(u, rho)
+        a.init_velocity(u)
+        for n in ("u1nCon", "u22", "u13"):
+            assert np.array_equal(a.download(n), b.download(n)), n
+        got = b.download("ronCon")[1:-1, 1:-1, 1:-1]
+        assert np.abs(got - rho).max() < 1e-15
+        # x-faces at the cell's low side: rho(x - dx/2) = 1 + 1e-3 (sin(x - dx/2) cos 2z + 0.5 cos y)
+        xf = np.arange(N) * 2 * np.pi / N
+        want = 1.0 + 1e-3 * (np.sin(xf)[None, None, :] * np.cos(2 * x)[:, None, None] + 0.5 * np.cos(x)[None, :, None])
+        p1 = b.download("p1")[1:-2, 1:-2, 1:-2]
+        assert np.abs(p1 - 100.0 * (want - 1.0)).max() < (1e-6 if prec == arkamd.PREC_MIXED else 1e-12)
+        if prec == arkamd.PREC_FP64:   # face densities derived from p, bit for bit
+            ro1 = b.download("ro1")[1:-2, 1:-2, 1:-2]
+            assert np.array_equal(ro1, 1.0 + p1 / (10.0 * 10.0))
+        fb = b.filter_density(4)
+        assert np.abs(fb - got.reshape(4, 4, 4, 4, 4, 4).mean(axis=(1, 3, 5))).max() < 1e-15
