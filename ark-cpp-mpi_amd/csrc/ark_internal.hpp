@@ -48,10 +48,9 @@ struct ark_ctx {
     unsigned long long force_seed = 0;
     double force_eps = 0.0;
     double force_tau = 0.0;           // forcing correlation time (0: new amplitudes every step)
-    // learned closure (ark_set_closure): network, grid scale, its inputs and outputs
+    // learned closure (ark_set_closure): network, grid scale, its stresses (6 per cell)
     struct ark_mlp* closure = nullptr;
     double closure_delta = 0.0;
-    float* les_X = nullptr;
     float* les_Y = nullptr;
     struct ark_closure_hp* closure_hp = nullptr;   // scale-selective application (ark_set_closure_cutoff)
     double* average = nullptr;         // AverageValues' three accumulators (cell fields, common index map)
@@ -98,7 +97,9 @@ ark_status ark_cfl_scan(ark_ctx* c);                    // running MIN of the CF
 ark_status ark_finish_init(ark_ctx* c);                 // after an initial state: FP32 faces, flags, TIME = 0
 ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st);   // this step's forcing field (mixed precision)
 ark_status ark_closure_step(ark_ctx* c, hipStream_t st);      // the closure's body force (after the forcing)
-ark_status ark_mlp_forward_device(struct ark_mlp* m, const float* X, float* Y, long long n, hipStream_t st);
+// the learned closure's stresses on the solver's grid (ark_mlp.hip)
+ark_status ark_mlp_closure_device(struct ark_mlp* m, const arkd::Geo& g, const double* u1, const double* u2,
+                                  const double* u3, const double inv[3], float delta, float* Y, hipStream_t st);
 ark_status ark_cfl_step(ark_ctx* c, int advance);       // TimeStepSize on Con, then dt := MIN
 ark_status ark_cfl_finalize(ark_ctx* c, int mode);      // see k_cfl_finalize
 ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pending MIN or Con), TIME, nStep

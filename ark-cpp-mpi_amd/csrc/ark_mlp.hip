@@ -15,6 +15,7 @@
 // deterministic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -37,16 +38,37 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NI, OFF_W2 = OFF_B1 + NH, OFF_B
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NO * NH, NPARAM = OFF_B3 + NO;
 constexpr int SLOT = NPARAM + 2;        // per-wave partial: gradient, loss sum, sample count
 // BF16 copies: W1b[64][32] (inputs padded to 32), W2b[64][64], W2t[64][64] (transposed),
-// W3b[16][64] (outputs padded to 16), W3t[64][32] (transposed, padded)
+// W3b[16][64] (outputs padded to 16), W3t[64][32] (transposed, padded); and
+// the inference kernel's copies W1i[64][32], W2i[64][64], W3i[16][64] with
+// their columns in the operand order of that kernel (in_pos, hid_pos below).
 constexpr int BOFF_W1 = 0, BOFF_W2 = BOFF_W1 + 64 * 32, BOFF_W2T = BOFF_W2 + 64 * 64, BOFF_W3 = BOFF_W2T + 64 * 64,
-              BOFF_W3T = BOFF_W3 + 16 * 64, NBF = BOFF_W3T + 64 * 32;
+              BOFF_W3T = BOFF_W3 + 16 * 64, BOFF_W1I = BOFF_W3T + 64 * 32, BOFF_W2I = BOFF_W1I + 64 * 32,
+              BOFF_W3I = BOFF_W2I + 64 * 64, NBF = BOFF_W3I + 16 * 64;
+
+// Operand order of the inference kernel.  Input feature f (d_j u_i at 3i + j,
+// then the filter width) sits at K position in_pos(f): lane group g of the
+// B operand holds the three derivatives of u_g, group 3 the width.  Hidden
+// unit k sits at K position hid_pos(k) of the next layer's B operand: with
+// that order a lane's four accumulator rows of output tiles 2c and 2c+1
+// (units 16t + 4g + v) are exactly its eight B rows of K chunk c, so the
+// activations never leave the lane's registers.
+__host__ __device__ constexpr int in_pos(int f) { return f < 9 ? 8 * (f / 3) + f % 3 : 24; }
+__host__ __device__ constexpr int hid_pos(int k) {
+    return 32 * (k / 32) + 8 * ((k % 16) / 4) + 4 * ((k % 32) / 16) + k % 4;
+}
 constexpr int WAVES = 2, THREADS = 64 * WAVES;
 
 struct Norm { float in_mean[NI], in_inv[NI], out_mean[NO], out_std[NO], out_inv[NO]; };
 
 __device__ __forceinline__ f4 mma(bf16x8 a, bf16x8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ bf16x8 ld8(const __bf16* p) { return *(const bf16x8*)p; }
-__device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
+// Logistic function from the hardware exp2 and reciprocal (about 1 ulp each;
+// the activations are rounded to BF16 next, 2^-9 relative), the same in
+// training and inference.  exp2 overflows to +inf for z < -88 (then 0) and
+// underflows to 0 for large z (then 1), the logistic's own limits.
+__device__ __forceinline__ float sigm(float z) {
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
+}
 
 // Per-wave LDS: bf16 activations of one chunk of 32 samples.
 struct WaveLds {
@@ -273,14 +295,17 @@ __device__ void refresh_one(const float* P, __bf16* Wb, int p) {
     if (p >= OFF_W1 && p < OFF_B1) {
         const int h = (p - OFF_W1) / NI, i = (p - OFF_W1) % NI;
         Wb[BOFF_W1 + h * 32 + i] = (__bf16)P[p];
+        Wb[BOFF_W1I + h * 32 + in_pos(i)] = (__bf16)P[p];
     } else if (p >= OFF_W2 && p < OFF_B2) {
         const int h = (p - OFF_W2) / NH, k = (p - OFF_W2) % NH;
         Wb[BOFF_W2 + h * 64 + k] = (__bf16)P[p];
         Wb[BOFF_W2T + k * 64 + h] = (__bf16)P[p];
+        Wb[BOFF_W2I + h * 64 + hid_pos(k)] = (__bf16)P[p];
     } else if (p >= OFF_W3 && p < OFF_B3) {
         const int o = (p - OFF_W3) / NH, h = (p - OFF_W3) % NH;
         Wb[BOFF_W3 + o * 64 + h] = (__bf16)P[p];
         Wb[BOFF_W3T + h * 32 + o] = (__bf16)P[p];
+        Wb[BOFF_W3I + o * 64 + hid_pos(h)] = (__bf16)P[p];
     }
 }
 
@@ -307,27 +332,118 @@ __global__ void k_reduce_adam(const float* partial, int nslots, double* G, float
     refresh_one(P, Wb, p);
 }
 
-// Inference: Y (raw units) for n samples of X (raw units).
-__global__ void __launch_bounds__(THREADS) k_mlp_forward(const float* X, long long n, const float* P, const __bf16* Wb,
-                                                         Norm nm, float* Y) {
-    __shared__ WaveLds lds[WAVES];
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    WaveLds& L = lds[wave];
-    const int r = lane % 16, g = lane / 16;
-    const long long gw = (long long)blockIdx.x * WAVES + wave, nw = (long long)gridDim.x * WAVES;
-    for (long long ch = gw; ch < (n + 31) / 32; ch += nw) {
-        const long long pos = ch * 32 + (lane % 32);
-        stage_inputs(L, X, pos < n ? pos : 0, pos < n, nm, lane);
-        Fwd F;
-        forward(L, Wb, P, F, lane);
-        for (int nn = 0; nn < 2; ++nn) {
-            const long long s = ch * 32 + 16 * nn + r;
-            for (int v = 0; v < 4; ++v) {
-                const int o = 4 * g + v;
-                if (o < NO && s < n) Y[s * NO + o] = F.y[nn][v] * nm.out_std[o] + nm.out_mean[o];
+// Inference of 16 samples per wave and step of the loop, everything in
+// registers: the wave's weights (BF16 operands, in_pos/hid_pos order) and
+// biases are loaded once; lane (g, r) works on sample r of the step.  Layer 1
+// is four 16x16x32 products (output tile t = units 16t..16t+15), layers 2 and
+// 3 take the SiLU activations of tiles 2c and 2c+1 as their K chunk c.  The
+// samples come from X (n x 10, raw units) or, on the solver's grid, from the
+// central-difference velocity gradient of interior cell s (x fastest) and
+// the filter width; Y receives n x 6 stresses (raw units).
+struct GridIn {
+    Geo g;
+    const double* u[3];
+    double inv[3];      // 1 / (2 dx_d)
+    float delta;
+};
+
+template <bool Grid>
+__global__ void __launch_bounds__(256) k_mlp_infer(const float* X, GridIn gi, long long n, const float* P,
+                                                   const __bf16* Wb, Norm nm, float* Y) {
+    const int lane = threadIdx.x % 64, r = lane % 16, g = lane / 16;
+    bf16x8 w1[4], w2[4][2], w3[2];
+    float b1[4][4], b2[4][4], b3[4];
+    for (int t = 0; t < 4; ++t) {
+        w1[t] = ld8(Wb + BOFF_W1I + (16 * t + r) * 32 + 8 * g);
+        for (int c = 0; c < 2; ++c) w2[t][c] = ld8(Wb + BOFF_W2I + (16 * t + r) * 64 + 32 * c + 8 * g);
+        for (int v = 0; v < 4; ++v) {
+            b1[t][v] = P[OFF_B1 + 16 * t + 4 * g + v];
+            b2[t][v] = P[OFF_B2 + 16 * t + 4 * g + v];
+        }
+    }
+    for (int c = 0; c < 2; ++c) w3[c] = ld8(Wb + BOFF_W3I + r * 64 + 32 * c + 8 * g);
+    for (int v = 0; v < 4; ++v) b3[v] = 4 * g + v < NO ? P[OFF_B3 + 4 * g + v] : 0.f;
+    // this lane's three input features (group 3: the filter width only)
+    const int nf = g < 3 ? 3 : 1;
+    float im[3], ii[3], om[4], os[4];
+    for (int j = 0; j < 3; ++j) {
+        const int f = g < 3 ? 3 * g + j : 9;
+        im[j] = nm.in_mean[f];
+        ii[j] = nm.in_inv[f];
+    }
+    for (int v = 0; v < 4; ++v) {
+        const int o = 4 * g + v < NO ? 4 * g + v : 0;
+        om[v] = nm.out_mean[o];
+        os[v] = nm.out_std[o];
+    }
+    const long long nblk = (n + 15) / 16;
+    const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const long long nw = (long long)gridDim.x * blockDim.x / 64;
+    // two independent blocks of 16 samples per loop step (blk, blk + nw), so
+    // one block's products overlap the other's activations
+    for (long long b0 = w0; b0 < nblk; b0 += 2 * nw) {
+        long long sq[2];
+        bool valid[2];
+        bf16x8 bx[2];
+        for (int q = 0; q < 2; ++q) {
+            sq[q] = (b0 + q * nw) * 16 + r;
+            valid[q] = sq[q] < n;
+            const long long s = sq[q];
+            float x[3] = {0.f, 0.f, 0.f};
+            if (valid[q]) {
+                if (Grid) {
+                    if (g < 3) {
+                        // n < 2^31 cells on the solver's grid (ark_create's limit)
+                        const Geo& G = gi.g;
+                        const int si = (int)s, row = si / G.N1;
+                        const int i = si - row * G.N1 + 1, j = row % G.N2 + 1, k = row / G.N2 + 1;
+                        const int ip = i == G.N1 ? 1 : i + 1, im_ = i == 1 ? G.N1 : i - 1;
+                        const int jp = j == G.N2 ? 1 : j + 1, jm = j == 1 ? G.N2 : j - 1;
+                        const int kp = k == G.N3 ? 1 : k + 1, km = k == 1 ? G.N3 : k - 1;
+                        const double* u = gi.u[g];
+                        x[0] = (float)((u[at(G, ip, j, k)] - u[at(G, im_, j, k)]) * gi.inv[0]);
+                        x[1] = (float)((u[at(G, i, jp, k)] - u[at(G, i, jm, k)]) * gi.inv[1]);
+                        x[2] = (float)((u[at(G, i, j, kp)] - u[at(G, i, j, km)]) * gi.inv[2]);
+                    } else {
+                        x[0] = gi.delta;
+                    }
+                } else {
+                    for (int j = 0; j < nf; ++j) x[j] = X[s * NI + (g < 3 ? 3 * g + j : 9)];
+                }
+            }
+            for (int v = 0; v < 8; ++v) bx[q][v] = (__bf16)0.f;
+            if (valid[q])
+                for (int j = 0; j < nf; ++j) bx[q][j] = (__bf16)((x[j] - im[j]) * ii[j]);
+        }
+        f4 z[2][4];
+        bf16x8 h[2][2];
+        for (int q = 0; q < 2; ++q)
+            for (int t = 0; t < 4; ++t) z[q][t] = mma(w1[t], bx[q], f4{0.f, 0.f, 0.f, 0.f});
+        for (int q = 0; q < 2; ++q)
+            for (int c = 0; c < 2; ++c)
+                for (int v = 0; v < 4; ++v) {
+                    const float za = z[q][2 * c][v] + b1[2 * c][v], zb = z[q][2 * c + 1][v] + b1[2 * c + 1][v];
+                    h[q][c][v] = (__bf16)(za * sigm(za));
+                    h[q][c][4 + v] = (__bf16)(zb * sigm(zb));
+                }
+        for (int q = 0; q < 2; ++q)
+            for (int t = 0; t < 4; ++t)
+                z[q][t] = mma(w2[t][1], h[q][1], mma(w2[t][0], h[q][0], f4{0.f, 0.f, 0.f, 0.f}));
+        for (int q = 0; q < 2; ++q)
+            for (int c = 0; c < 2; ++c)
+                for (int v = 0; v < 4; ++v) {
+                    const float za = z[q][2 * c][v] + b2[2 * c][v], zb = z[q][2 * c + 1][v] + b2[2 * c + 1][v];
+                    h[q][c][v] = (__bf16)(za * sigm(za));
+                    h[q][c][4 + v] = (__bf16)(zb * sigm(zb));
+                }
+        for (int q = 0; q < 2; ++q) {
+            const f4 y = mma(w3[1], h[q][1], mma(w3[0], h[q][0], f4{0.f, 0.f, 0.f, 0.f}));
+            if (valid[q] && g < 2) {
+                float* o = Y + sq[q] * NO + 4 * g;
+                for (int v = 0; v < 4; ++v)
+                    if (4 * g + v < NO) o[v] = (y[v] + b3[v]) * os[v] + om[v];
             }
         }
-        wave_sync();
     }
 }
 
@@ -361,6 +477,13 @@ struct ark_mlp {
 namespace {
 
 constexpr int kGradBlocks = 256;
+
+// Inference grid: 4 waves per block, about eight blocks per CU of a 256-CU
+// chip, or fewer for small n (16 samples per wave and loop step).
+int infer_blocks(long long n) {
+    const long long waves = (n + 15) / 16;
+    return (int)std::max<long long>(1, std::min<long long>(2048, (waves + 3) / 4));
+}
 
 ark_status ensure_partial(ark_mlp* m) {
     if (m->partial) return ARK_OK;
@@ -553,7 +676,8 @@ ark_status ark_mlp_predict(ark_mlp* m, const float* X, float* Y, long long n) {
     if (s == ARK_OK)
         s = m->hip(hipMemcpyAsync(dX, X, n * NI * sizeof(float), hipMemcpyHostToDevice, m->stream), "copy");
     if (s == ARK_OK) {
-        hipLaunchKernelGGL(k_mlp_forward, dim3(kGradBlocks), dim3(THREADS), 0, m->stream, dX, n, m->P, m->Wb, m->nm, dY);
+        hipLaunchKernelGGL(k_mlp_infer<false>, dim3(infer_blocks(n)), dim3(256), 0, m->stream, dX, GridIn{}, n, m->P,
+                           m->Wb, m->nm, dY);
         s = m->hip(hipGetLastError(), "mlp forward");
     }
     if (s == ARK_OK)
@@ -566,12 +690,23 @@ ark_status ark_mlp_predict(ark_mlp* m, const float* X, float* Y, long long n) {
 
 }  // extern "C"
 
-// Inference on device arrays (used by the closure on the solver's stream),
-// ordered after everything already enqueued on the network's own stream.
-ark_status ark_mlp_forward_device(ark_mlp* m, const float* X, float* Y, long long n, hipStream_t st) {
+// The closure's stresses on the solver's grid (its interior cells, x fastest,
+// 6 per cell into Y), on the solver's stream, ordered after everything
+// already enqueued on the network's own stream.
+ark_status ark_mlp_closure_device(ark_mlp* m, const arkd::Geo& g, const double* u1, const double* u2, const double* u3,
+                                  const double inv[3], float delta, float* Y, hipStream_t st) {
     ark_status s = m->hip(hipEventRecord(m->done, m->stream), "event");
     if (s == ARK_OK) s = m->hip(hipStreamWaitEvent(st, m->done, 0), "wait");
     if (s != ARK_OK) return s;
-    hipLaunchKernelGGL(k_mlp_forward, dim3(kGradBlocks), dim3(THREADS), 0, st, X, n, m->P, m->Wb, m->nm, Y);
-    return m->hip(hipGetLastError(), "mlp forward");
+    GridIn gi;
+    gi.g = g;
+    gi.u[0] = u1;
+    gi.u[1] = u2;
+    gi.u[2] = u3;
+    for (int d = 0; d < 3; ++d) gi.inv[d] = inv[d];
+    gi.delta = delta;
+    const long long n = (long long)g.N1 * g.N2 * g.N3;
+    hipLaunchKernelGGL(k_mlp_infer<true>, dim3(infer_blocks(n)), dim3(256), 0, st, (const float*)nullptr, gi, n, m->P,
+                       m->Wb, m->nm, Y);
+    return m->hip(hipGetLastError(), "mlp closure");
 }

@@ -424,27 +424,10 @@ __global__ void k_coarse_gradient(int Nc1, int Nc2, int Nc3, const double* ubar,
 }
 
 // ---------------------------------------------------------------------------
-// Learned closure on the solver's grid: per cell the network's inputs (the
-// central-difference velocity gradient d_j u_i, component 3i + j, and the
-// grid scale), and from its output, the subgrid stresses tau_ij, the
+// Learned closure on the solver's grid: from the network's output, the
+// subgrid stresses tau_ij of every cell (the network and its inputs, the
+// central-difference velocity gradient, run in ark_mlp.hip), the
 // acceleration -d_j tau_ij added to (or written into) the body force.
-__global__ void k_les_inputs(Geo g, const double* u1, const double* u2, const double* u3, double inv0, double inv1,
-                             double inv2, float delta, float* X) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
-    if (i > g.N1) return;
-    const int ip = i == g.N1 ? 1 : i + 1, im = i == 1 ? g.N1 : i - 1;
-    const int jp = j == g.N2 ? 1 : j + 1, jm = j == 1 ? g.N2 : j - 1;
-    const int kp = k == g.N3 ? 1 : k + 1, km = k == 1 ? g.N3 : k - 1;
-    const double* u[3] = {u1, u2, u3};
-    float* x = X + (((size_t)(k - 1) * g.N2 + (j - 1)) * g.N1 + (i - 1)) * 10;
-    for (int m = 0; m < 3; ++m) {
-        x[3 * m + 0] = (float)((u[m][at(g, ip, j, k)] - u[m][at(g, im, j, k)]) * inv0);
-        x[3 * m + 1] = (float)((u[m][at(g, i, jp, k)] - u[m][at(g, i, jm, k)]) * inv1);
-        x[3 * m + 2] = (float)((u[m][at(g, i, j, kp)] - u[m][at(g, i, j, km)]) * inv2);
-    }
-    x[9] = delta;
-}
-
 __global__ void k_les_force(Geo g, const float* Y, float inv0, float inv1, float inv2, float* force, long long stride,
                             int add) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
@@ -783,11 +766,9 @@ void ark_closure_hp_free(ark_ctx* c) {
 ark_status ark_closure_step(ark_ctx* c, hipStream_t st) {
     if (!c->closure) return ARK_OK;
     const Geo& g = c->geo;
-    hipLaunchKernelGGL(k_les_inputs, interior_grid(g, 0), dim3(256), 0, st, g, c->cur[F_u1nCon], c->cur[F_u2nCon],
-                       c->cur[F_u3nCon], 0.5 / c->prm.dx1, 0.5 / c->prm.dx2, 0.5 / c->prm.dx3,
-                       (float)c->closure_delta, c->les_X);
-    ark_status s = c->hip(hipGetLastError(), "closure inputs");
-    if (s == ARK_OK) s = ark_mlp_forward_device(c->closure, c->les_X, c->les_Y, (long long)g.N1 * g.N2 * g.N3, st);
+    const double inv[3] = {0.5 / c->prm.dx1, 0.5 / c->prm.dx2, 0.5 / c->prm.dx3};
+    ark_status s = ark_mlp_closure_device(c->closure, g, c->cur[F_u1nCon], c->cur[F_u2nCon], c->cur[F_u3nCon], inv,
+                                          (float)c->closure_delta, c->les_Y, st);
     if (s != ARK_OK) return s;
     const int add = c->force_eps > 0.0 ? 1 : 0;
     const float inv0 = (float)(0.5 / c->prm.dx1), inv1 = (float)(0.5 / c->prm.dx2), inv2 = (float)(0.5 / c->prm.dx3);
@@ -830,12 +811,11 @@ ark_status ark_set_closure(ark_ctx* c, ark_mlp* mlp, double delta) {
     }
     ark_status s = ensure_force(c);
     const size_t n = (size_t)c->geo.N1 * c->geo.N2 * c->geo.N3;
-    if (s == ARK_OK && !c->les_X) {
-        s = c->hip(hipMalloc(&c->les_X, n * 16 * sizeof(float)), "hipMalloc closure");
+    if (s == ARK_OK && !c->les_Y) {
+        s = c->hip(hipMalloc(&c->les_Y, n * 6 * sizeof(float)), "hipMalloc closure");
         if (s == ARK_OK) {
-            c->allocs.push_back(c->les_X);
-            c->bytes += n * 16 * sizeof(float);
-            c->les_Y = c->les_X + n * 10;
+            c->allocs.push_back(c->les_Y);
+            c->bytes += n * 6 * sizeof(float);
         }
     }
     if (s == ARK_OK) {

@@ -190,6 +190,10 @@ def batched(nboxes, steps, seed=4, width=4):
         b.step(steps)
         b.synchronize()
         el = time.perf_counter() - t0
+    # the batch kernel alone (back-to-back launches of the kept configuration,
+    # as the autotuner timed them) against the graph replay of whole steps
+    # (the dt update, the two forcing launches and the batch kernel)
+    kernel_ms = tune[(cfg["shape"], cfg["lz"])]
     nonfinite = sum(s.diagnostics(allow_nonfinite=True)["nonfinite"] for s in boxes)
     t0 = time.perf_counter()
     pairs = 0
@@ -205,6 +209,8 @@ def batched(nboxes, steps, seed=4, width=4):
             "kinds": {"taylor_green": (nboxes + 1) // 2, "forced_turbulence": nboxes // 2},
             "scheme": "corrected",
             "value": round(cells * steps / el / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(el * 1e3 / steps, 4),
+            "batch_kernel_ms": round(kernel_ms, 4),
+            "outside_batch_kernel_frac": round((el * 1e3 / steps - kernel_ms) / (el * 1e3 / steps), 4),
             "config": cfg, "autotune_ms": {f"{k[0]}/lz{k[1]}": round(v, 4) for k, v in tune.items()},
             "nonfinite": nonfinite,
             "training_pairs": {"filter_width_cells": width, "pairs": pairs, "wall_s_incl_download": round(filt, 3)}}
@@ -259,6 +265,39 @@ def closure_experiment(N, steps, width, train_steps, batch, pairs=8, horizon=8):
     out["spectrum_filtered_fine"] = [float("%.5g" % v) for v in Eref[: nc // 2 + 1]]
     for name, r in (("plain", plain), ("closure", model), ("closure_a_priori", prior)):
         out["spectrum_" + name] = [float("%.5g" % v) for v in r["E"][: nc // 2 + 1]]
+    return out
+
+
+def les_timing(N, steps, width=4):
+    """The learned-closure coarse step alone: an N^3 forced-turbulence run
+    (nu = 1/360, mixed precision, reference scheme) with and without a closure
+    network attached (weights from its seeded initialisation: the cost does
+    not depend on them), ms per step of each, and the closure kernels' share
+    (MLP inference, its inputs and the stress divergence) by HIP events."""
+    import arkamd
+    out = {}
+    for name in ("plain", "closure"):
+        s = arkamd.Solver(N, N, N, VIS=1.0 / 360.0, precision=arkamd.PREC_MIXED)
+        s.init_turbulence(31, 4.0, 1.0)
+        s.set_forcing(32, 0.1, 0.01)
+        m = None
+        if name == "closure":
+            m = arkamd.MLP(seed=1)
+            s.set_closure(m, width * s.dx[0])
+        s.step(5)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        out[name] = {"ms_per_step": round((time.perf_counter() - t0) * 1e3 / steps, 4)}
+        s.close()
+        if m is not None:
+            m.close()
+    out["closure_overhead_ms_per_step"] = round(out["closure"]["ms_per_step"] - out["plain"]["ms_per_step"], 4)
+    flop = 2 * (10 * 64 + 64 * 64 + 64 * 6) * N ** 3
+    out["mlp_flop_per_step"] = flop
+    out["grid"] = N
+    out["steps"] = steps
     return out
 
 
@@ -429,7 +468,7 @@ def main():
                     help="steps of the CPU baseline's 256^3 sample (about 45 s on 16 host cores)")
     ap.add_argument("--cpu-ranks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["fp64", "mixed", "channel"], default=None,
+    ap.add_argument("--only", choices=["fp64", "mixed", "channel", "batched", "closure", "les"], default=None,
                     help="time one precision only, or only the reference's channel (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
@@ -445,6 +484,16 @@ def main():
     import arkamd
     if args.only == "channel":
         print(json.dumps({"reference_channel": reference_channel(args.channel_steps, 0)}))
+        return
+    if args.only == "batched":
+        print(json.dumps({"batched": batched(args.batch_boxes, args.batch_steps)}))
+        return
+    if args.only == "les":
+        print(json.dumps({"les_timing": les_timing(args.closure_size // 4, args.steps)}))
+        return
+    if args.only == "closure":
+        print(json.dumps({"forced_turbulence_closure": closure_experiment(
+            args.closure_size, args.closure_steps, 4, args.closure_train_steps, 65536)}))
         return
     N = args.size
     cells = N ** 3

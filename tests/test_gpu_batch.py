@@ -125,3 +125,23 @@ def test_batch_rejects_bad_boxes():
         arkamd.Batch([m, m])
     f64.close()
     m.close()
+
+
+def test_batch_config4_scale_equals_standalone():
+    """Boxes of the ensemble's larger edges (64, 96, 128), Taylor-Green and
+    forced turbulence mixed, Re spread over [400, 3200], autotuned tiling:
+    every box after 5 batched steps is bit for bit its standalone run."""
+    boxes = [((64, 64, 64), 1 / 400), ((96, 96, 96), 1 / 1800), ((128, 128, 128), 1 / 3200),
+             ((96, 96, 96), 1 / 700), ((64, 64, 64), 1 / 2600), ((128, 128, 128), 1 / 1200)]
+    ref, bat = make_mixed_kinds(boxes, 0.01), make_mixed_kinds(boxes, 0.01)
+    for s in ref:
+        s.step(5)
+    with arkamd.Batch(bat) as b:
+        b.autotune(reps=1)
+        b.step(2)
+        b.step(3)
+    assert_same(ref, bat, "config-4 scale")
+    for s in bat:
+        assert s.diagnostics()["nonfinite"] == 0
+    for s in ref + bat:
+        s.close()
