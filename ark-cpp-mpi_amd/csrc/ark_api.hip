@@ -140,6 +140,13 @@ ark_status ark_create(const ark_params* p, ark_ctx** out) {
         *out = c;
         return ARK_ERR_UNSUPPORTED;
     }
+    // Mixed precision: a face record is 16 bytes, also at a 32-bit offset.
+    if (p->precision == ARK_PREC_MIXED && elems * 16 >= (1ll << 32)) {
+        c->err = "grid too large for mixed precision: a face record's byte offset must fit in 32 bits (at most "
+                 "2^28 elements per field, about 640^3 cells)";
+        *out = c;
+        return ARK_ERR_UNSUPPORTED;
+    }
     g.px = (int)px;
     g.sxy = (int)(px * (p->N2 + 3));
     c->elems = elems + kOriginPad + 16;
@@ -935,16 +942,8 @@ ark_status ark_init_tgv(ark_ctx* c, double U0) {
 ark_status ark_finish_init(ark_ctx* c) {
     ark_status s = ARK_OK;
     if (c->prm.precision == ARK_PREC_MIXED) {
-        // FP32 face storage: convert each face field (both buffers hold it)
-        for (int g = 1; g < 4 && s == ARK_OK; ++g)
-            for (int v = 0; v < 5 && s == ARK_OK; ++v) {
-                const int f = kGroupFields[g][v];
-                // FP64 values in cur -> staging -> FP32 into cur and alt (never in place)
-                s = c->hip(hipMemcpyAsync(c->staging - kOriginPad, c->cur[f] - kOriginPad,
-                                          c->elems * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
-                           "copy to staging");
-                if (s == ARK_OK) s = ark_mixed_face_from_host_layout(c, f, c->staging);
-            }
+        // FP32 face records from the FP64 face fields just written
+        s = ark_mixed_faces_from_fp64(c);
         if (s == ARK_OK) s = c->hip(hipStreamSynchronize(c->stream), "sync");
     }
     c->has_solid = false;

@@ -125,10 +125,15 @@ extern const int kGroupFields[4][5];
 void ark_swap_group(ark_ctx* c, int g);
 
 // Mixed precision (ark_mixed.hip): fused steps, and the conversion of one
-// face field between its FP32 storage and an FP64 field in the common layout.
+// face field between its FP32 record storage and an FP64 field in the common
+// layout.
 ark_status ark_mixed_steps(ark_ctx* c, int nsteps);
 ark_status ark_mixed_face_to_host_layout(ark_ctx* c, int f, double* staging);
 ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* staging);
+// After an initial state in FP64 face fields (both buffers): the FP32 records.
+ark_status ark_mixed_faces_from_fp64(ark_ctx* c);
+// The reference's vorticity from the face records (ark_write_paraview).
+ark_status ark_mixed_vorticity(ark_ctx* c, double* o1, double* o2, double* o3);
 
 // Chunk tables of the z-marching fused steps (ark_mixed.hip): (tile column
 // x fastest then y, first plane, end plane) per chunk, for tiles of TX x TY

@@ -500,17 +500,16 @@ __global__ void __launch_bounds__(256) k_phase2_xyz(Geo g, Fields F, NextFaces n
 // WriteDataParaView (source/main.cpp:2712-2733), interior cells; T is the
 // face storage type (double, or float for mixed precision).  out[m] holds
 // component m in the common index map.
-template <class T>
 __global__ void __launch_bounds__(256) k_vorticity(Geo g, Fields F, Consts K, double* o1, double* o2, double* o3) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
     if (i > g.N1) return;
     double* const* a = F.a;
-    const T* u32 = (const T*)a[F_u32];
-    const T* u23 = (const T*)a[F_u23];
-    const T* u13 = (const T*)a[F_u13];
-    const T* u31 = (const T*)a[F_u31];
-    const T* u21 = (const T*)a[F_u21];
-    const T* u12 = (const T*)a[F_u12];
+    const double* u32 = a[F_u32];
+    const double* u23 = a[F_u23];
+    const double* u13 = a[F_u13];
+    const double* u31 = a[F_u31];
+    const double* u21 = a[F_u21];
+    const double* u12 = a[F_u12];
     const int c = at(g, i, j, k);
     const double dx1 = K.dx[0], dx2 = K.dx[1], dx3 = K.dx[2];
     double d2u3 = ((double)u32[c + g.px] - (double)u32[c]) / dx2,
@@ -754,11 +753,10 @@ hipError_t launch_phase2_xyz(hipStream_t st, int scheme, const Geo& g, const Fie
     return hipGetLastError();
 }
 
-hipError_t launch_vorticity(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, bool float_faces,
+hipError_t launch_vorticity(hipStream_t st, const Geo& g, const Fields& F, const Consts& K,
                             double* o1, double* o2, double* o3) {
     dim3 grid((g.N1 + 255) / 256, g.N2, g.N3);
-    if (float_faces) hipLaunchKernelGGL(k_vorticity<float>, grid, dim3(256), 0, st, g, F, K, o1, o2, o3);
-    else hipLaunchKernelGGL(k_vorticity<double>, grid, dim3(256), 0, st, g, F, K, o1, o2, o3);
+    hipLaunchKernelGGL(k_vorticity, grid, dim3(256), 0, st, g, F, K, o1, o2, o3);
     return hipGetLastError();
 }
 

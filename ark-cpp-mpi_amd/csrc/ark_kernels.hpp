@@ -37,7 +37,7 @@ hipError_t launch_cfl_finalize(hipStream_t st, Scalars* S, int advance);
 hipError_t launch_clock(hipStream_t st, Scalars* S, int what);   // 0: nStep++, 1: TIME += 0.5 dt
 hipError_t launch_phase1(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, const Scalars* S);
 hipError_t launch_exchange(hipStream_t st, const Geo& g, const ExchangePlan& plan);
-hipError_t launch_vorticity(hipStream_t st, const Geo& g, const Fields& F, const Consts& K, bool float_faces,
+hipError_t launch_vorticity(hipStream_t st, const Geo& g, const Fields& F, const Consts& K,
                             double* o1, double* o2, double* o3);
 hipError_t launch_average(hipStream_t st, const Geo& g, const Fields& F, const Scalars* S, double* a1, double* a2,
                           double* a3);

@@ -1,7 +1,8 @@
 // Mixed-precision fused CABARET/ARK step.
 //
 // Storage: the level-n cell fields stay FP64 (rho, u1, u2, u3, t); the face
-// fields are FP32: u1, u2, u3, p and t' = t - t0.  The face density is
+// fields are FP32: u1, u2, u3, p and t' = t - t0, as one record per face:
+// (u1, u2, u3, p) a float4, t' a float of its own (faces_of below).  The face density is
 // rho0 + p/c^2 and is never stored; p itself is the density perturbation
 // c^2 (rho - rho0), so the weakly compressible equation of state
 // p = c^2 (rho - rho0) (source/main.cpp:1774-1775) never subtracts two
@@ -90,11 +91,16 @@ __device__ __forceinline__ double ldd(const char* grp, unsigned long long fb, in
 __device__ __forceinline__ void std_(char* grp, unsigned long long fb, int v, int idx, double x) {
     *(double*)(grp + v * fb + (unsigned)idx * 8u) = x;
 }
-__device__ __forceinline__ float ldf(const char* grp, unsigned long long fb, int v, int idx) {
-    return *(const float*)(grp + v * fb + (unsigned)idx * 4u);
+// Face records of one orientation, in the allocation of its five FP64 fields
+// (group pointer grp = FP64 element (0,0,0), 120 bytes into the allocation):
+// face idx has (u1, u2, u3, p) as the float4 at grp + 8 + 16 idx (16-byte
+// aligned) and t' as the float at grp + 2 fb + 4 idx.  One record is two
+// memory instructions instead of five.
+__device__ __forceinline__ const float4* rec_of(const char* grp, unsigned idx) {
+    return (const float4*)(grp + 8 + idx * 16u);
 }
-__device__ __forceinline__ void stf(char* grp, unsigned long long fb, int v, int idx, float x) {
-    *(float*)(grp + v * fb + (unsigned)idx * 4u) = x;
+__device__ __forceinline__ const float* tp_of(const char* grp, unsigned long long fb, unsigned idx) {
+    return (const float*)(grp + 2 * fb + idx * 4u);
 }
 
 __device__ __forceinline__ int wrapc(int q, int N) { return q < 1 ? q + N : (q > N ? q - N : q); }
@@ -122,11 +128,12 @@ __device__ __forceinline__ CellF to_f(const CellD& c, const Args& a) {
 }
 __device__ __forceinline__ FaceF load_face_g(const char* grp, unsigned long long fb, int idx) {
     FaceF f;
-    f.u[0] = ldf(grp, fb, 0, idx);
-    f.u[1] = ldf(grp, fb, 1, idx);
-    f.u[2] = ldf(grp, fb, 2, idx);
-    f.p = ldf(grp, fb, 3, idx);
-    f.tp = ldf(grp, fb, 4, idx);
+    const float4 q = *rec_of(grp, (unsigned)idx);
+    f.u[0] = q.x;
+    f.u[1] = q.y;
+    f.u[2] = q.z;
+    f.p = q.w;
+    f.tp = *tp_of(grp, fb, (unsigned)idx);
     return f;
 }
 __device__ __forceinline__ FaceF load_face(const Args& a, int d, int idx) { return load_face_g(a.F[d], a.fb, idx); }
@@ -251,11 +258,12 @@ __device__ __forceinline__ Flux out_flux(const FaceOutF& o, const Args& a) {
 template <int D>
 __device__ __forceinline__ void store_face(const Args& a, int idx, const FaceOutF& o) {
     constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
-    stf(a.Fo[D], a.fb, D, idx, o.un);
-    stf(a.Fo[D], a.fb, TA, idx, o.a);
-    stf(a.Fo[D], a.fb, TB, idx, o.b);
-    stf(a.Fo[D], a.fb, 3, idx, o.p);
-    stf(a.Fo[D], a.fb, 4, idx, o.t);
+    float u[3];
+    u[D] = o.un;
+    u[TA] = o.a;
+    u[TB] = o.b;
+    *(float4*)rec_of(a.Fo[D], (unsigned)idx) = make_float4(u[0], u[1], u[2], o.p);
+    *(float*)tp_of(a.Fo[D], a.fb, (unsigned)idx) = o.t;
 }
 
 template <int D>
@@ -632,19 +640,43 @@ __global__ void k_batch_begin_step(Scalars* const* __restrict__ S, int nboxes) {
 }
 
 // ---------------------------------------------------------------------------
-// Conversions between the FP64 API arrays and the FP32 face storage.
-// mode 0: u or p (plain), mode 1: t (t' = t - t0).
-__global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, long long n, int mode, double t0) {
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
-        dst[q] = (float)(mode == 1 ? src[q] - t0 : src[q]);
-}
-// mode 0: plain, mode 1: t = t0 + t', mode 2: rho = rho0 + p/c^2 from p
-__global__ void k_to_double(const float* __restrict__ src, double* __restrict__ dst, long long n, int mode,
-                            double t0, double ro0, double c2) {
+// Conversions between an FP64 field (element (0,0,0) at src/dst, elements
+// 0..n-1) and component v of the face records of a group (v < 4: u1, u2, u3,
+// p; v = 4: t').  To the records: t' = t - t0 for v = 4.  From them: t0 + t'
+// for v = 4, and with rho set, rho0 + p/c^2 from p.
+__global__ void k_face_from_double(const double* __restrict__ src, char* grp, unsigned long long fb, int v,
+                                   long long n, double t0) {
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
-        const double v = (double)src[q];
-        dst[q] = mode == 1 ? t0 + v : (mode == 2 ? ro0 + v / c2 : v);
+        if (v == 4) *(float*)tp_of(grp, fb, (unsigned)q) = (float)(src[q] - t0);
+        else ((float*)rec_of(grp, (unsigned)q))[v] = (float)src[q];
     }
+}
+__global__ void k_face_to_double(const char* grp, unsigned long long fb, int v, int rho, long long n, double t0,
+                                 double ro0, double c2, double* __restrict__ dst) {
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        if (v == 4) {
+            dst[q] = t0 + (double)*tp_of(grp, fb, (unsigned)q);
+        } else {
+            const double x = (double)((const float*)rec_of(grp, (unsigned)q))[v];
+            dst[q] = rho ? ro0 + x / c2 : x;
+        }
+    }
+}
+
+// The reference's vorticity from the face velocities (source/main.cpp:
+// 2712-2733, as k_vorticity) on the face records of the three orientations.
+__global__ void k_vorticity_records(Geo g, const char* fx, const char* fy, const char* fz, double dx1, double dx2,
+                                    double dx3, double* o1, double* o2, double* o3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1, k = blockIdx.z + 1;
+    if (i > g.N1) return;
+    const unsigned c = (unsigned)at(g, i, j, k);
+    auto u = [&](const char* grp, unsigned q, int m) { return (double)((const float*)rec_of(grp, q))[m]; };
+    const double d2u3 = (u(fy, c + g.px, 2) - u(fy, c, 2)) / dx2, d3u2 = (u(fz, c + g.sxy, 1) - u(fz, c, 1)) / dx3,
+                 d3u1 = (u(fz, c + g.sxy, 0) - u(fz, c, 0)) / dx3, d1u3 = (u(fx, c + 1, 2) - u(fx, c, 2)) / dx1,
+                 d1u2 = (u(fx, c + 1, 1) - u(fx, c, 1)) / dx1, d2u1 = (u(fy, c + g.px, 0) - u(fy, c, 0)) / dx2;
+    o1[c] = d2u3 - d3u2;
+    o2[c] = d3u1 - d1u3;
+    o3[c] = d1u2 - d2u1;
 }
 
 }  // namespace mixed
@@ -819,7 +851,19 @@ ark_status tune_single_shape(ark_ctx* c) {
     return s;
 }
 
-int face_mode(int f) { return (f == F_t1 || f == F_t2 || f == F_t3) ? 1 : 0; }
+// Group (1..3) and record component (0..4) of a face field.
+int face_slot(int f, int* v) {
+    for (int g = 1; g < 4; ++g)
+        for (int q = 0; q < 5; ++q)
+            if (kGroupFields[g][q] == f) {
+                *v = q;
+                return g;
+            }
+    return -1;
+}
+
+// Elements of a field in the common index map (the records cover these).
+long long face_elems(const ark_ctx* c) { return (long long)c->elems - kOriginPad - 16; }
 
 }  // namespace
 
@@ -925,36 +969,65 @@ ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     return s;
 }
 
-// The FP32 storage of one face field, as seen through the FP64 API.  A face
-// field's FP32 element i sits 4*i bytes from the field's element (0,0,0),
-// its FP64 element i 8*i bytes; both views cover elems elements starting
-// 15 elements before (0,0,0).
+// The FP32 storage of one face field, as seen through the FP64 API (the face
+// densities ro1..ro3 from the pressures).
 ark_status ark_mixed_face_to_host_layout(ark_ctx* c, int f, double* staging) {
     const ark_params& P = c->prm;
-    const long long n = (long long)c->elems;
-    if (f == F_ro1 || f == F_ro2 || f == F_ro3) {
-        const int pf = f == F_ro1 ? F_p1 : f == F_ro2 ? F_p2 : F_p3;
-        hipLaunchKernelGGL(mixed::k_to_double, dim3(2048), dim3(256), 0, c->stream,
-                           (const float*)c->cur[pf] - 15, staging - 15, n, 2, P.t0, P.ro0, P.sound * P.sound);
-    } else {
-        hipLaunchKernelGGL(mixed::k_to_double, dim3(2048), dim3(256), 0, c->stream,
-                           (const float*)c->cur[f] - 15, staging - 15, n, face_mode(f), P.t0, P.ro0,
-                           P.sound * P.sound);
-    }
-    return c->hip(hipGetLastError(), "to_double");
+    const bool rho = f == F_ro1 || f == F_ro2 || f == F_ro3;
+    const int fld = f == F_ro1 ? F_p1 : f == F_ro2 ? F_p2 : f == F_ro3 ? F_p3 : f;
+    int v = 0;
+    const int g = face_slot(fld, &v);
+    if (g < 0) return c->fail(ARK_ERR_INVALID, "not a face field");
+    hipLaunchKernelGGL(mixed::k_face_to_double, dim3(2048), dim3(256), 0, c->stream, c->gbase[g][c->gpar[g]],
+                       (unsigned long long)c->field_bytes, v, rho ? 1 : 0, face_elems(c), P.t0, P.ro0,
+                       P.sound * P.sound, staging);
+    return c->hip(hipGetLastError(), "face to double");
 }
 
-// FP64 values (staging, field layout) into the FP32 storage of face field f,
-// both buffers.
-ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* staging) {
-    const long long n = (long long)c->elems;
-    for (double* dst : {c->cur[f], c->alt[f]}) {
-        hipLaunchKernelGGL(mixed::k_to_float, dim3(2048), dim3(256), 0, c->stream, staging - 15,
-                           (float*)dst - 15, n, face_mode(f), c->prm.t0);
-        ark_status s = c->hip(hipGetLastError(), "to_float");
+// FP64 values (src, field layout) into the records of face field f in the
+// group buffers named by `which` (1: current, 2: other, 3: both).
+static ark_status face_from(ark_ctx* c, int f, const double* src, int which) {
+    int v = 0;
+    const int g = face_slot(f, &v);
+    if (g < 0) return c->fail(ARK_ERR_INVALID, "not a face field");
+    for (int b = 0; b < 2; ++b) {
+        if (!(which & (1 << b))) continue;
+        hipLaunchKernelGGL(mixed::k_face_from_double, dim3(2048), dim3(256), 0, c->stream, src,
+                           c->gbase[g][c->gpar[g] ^ b], (unsigned long long)c->field_bytes, v, face_elems(c),
+                           c->prm.t0);
+        ark_status s = c->hip(hipGetLastError(), "face from double");
         if (s != ARK_OK) return s;
     }
     return ARK_OK;
+}
+
+ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* staging) {
+    return face_from(c, f, staging, 3);
+}
+
+// After an initial state written as FP64 fields into both buffers of each
+// face group: the records of both buffers.  The records overlap the FP64
+// fields of their own allocation, so each group is converted from the other
+// buffer's FP64 copy, then copied over it.
+ark_status ark_mixed_faces_from_fp64(ark_ctx* c) {
+    const size_t group_bytes = 5 * c->field_bytes;
+    ark_status s = ARK_OK;
+    for (int g = 1; g < 4 && s == ARK_OK; ++g) {
+        char* cur = c->gbase[g][c->gpar[g]] - kOriginPad * sizeof(double);
+        char* oth = c->gbase[g][c->gpar[g] ^ 1] - kOriginPad * sizeof(double);
+        s = c->hip(hipMemcpyAsync(oth, cur, group_bytes, hipMemcpyDeviceToDevice, c->stream), "copy faces");
+        for (int v = 0; v < 5 && s == ARK_OK; ++v) s = face_from(c, kGroupFields[g][v], c->alt[kGroupFields[g][v]], 1);
+        if (s == ARK_OK) s = c->hip(hipMemcpyAsync(oth, cur, group_bytes, hipMemcpyDeviceToDevice, c->stream), "copy faces");
+    }
+    return s;
+}
+
+ark_status ark_mixed_vorticity(ark_ctx* c, double* o1, double* o2, double* o3) {
+    const Geo& g = c->geo;
+    hipLaunchKernelGGL(mixed::k_vorticity_records, dim3((g.N1 + 255) / 256, g.N2, g.N3), dim3(256), 0, c->stream, g,
+                       c->gbase[1][c->gpar[1]], c->gbase[2][c->gpar[2]], c->gbase[3][c->gpar[3]], c->prm.dx1,
+                       c->prm.dx2, c->prm.dx3, o1, o2, o3);
+    return c->hip(hipGetLastError(), "vorticity");
 }
 
 // ---------------------------------------------------------------------------

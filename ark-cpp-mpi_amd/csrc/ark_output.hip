@@ -118,9 +118,11 @@ ark_status ark_write_paraview(ark_ctx* c, const char* dir_path, int need_swap) {
     if (s == ARK_OK) s = c->hip(hipMalloc(&t.p, 3 * c->elems * sizeof(double)), "hipMalloc vorticity");
     if (s == ARK_OK) {
         double* o = t.p + kOriginPad;
-        s = c->hip(launch_vorticity(c->stream, c->geo, c->fields(), c->K, c->prm.precision == ARK_PREC_MIXED, o,
-                                    o + c->elems, o + 2 * c->elems),
-                   "vorticity");
+        if (c->prm.precision == ARK_PREC_MIXED)
+            s = ark_mixed_vorticity(c, o, o + c->elems, o + 2 * c->elems);
+        else
+            s = c->hip(launch_vorticity(c->stream, c->geo, c->fields(), c->K, o, o + c->elems, o + 2 * c->elems),
+                       "vorticity");
         for (int m = 0; m < 3 && s == ARK_OK; ++m) s = interior_of(c, o + m * c->elems, w[m]);
     }
     if (s != ARK_OK) return s;

@@ -274,3 +274,30 @@ def test_mixed_256_tile_shapes_bitwise(scheme):
             outs.append({n: s.download(n) for n in ("ronCon", "u1nCon", "u3nCon", "p1", "u22", "u33", "t3")})
     for n in outs[0]:
         assert bits_equal(outs[0][n], outs[1][n]), n
+
+
+def test_mixed_vtk_vorticity_from_face_records(tmp_path):
+    """ark_write_paraview on a mixed-precision context: its vorticity R is the
+    reference's face-difference expression (source/main.cpp:2712-2733) on the
+    FP32 face values, which download exactly as FP64; recomputed here in FP64
+    from the downloaded faces, bit for bit."""
+    N1, N2, N3 = 12, 10, 8
+    with arkamd.Solver(N1, N2, N3, precision=arkamd.PREC_MIXED, scheme=arkamd.SCHEME_CORRECTED) as s:
+        s.init_tgv()
+        s.step(3)
+        path = s.write_paraview(str(tmp_path))
+        f = {n: s.download(n) for n in ("u12", "u13", "u21", "u23", "u31", "u32")}
+        dx1, dx2, dx3 = s.dx
+    data = open(path, "rb").read()
+    at = data.index(b"VECTORS R double\n") + len(b"VECTORS R double\n")
+    R = np.frombuffer(data[at:at + 3 * N1 * N2 * N3 * 8], dtype=">f8").reshape(N3, N2, N1, 3)
+    I, J, K = slice(1, N1 + 1), slice(1, N2 + 1), slice(1, N3 + 1)
+    Jp, Kp, Ip = slice(2, N2 + 2), slice(2, N3 + 2), slice(2, N1 + 2)
+    d2u3 = (f["u32"][K, Jp, I] - f["u32"][K, J, I]) / dx2
+    d3u2 = (f["u23"][Kp, J, I] - f["u23"][K, J, I]) / dx3
+    d3u1 = (f["u13"][Kp, J, I] - f["u13"][K, J, I]) / dx3
+    d1u3 = (f["u31"][K, J, Ip] - f["u31"][K, J, I]) / dx1
+    d1u2 = (f["u21"][K, J, Ip] - f["u21"][K, J, I]) / dx1
+    d2u1 = (f["u12"][K, Jp, I] - f["u12"][K, J, I]) / dx2
+    want = np.stack([d2u3 - d3u2, d3u1 - d1u3, d1u2 - d2u1], axis=-1)
+    assert bits_equal(R.astype(np.float64), want)
