@@ -34,8 +34,8 @@
 // This file is not bound to the reference's rounding (its results are
 // checked against the FP64 path with a tolerance), so a*b+c may become one
 // fused multiply-add and divisions by cell quantities use the hardware
-// reciprocal (1 ulp in FP32; refined to full FP64 precision by two Newton
-// steps in the closing update).
+// reciprocal (1 ulp in FP32; in the closing update refined by one Newton
+// step to about 2^-46, far below the FP32 flux arithmetic's rounding).
 #pragma clang fp contract(fast)
 
 namespace arkd {
@@ -43,8 +43,7 @@ namespace mixed {
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcpd(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    y = fma(y, fma(-x, y, 1.0), y);
+    const double y = __builtin_amdgcn_rcp(x);
     return fma(y, fma(-x, y, 1.0), y);
 }
 
@@ -165,7 +164,10 @@ struct Side {   // one cell along the sweep axis: level n (c), half level (cn), 
 struct FwdF { float r, t, a, b; };
 struct BwdF { float q, t, a, b; };
 
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+// The limiter's clamp.  Its bounds are min3 + dt g and max3 + dt g with the
+// same dt g, so lo <= hi, and the median of (v, lo, hi) is min(max(v, lo), hi)
+// exactly: one v_med3_f32.
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return __builtin_amdgcn_fmed3f(v, lo, hi); }
 __device__ __forceinline__ float max3f(float x, float y, float z) { return fmaxf(fmaxf(x, y), z); }
 __device__ __forceinline__ float min3f(float x, float y, float z) { return fminf(fminf(x, y), z); }
 
@@ -299,14 +301,15 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     __shared__ float4 s_h[NTHR];             // half level: u1, u2, u3, t'
     __shared__ float4 s_fx[NTHR], s_fy[NTHR];  // forward extrapolations along x, y
     __shared__ float s_m[10][NTHR];          // new x- and y-face fluxes
-    // Level-n velocities of a plane and the x- and y-faces of each lane's
-    // cell, with a halo that the edge lanes fill with the value beyond the
-    // region, so every neighbour read is one unconditional LDS load.  The
-    // velocities of two planes are held (p and p+1); the faces of one plane,
-    // read before the plane's first barrier and rewritten after it.
+    // Level-n velocities of a plane (FP32: the stress is FP32 arithmetic) and
+    // the x- and y-faces of each lane's cell, with a halo that the edge lanes
+    // fill with the value beyond the region, so every neighbour read is one
+    // unconditional LDS load.  The velocities of two planes are held (p and
+    // p+1); the faces of one plane, read before the plane's first barrier and
+    // rewritten after it.
     constexpr int UX = RX + 2, UN = (RX + 2) * (RY + 2);   // velocities: one-cell halo all round
     constexpr int FX = RX + 1, FN = (RX + 1) * (RY + 1);   // faces: one more column and row
-    __shared__ double s_u[2][3][UN];
+    __shared__ float s_u[2][3][UN];
     __shared__ float s_f[10][FN];
     const Geo& g = a.g;
     const double dtd = a.S->dt;
@@ -377,11 +380,11 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
             else put_face(f + 5 * FN, fpos + FX, r.ex);
         }
         if (put_o)
-            for (int m = 0; m < 3; ++m) s_u[p & 1][m][opos] = r.ou[m];
+            for (int m = 0; m < 3; ++m) s_u[p & 1][m][opos] = (float)r.ou[m];
     };
     auto put_u = [&](int p, const CellD& c) {
         if (in_region)
-            for (int m = 0; m < 3; ++m) s_u[p & 1][m][upos] = c.u[m];
+            for (int m = 0; m < 3; ++m) s_u[p & 1][m][upos] = (float)c.u[m];
     };
     auto staged_face = [&](int q, int off) {
         const float* f = &s_f[off][0];
@@ -397,7 +400,7 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     float sz_lo[3];
     {
         const CellD Ab = load_cell(A, base + wrapc(k0 - 2, g.N3) * sxy);
-        for (int m = 0; m < 3; ++m) sz_lo[m] = A.VIS * (float)(Acur.u[m] - Ab.u[m]) * A.inv_dx[2];
+        for (int m = 0; m < 3; ++m) sz_lo[m] = A.VIS * ((float)Acur.u[m] - (float)Ab.u[m]) * A.inv_dx[2];
     }
     PlaneIn cur = load_plane(k0 - 1);
     FaceF zb = load_face(A, 2, base + wrapc(k0 - 1, g.N3) * sxy);
@@ -428,24 +431,22 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
             nxt = load_plane(p + 1);
             Anext2 = load_cell(A, base + wrapc(p + 2, g.N3) * sxy);
         }
-        CellD Ac = Acur;
-        for (int m = 0; m < 3; ++m) Ac.u[m] = s_u[p & 1][m][upos];
-        const CellF Af = to_f(Ac, A);
+        const CellF Af = to_f(Acur, A);
 
-        // stress force (FP32 differences of FP64 velocities)
+        // stress force (FP32 differences of the FP32-rounded velocities)
         float f[3];
         {
             const float roc = (float)Acur.ro;
             for (int m = 0; m < 3; ++m) {
-                const double* u = &s_u[p & 1][m][0];
-                const double uc = Ac.u[m];
-                const double uW = u[upos - 1], uE = u[upos + 1];
-                const double uS = u[upos - UX], uN = u[upos + UX];
-                const float sxl = A.VIS * (float)(uc - uW) * A.inv_dx[0];
-                const float sxh = A.VIS * (float)(uE - uc) * A.inv_dx[0];
-                const float syl = A.VIS * (float)(uc - uS) * A.inv_dx[1];
-                const float syh = A.VIS * (float)(uN - uc) * A.inv_dx[1];
-                const float szh = A.VIS * (float)(s_u[(p + 1) & 1][m][upos] - uc) * A.inv_dx[2];
+                const float* u = &s_u[p & 1][m][0];
+                const float uc = Af.u[m];
+                const float uW = u[upos - 1], uE = u[upos + 1];
+                const float uS = u[upos - UX], uN = u[upos + UX];
+                const float sxl = A.VIS * (uc - uW) * A.inv_dx[0];
+                const float sxh = A.VIS * (uE - uc) * A.inv_dx[0];
+                const float syl = A.VIS * (uc - uS) * A.inv_dx[1];
+                const float syh = A.VIS * (uN - uc) * A.inv_dx[1];
+                const float szh = A.VIS * (s_u[(p + 1) & 1][m][upos] - uc) * A.inv_dx[2];
                 f[m] = ((sxh - sxl) * A.ds[0] + (syh - syl) * A.ds[1] + (szh - sz_lo[m]) * A.ds[2]) * roc +
                        roc * A.dvcf * cur.fo[m];
                 sz_lo[m] = szh;
@@ -577,10 +578,11 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         Hprev = H;
         fz_prev = fz;
         for (int v = 0; v < 5; ++v) {
-            double inc = (double)Sp[v] + (double)(Sxy[v] - mz.m[v] * A.ds[2]);
-            if (v >= 1 && v <= 3) inc -= 2.0 * (double)f[v - 1];
-            const double q = v == 0 ? Acur.ro : Acur.ro * (v == 4 ? Acur.t - A.t0d : s_u[p & 1][v - 1][upos]);
-            cons_prev[v] = q - cd * inc;
+            // this step's FP32 increments summed in FP32, accumulated in FP64
+            float inc = Sp[v] + (Sxy[v] - mz.m[v] * A.ds[2]);
+            if (v >= 1 && v <= 3) inc -= 2.f * f[v - 1];
+            const double q = v == 0 ? Acur.ro : Acur.ro * (v == 4 ? Acur.t - A.t0d : Acur.u[v - 1]);
+            cons_prev[v] = q - cd * (double)inc;
         }
         zb = cur.zt;
         cur = nxt;
