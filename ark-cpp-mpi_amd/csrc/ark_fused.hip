@@ -12,11 +12,11 @@
 // need the half-level state of the neighbouring column, which the ring
 // threads recompute.  Per plane the block computes, in order:
 //   stress force f and the predictor (Phase1 + UseForces) for every thread's
-//   cell, the Phase2 extrapolations of every cell (forward and backward, per
-//   axis), the new x- and y-faces (LDS exchange of the neighbour's forward
-//   extrapolation), the new z-face (registers: the previous plane's forward
-//   extrapolation), and the corrector (Phase1 + UseForces) of the previous
-//   plane, whose top face has just been formed.
+//   cell, its Phase2 extrapolations along z, the new z-face below it (from
+//   the previous plane's forward extrapolation) and the corrector (Phase1 +
+//   UseForces) of the previous plane, whose top face has just been formed;
+//   then its Phase2 extrapolations along x and y and the new x- and y-faces
+//   (LDS exchange of the neighbour's forward extrapolation).
 // Flux terms such as rho*u1*u2 are computed once per face and used by both
 // cells that share it; the operands and their association are the
 // reference's, so FP64 results are bitwise equal to the phase-by-phase path.
@@ -34,7 +34,7 @@
 namespace arkd {
 
 // 32x13 owned columns plus the ring: 34x15 = 510 threads, eight wavefronts
-// (the last one 62 lanes), two per SIMD (256 VGPRs).  Eight wavefronts keep
+// (the last one 62 lanes), two per SIMD (at most 256 VGPRs).  Eight wavefronts keep
 // the four SIMDs of a CU equally loaded.  32x13 covers a 256^2 plane with 160
 // tiles, 1.25 threads per owned column (30x14: 171 tiles, 1.34; 62x6: 1.68),
 // and the step at 256^3 in 2.21 ms (30x14: 2.27), at 320^3 in 4.18 ms (4.28);
@@ -234,11 +234,19 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         sz_lo[1] = div_rn(VIS * (A_cur.u2 - A_below.u2), K.dx[2], K.rdx[2]);
         sz_lo[2] = div_rn(VIS * (A_cur.u3 - A_below.u3), K.dx[2], K.rdx[2]);
     }
-    Cell5 H_prev{};
-    Fwd fz_prev{};
-    double f_prev[3] = {0, 0, 0};
-    double S_prev[5] = {0, 0, 0, 0, 0};
-    Flux5 mz_prev{};
+    // What plane p-1 hands to plane p -- its half level H, forward z
+    // extrapolation, stress force, in-plane flux sums and z-face flux -- waits
+    // in the lane's own LDS column (22 doubles) rather than in registers: only
+    // the lane itself reads it, so no barrier guards it, and the 44 VGPRs it
+    // frees keep the plane's work within 256 without spills.
+    __shared__ double carry[22][NTHR];
+    double* const cy = &carry[0][tid];
+    auto cy_put = [&](int q, double v) { cy[q * NTHR] = v; };
+    auto cy_get = [&](int q) { return cy[q * NTHR]; };
+    // The top z-face of plane p-1 is the bottom z-face of plane p (the same
+    // stored element) except where z wraps.
+    Face5 zt_prev{};
+    int zt_prev_at = -1;    // stored plane of zt_prev
     unsigned long long dtbits = ~0ull;
     unsigned bad = 0;
 
@@ -270,15 +278,19 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         }
 
         // ---- predictor (Phase1 + UseForces, old faces) and the Phase2 -------
-        // ---- extrapolations of this cell along x, y, z -----------------------
+        // ---- extrapolations of this cell along z -----------------------------
         Cell5 H;
-        Fwd fx, fy, fz;
-        Bwd bx, by, bz;
-        double ox[3], oy[3], oz[3];   // old t, tangential a, tangential b of the low faces
+        Face5 Pxb, Pxf, Pyb, Pyf;     // the faces of the x and y extrapolations
+        Fwd fz;
+        Bwd bz;
+        double oz[3];                 // old t, tangential a, tangential b of the low z-face
         {
             const Face5 Xw = load_face(A, 0, c), Xe = load_face(A, 0, c + 1);
             const Face5 Ys = load_face(A, 1, c), Yn = load_face(A, 1, c + px);
-            const Face5 Zb = load_face(A, 2, c), Zt = load_face(A, 2, c + sxy);
+            const Face5 Zb = zt_prev_at == pw ? zt_prev : load_face(A, 2, c);
+            const Face5 Zt = load_face(A, 2, c + sxy);
+            zt_prev = Zt;
+            zt_prev_at = pw + 1;
             {
                 const Flux5 fxw = old_flux<0>(Xw, K), fxe = old_flux<0>(Xe, K);
                 const Flux5 fys = old_flux<1>(Ys, K), fyn = old_flux<1>(Yn, K);
@@ -293,7 +305,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
             // predictor's except for a ring cell just outside the box, whose
             // u/p ghost faces map into the box and whose t ghost faces are read
             // as stored
-            Face5 Pxb = Xw, Pxf = Xe, Pyb = Ys, Pyf = Yn, Pzb = Zb, Pzf = Zt;
+            Pxb = Xw; Pxf = Xe; Pyb = Ys; Pyf = Yn;
+            Face5 Pzb = Zb, Pzf = Zt;
             if (i != iw) {
                 const int cl = at(g, 0, jw, pw);
                 Pxb = load_face(A, 0, cl + mapf(i, g.N1));
@@ -316,22 +329,10 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
                 Pzf.t = gld(A.F[2], A.fb, 4, cl + (p + 1) * sxy);
             }
             {
-                const CellSide s = make_side<0>(A_cur, H, Pxb, Pxf);
-                fx = cell_forward<true>(s, dt, K.dx[0], K, rdt, K.rdx[0]);
-                bx = cell_backward<Quirk, true>(s, dt, K.dx[0], K, rdt, K.rdx[0]);
-            }
-            {
-                const CellSide s = make_side<1>(A_cur, H, Pyb, Pyf);
-                fy = cell_forward<true>(s, dt, K.dx[1], K, rdt, K.rdx[1]);
-                by = cell_backward<false, true>(s, dt, K.dx[1], K, rdt, K.rdx[1]);
-            }
-            {
                 const CellSide s = make_side<2>(A_cur, H, Pzb, Pzf);
                 fz = cell_forward<true>(s, dt, K.dx[2], K, rdt, K.rdx[2]);
                 bz = cell_backward<Quirk, true>(s, dt, K.dx[2], K, rdt, K.rdx[2]);
             }
-            ox[0] = Pxb.t; ox[1] = Pxb.u2; ox[2] = Pxb.u3;
-            oy[0] = Pyb.t; oy[1] = Pyb.u1; oy[2] = Pyb.u3;
             oz[0] = Pzb.t; oz[1] = Pzb.u1; oz[2] = Pzb.u2;
         }
         if (owned && p >= k0 && p < kend) {
@@ -348,14 +349,19 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         // ---- z-face p and the corrector of plane p-1 ------------------------
         Flux5 mz{};
         if (p >= k0) {
+            Cell5 H_prev;
+            H_prev.ro = cy_get(0); H_prev.u1 = cy_get(1); H_prev.u2 = cy_get(2);
+            H_prev.u3 = cy_get(3); H_prev.t = cy_get(4);
+            const Fwd fz_prev{cy_get(5), cy_get(6), cy_get(7), cy_get(8)};
             const FaceOut o = face_combine<true>(fz_prev, bz, H_prev.u3, H.u3, H_prev.t, H.t, H_prev.u1, H.u1,
                                                  H_prev.u2, H.u2, oz[0], oz[1], oz[2], K);
             mz = new_flux<2>(o);
             if (owned) {
                 if (p < kend || kend == g.N3 + 1) store_face<2>(A, at(g, i, j, p), o);
                 if (p > k0) {
-                    double S[5];
-                    for (int v = 0; v < 5; ++v) S[v] = S_prev[v] + (mz.m[v] - mz_prev.m[v]) * K.ds3;
+                    double S[5], f_prev[3];
+                    for (int v = 0; v < 5; ++v) S[v] = cy_get(12 + v) + (mz.m[v] - cy_get(17 + v)) * K.ds3;
+                    for (int m = 0; m < 3; ++m) f_prev[m] = cy_get(9 + m);
                     const Cell5 An = conservative_update(H_prev, S, f_prev, dt, K);
                     const int co = at(g, i, j, p - 1);
                     gst(A.Ao, A.fb, 0, co, An.ro);
@@ -366,6 +372,23 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
                 }
             }
         }
+
+        // ---- Phase2 extrapolations along x and y ----------------------------
+        Fwd fx, fy;
+        Bwd bx, by;
+        {
+            const CellSide s = make_side<0>(A_cur, H, Pxb, Pxf);
+            fx = cell_forward<true>(s, dt, K.dx[0], K, rdt, K.rdx[0]);
+            bx = cell_backward<Quirk, true>(s, dt, K.dx[0], K, rdt, K.rdx[0]);
+        }
+        {
+            const CellSide s = make_side<1>(A_cur, H, Pyb, Pyf);
+            fy = cell_forward<true>(s, dt, K.dx[1], K, rdt, K.rdx[1]);
+            by = cell_backward<false, true>(s, dt, K.dx[1], K, rdt, K.rdx[1]);
+        }
+        // old t, tangential a, tangential b of the low x- and y-faces
+        const double ox[3] = {Pxb.t, Pxb.u2, Pxb.u3};
+        const double oy[3] = {Pyb.t, Pyb.u1, Pyb.u3};
 
         // ---- x/y faces and the in-plane part of the corrector ---------------
         double S_xy[5] = {0, 0, 0, 0, 0};
@@ -404,11 +427,10 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
             __syncthreads();
         }
 
-        H_prev = H;
-        fz_prev = fz;
-        for (int m = 0; m < 3; ++m) f_prev[m] = f[m];
-        for (int v = 0; v < 5; ++v) S_prev[v] = S_xy[v];
-        mz_prev = mz;
+        cy_put(0, H.ro); cy_put(1, H.u1); cy_put(2, H.u2); cy_put(3, H.u3); cy_put(4, H.t);
+        cy_put(5, fz.r); cy_put(6, fz.t); cy_put(7, fz.a); cy_put(8, fz.b);
+        for (int m = 0; m < 3; ++m) cy_put(9 + m, f[m]);
+        for (int v = 0; v < 5; ++v) { cy_put(12 + v, S_xy[v]); cy_put(17 + v, mz.m[v]); }
         A_cur = A_next;
     }
     block_min_count(dtbits, bad, A.dt_next, A.nf, A.S->nstep);
