@@ -225,8 +225,9 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
     const double VIS = K.VIS;
     FusedArgs A = a;
 
-    // rolling state along z
+    // rolling state along z; plane p+1's cells are loaded during plane p-1
     Cell5 A_cur = load_cell(A, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
+    Cell5 A_nx = load_cell(A, at(g, iw, jw, wrapc(k0, g.N3)));
     double sz_lo[3];
     {
         const Cell5 A_below = load_cell(A, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
@@ -256,7 +257,8 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         asm volatile("" : "+s"(A.fb));
         const int pw = wrapc(p, g.N3);
         const int c = at(g, iw, jw, pw);
-        const Cell5 A_next = load_cell(A, at(g, iw, jw, wrapc(p + 1, g.N3)));
+        const Cell5 A_next = A_nx;
+        if (p < kend) A_nx = load_cell(A, at(g, iw, jw, wrapc(p + 2, g.N3)));
 
         // ---- stress force f (StressTensor, source/main.cpp:856-1023) -------
         double f[3];
