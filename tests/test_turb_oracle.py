@@ -100,3 +100,24 @@ def test_dissipation_closed_form():
          np.zeros((N, N, N))]
     want = nu * A * A * np.sin(k * dx) ** 2 / (2 * dx * dx)
     assert abs(T.dissipation(u, (dx, dx, dx), nu) - want) < 1e-14
+
+
+def test_coarse_gradient_closed_form():
+    """coarse_gradient's component 3i + j is the central difference of u_i
+    along axis j: one single-mode field per component, each varying along a
+    different axis, on a grid with a different spacing per axis, so that a
+    swapped component order, axis map or spacing fails."""
+    N3, N2, N1 = 8, 10, 12
+    h = (2 * np.pi / N1, 2 * np.pi / N2, 2 * np.pi / N3)
+    x = (np.arange(N1) + 0.5) * h[0]
+    y = (np.arange(N2) + 0.5) * h[1]
+    z = (np.arange(N3) + 0.5) * h[2]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    ub = np.array([np.sin(2 * Y), 3 * np.sin(Z), 0.5 * np.sin(3 * X)])
+    g = T.coarse_gradient(ub, h)
+    want = np.zeros((9,) + X.shape)
+    want[3 * 0 + 1] = np.cos(2 * Y) * np.sin(2 * h[1]) / h[1]          # d u_x / d y
+    want[3 * 1 + 2] = 3 * np.cos(Z) * np.sin(h[2]) / h[2]              # d u_y / d z
+    want[3 * 2 + 0] = 0.5 * np.cos(3 * X) * np.sin(3 * h[0]) / h[0]    # d u_z / d x
+    assert g.shape == want.shape
+    assert np.abs(g - want).max() < 1e-13
