@@ -18,6 +18,17 @@ struct ark_prof_event {
 // interior element i = 1 of every row starts a 128-byte line.
 constexpr int kOriginPad = 15;
 
+// The 16 forced wavevectors m (k = 2 pi m / L) of ark_set_forcing: every
+// 1 <= |m|^2 <= 4 with its first non-zero component positive (the conjugates
+// complete them), in the order the mode table holds them.  Component d of
+// mode n, as a compile-time constant wherever n and d are.
+__host__ __device__ constexpr int forced_mode(int n, int d) {
+    constexpr int m[16][3] = {{0, 0, 1},  {0, 0, 2},  {0, 1, -1}, {0, 1, 0},  {0, 1, 1},  {0, 2, 0},
+                              {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1}, {1, 0, 0},  {1, 0, 1},
+                              {1, 1, -1},  {1, 1, 0},  {1, 1, 1},  {2, 0, 0}};
+    return m[n][d];
+}
+
 struct ark_ctx {
     ark_params prm{};
     arkd::Geo geo{};

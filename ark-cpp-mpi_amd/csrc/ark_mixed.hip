@@ -81,6 +81,12 @@ struct Args {
     // field index map) or null; the cell force is rho dvc F
     const float* force;
     long long fstride;
+    // or, with force null, the forcing evaluated per cell: this step's
+    // 16-mode table (ark_turb: mode n, component d at 6n + 2d, re then im)
+    // and the per-axis {cos, sin} of theta and 2 theta at the cell centres
+    // (x entries, then y, then z); null when the box is not forced
+    const float* fmodes;
+    const float4* ftrig;
     float dvcf;
 };
 
@@ -348,6 +354,45 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     const bool put_o = in_region && has_o, put_ex = in_region && has_ex;
     Args A = a;
 
+    // Forcing evaluated in place.  F_d(x) = sum_n 2 Re(a_nd e^{i m_n.theta}),
+    // theta the per-axis phases of the cell centre.  The x and y factors are
+    // fixed along the lane's column, so the 16 modes collapse by their z
+    // component m_z in {-1, 0, 1, 2} into five coefficients per component:
+    //   F_d = R0 + R1 cos(tz) + R2 sin(tz) + R3 cos(2 tz) + R4 sin(2 tz).
+    float fr[3][5] = {};
+    if (A.fmodes) {
+        const float4 px4 = A.ftrig[iw - 1], py4 = A.ftrig[g.N1 + jw - 1];
+        auto phase = [](const float4& t, int m, float& r, float& i) {
+            r = m == 0 ? 1.f : m == 2 ? t.z : t.x;
+            i = m == 0 ? 0.f : m == 2 ? t.w : m == 1 ? t.y : -t.y;
+        };
+#pragma unroll
+        for (int n = 0; n < 16; ++n) {
+            float xr, xi, yr, yi;
+            phase(px4, forced_mode(n, 0), xr, xi);
+            phase(py4, forced_mode(n, 1), yr, yi);
+            const float er = xr * yr - xi * yi, ei = xr * yi + xi * yr;
+            const int mz = forced_mode(n, 2);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float tr = A.fmodes[6 * n + 2 * d], ti = A.fmodes[6 * n + 2 * d + 1];
+                const float qr = 2.f * (tr * er - ti * ei), qi = 2.f * (tr * ei + ti * er);
+                if (mz == 0) {
+                    fr[d][0] += qr;
+                } else if (mz == 1) {
+                    fr[d][1] += qr;
+                    fr[d][2] -= qi;
+                } else if (mz == -1) {
+                    fr[d][1] += qr;
+                    fr[d][2] += qi;
+                } else {
+                    fr[d][3] += qr;
+                    fr[d][4] -= qi;
+                }
+            }
+        }
+    }
+
     auto load_plane = [&](int p) {
         PlaneIn r;
         const int zo = wrapc(p, g.N3) * sxy;
@@ -360,8 +405,13 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         if (has_o)
             for (int m = 0; m < 3; ++m) r.ou[m] = ldd(A.A, A.fb, 1 + m, o_off + zo);
         r.fo[0] = r.fo[1] = r.fo[2] = 0.f;
-        if (A.force)
+        if (A.force) {
             for (int m = 0; m < 3; ++m) r.fo[m] = A.force[m * A.fstride + base + zo];
+        } else if (A.fmodes) {
+            const float4 tz = A.ftrig[g.N1 + g.N2 + wrapc(p, g.N3) - 1];
+            for (int m = 0; m < 3; ++m)
+                r.fo[m] = fr[m][0] + fr[m][1] * tz.x + fr[m][2] * tz.y + fr[m][3] * tz.z + fr[m][4] * tz.w;
+        }
         return r;
     };
     auto put_face = [&](float* f, int q, const FaceF& r) {
@@ -737,8 +787,11 @@ mixed::Args make_args(const ark_ctx* c, int LZ, unsigned long long* dt_next, lon
     a.dt_next = dt_next;
     a.nf = nf;
     a.LZ = LZ;
-    a.force = (c->force_eps > 0.0 || c->closure) ? c->force : nullptr;
+    // the force field when a closure adds to it, else the forcing in place
+    a.force = c->closure ? c->force : nullptr;
     a.fstride = c->force_stride;
+    a.fmodes = (!c->closure && c->force_eps > 0.0) ? c->force_table : nullptr;
+    a.ftrig = (const float4*)c->force_trig;
     a.dvcf = (float)c->K.dvc;
     return a;
 }
@@ -1050,7 +1103,7 @@ struct ark_batch {
                                                   // three per box
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
-    ark_force_batch* forcing = nullptr;           // the forced boxes' force kernels, two launches per step
+    ark_force_batch* forcing = nullptr;           // the forced boxes' mode tables, one launch per step
     bool built = false;
     std::vector<int> gpar_at_build;               // box buffer parities the tables were built for
     int parity = 0;                               // which table the next step uses

@@ -268,14 +268,10 @@ __global__ void k_force_modes(const Scalars* S, unsigned long long seed, double 
 
 // Forcing of many boxes at once (batched steps): one record per forced box.
 struct ForceBox {
-    Geo g;
     const Scalars* S;
     unsigned long long seed;
     double eps, tau, L0, L1, L2;
     float* table;
-    const float4* trig;
-    float* force;
-    long long stride;
 };
 
 // Block b: the 16 modes of box b.
@@ -300,18 +296,14 @@ __device__ __forceinline__ void force_cell(const Geo& g, const float* tab, const
         cr[d][2] = t[d].z;
         ci[d][2] = t[d].w;
     }
-    // the 16 forced wavevectors in the order of c_modes (ark_set_forcing),
-    // as compile-time constants so every index below is one
-    constexpr int kM[16][3] = {{0, 0, 1},  {0, 0, 2},  {0, 1, -1},  {0, 1, 0},  {0, 1, 1},  {0, 2, 0},
-                               {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1}, {1, 0, 0},  {1, 0, 1},
-                               {1, 1, -1},  {1, 1, 0},  {1, 1, 1},  {2, 0, 0}};
+    // the 16 forced wavevectors (forced_mode), compile-time constants once unrolled
     float F[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
         float er = 1.f, ei = 0.f;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const int m = kM[n][d];
+            const int m = forced_mode(n, d);
             const float r = cr[d][m < 0 ? -m : m], s = m < 0 ? -ci[d][-m] : ci[d][m];
             const float nr = er * r - ei * s, ni = er * s + ei * r;
             er = nr;
@@ -332,26 +324,6 @@ __global__ void k_force_field(Geo g, const float* table, const float4* trig, flo
     __syncthreads();
     if (i > g.N1) return;
     force_cell(g, tab, trig, force, stride, i, j, k);
-}
-
-// All forced boxes' fields in one launch: box b owns the workgroups
-// start[b] .. start[b+1]-1, 256 interior cells each (x fastest).
-__global__ void k_force_field_batch(const ForceBox* __restrict__ fb, const int* __restrict__ start, int nbox) {
-    int lo = 0, hi = nbox - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= (int)blockIdx.x) lo = mid;
-        else hi = mid - 1;
-    }
-    const ForceBox& f = fb[lo];
-    __shared__ float tab[96];
-    for (int q = threadIdx.x; q < 96; q += blockDim.x) tab[q] = f.table[q];
-    __syncthreads();
-    const long long cell = (long long)(blockIdx.x - start[lo]) * blockDim.x + threadIdx.x;
-    const long long n1 = f.g.N1, n12 = n1 * f.g.N2;
-    if (cell >= n12 * f.g.N3) return;
-    force_cell(f.g, tab, f.trig, f.force, f.stride, (int)(cell % n1) + 1, (int)((cell / n1) % f.g.N2) + 1,
-               (int)(cell / n12) + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -630,31 +602,34 @@ ark_status install_velocity(ark_ctx* c, Fft& f, const hipfftDoubleComplex* U, hi
 
 }  // namespace
 
-// Mixed precision: the forcing field of the current step (null when off).
+// Mixed precision: this step's forcing (nothing when off).  The 16-mode
+// table always; the acceleration field only when a learned closure will add
+// its own acceleration to it (ark_closure_step).  Without a closure the fused
+// step evaluates the forcing per cell from the table itself (mixed::Args::
+// fmodes), which keeps the field's 12 bytes per cell out of HBM.
 ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st) {
     if (!c->force || c->force_eps <= 0.0) return ARK_OK;
     const double L0 = c->prm.dx1 * c->geo.N1, L1 = c->prm.dx2 * c->geo.N2, L2 = c->prm.dx3 * c->geo.N3;
     hipLaunchKernelGGL(k_force_modes, dim3(1), dim3(64), 0, st, c->S, (unsigned long long)c->force_seed,
                        c->force_eps, c->force_tau, L0, L1, L2, c->force_table);
     ark_status s = c->hip(hipGetLastError(), "force modes");
-    if (s != ARK_OK) return s;
+    if (s != ARK_OK || !c->closure) return s;
     hipLaunchKernelGGL(k_force_field, interior_grid(c->geo, 0), dim3(256), 0, st, c->geo, c->force_table,
                        (const float4*)c->force_trig, c->force, (long long)c->force_stride);
     return c->hip(hipGetLastError(), "force field");
 }
 
-// Forcing of a batch: one record per forced box and the first workgroup of
-// each box's field, in device memory, built once per set of boxes.
+// Forcing of a batch: one record per forced box, in device memory, built
+// once per set of boxes.  Batched boxes carry no closure, so the batch kernel
+// evaluates each box's forcing from its mode table.
 struct ark_force_batch {
     ForceBox* fb = nullptr;
-    int* start = nullptr;
-    int n = 0, blocks = 0;
+    int n = 0;
 };
 
 void ark_force_batch_free(ark_force_batch* f) {
     if (!f) return;
     if (f->fb) (void)hipFree(f->fb);
-    if (f->start) (void)hipFree(f->start);
     delete f;
 }
 
@@ -662,12 +637,10 @@ ark_status ark_force_batch_build(ark_ctx* const* boxes, int nbox, ark_force_batc
     ark_force_batch_free(*out);
     *out = nullptr;
     std::vector<ForceBox> h;
-    std::vector<int> start(1, 0);
     for (int b = 0; b < nbox; ++b) {
         const ark_ctx* c = boxes[b];
         if (!c->force || c->force_eps <= 0.0) continue;
         ForceBox f;
-        f.g = c->geo;
         f.S = c->S;
         f.seed = c->force_seed;
         f.eps = c->force_eps;
@@ -676,21 +649,13 @@ ark_status ark_force_batch_build(ark_ctx* const* boxes, int nbox, ark_force_batc
         f.L1 = c->prm.dx2 * c->geo.N2;
         f.L2 = c->prm.dx3 * c->geo.N3;
         f.table = c->force_table;
-        f.trig = (const float4*)c->force_trig;
-        f.force = c->force;
-        f.stride = c->force_stride;
         h.push_back(f);
-        const long long cells = (long long)c->geo.N1 * c->geo.N2 * c->geo.N3;
-        start.push_back(start.back() + (int)((cells + 255) / 256));
     }
     ark_force_batch* f = new ark_force_batch();
     f->n = (int)h.size();
-    f->blocks = start.back();
     if (f->n > 0) {
         if (hipMalloc(&f->fb, h.size() * sizeof(ForceBox)) != hipSuccess ||
-            hipMalloc(&f->start, start.size() * sizeof(int)) != hipSuccess ||
-            hipMemcpy(f->fb, h.data(), h.size() * sizeof(ForceBox), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(f->start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            hipMemcpy(f->fb, h.data(), h.size() * sizeof(ForceBox), hipMemcpyHostToDevice) != hipSuccess) {
             ark_force_batch_free(f);
             return ARK_ERR_HIP;
         }
@@ -699,11 +664,10 @@ ark_status ark_force_batch_build(ark_ctx* const* boxes, int nbox, ark_force_batc
     return ARK_OK;
 }
 
-// This step's forcing of every forced box: two launches, whatever the count.
+// This step's mode tables of every forced box: one launch, whatever the count.
 ark_status ark_force_batch_enqueue(const ark_force_batch* f, hipStream_t st) {
     if (!f || f->n == 0) return ARK_OK;
     hipLaunchKernelGGL(k_force_modes_batch, dim3(f->n), dim3(64), 0, st, f->fb);
-    hipLaunchKernelGGL(k_force_field_batch, dim3(f->blocks), dim3(256), 0, st, f->fb, f->start, f->n);
     return hipGetLastError() == hipSuccess ? ARK_OK : ARK_ERR_HIP;
 }
 
@@ -1012,13 +976,11 @@ ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps, doub
                     const int first = a != 0 ? a : b != 0 ? b : d;
                     if (k2 >= 1 && k2 <= 4 && first > 0) h[q++] = ForceMode{{a, b, d}};
                 }
-        // the force-field kernel holds the same list as compile-time constants
-        static const int kHostM[16][3] = {{0, 0, 1},  {0, 0, 2},  {0, 1, -1},  {0, 1, 0},  {0, 1, 1},  {0, 2, 0},
-                                          {1, -1, -1}, {1, -1, 0}, {1, -1, 1}, {1, 0, -1}, {1, 0, 0},  {1, 0, 1},
-                                          {1, 1, -1},  {1, 1, 0},  {1, 1, 1},  {2, 0, 0}};
+        // the force kernels hold the same list as compile-time constants
         for (int n = 0; n < 16; ++n)
             for (int d = 0; d < 3; ++d)
-                if (q != 16 || h[n].m[d] != kHostM[n][d]) return c->fail(ARK_ERR_STATE, "forcing: mode table mismatch");
+                if (q != 16 || h[n].m[d] != forced_mode(n, d))
+                    return c->fail(ARK_ERR_STATE, "forcing: mode table mismatch");
         s = c->hip(hipMemcpyToSymbol(HIP_SYMBOL(c_modes), h, sizeof h), "force modes");
     }
     if (s == ARK_OK && eps > 0.0) s = ensure_force(c);
