@@ -1,6 +1,6 @@
 """Test infrastructure only: numpy restatements of the homogeneous-turbulence
-functions of the library (seeded initial field, forcing modes, spectrum, box
-filter, filtered gradients, dissipation).  They have no counterpart in the
+functions of the library (seeded initial field, forcing modes and field,
+spectrum, box filter, filtered gradients, dissipation).  They have no counterpart in the
 reference (a channel solver); each is pinned by the closed forms and
 published known-answer vectors its tests check, and serves as the checker
 of the HIP kernels.  Nothing in the product imports this module.
@@ -119,6 +119,60 @@ def coarse_gradient(ub, h):
         for j, ax in enumerate((2, 1, 0)):          # x, y, z are numpy axes 2, 1, 0
             out.append((np.roll(ub[i], -1, axis=ax) - np.roll(ub[i], 1, axis=ax)) / (2.0 * h[j]))
     return np.array(out)
+
+
+# ---- forcing (ark_set_forcing) ---------------------------------------------------
+def forced_modes():
+    """The forced integer wavevectors m: every 1 <= |m|^2 <= 4 whose first
+    non-zero component is positive (the conjugates complete the real field),
+    in lexicographic order of (m_x, m_y, m_z)."""
+    out = []
+    for a in range(-2, 3):
+        for b in range(-2, 3):
+            for d in range(-2, 3):
+                first = a if a != 0 else (b if b != 0 else d)
+                if 1 <= a * a + b * b + d * d <= 4 and first > 0:
+                    out.append((a, b, d))
+    return out
+
+
+def forcing_table(seed: int, counter: int, eps: float, delta: float, L):
+    """The complex amplitudes a_n (16 x 3) of one forcing interval: per mode
+    n, eight normals from Philox counters (n, counter, 0xf0c3, 0 / 1); the
+    first three are the real part, the next three the imaginary part; each is
+    projected normal to k_n, and the pair scaled together to the norm
+    A = sqrt(eps / (16 delta)), so that the field injects eps on average over
+    an interval delta.  `counter` is the step number (white forcing, delta =
+    dt) or the index of the correlation interval."""
+    modes = forced_modes()
+    table = np.zeros((len(modes), 3), dtype=np.complex128)
+    for n, m in enumerate(modes):
+        g = gauss4(seed, n, counter, 0xF0C3, 0) + gauss4(seed, n, counter, 0xF0C3, 1)
+        g = [float(x) for x in g]
+        k = np.array([2 * np.pi / L[d] * m[d] for d in range(3)])
+        re, im = np.array(g[0:3]), np.array(g[3:6])
+        re = re - (k @ re) / (k @ k) * k
+        im = im - (k @ im) / (k @ k) * k
+        nrm = float(re @ re + im @ im)
+        A = np.sqrt(eps / (delta * 16.0)) / np.sqrt(nrm)
+        table[n] = A * re + 1j * A * im
+    return table
+
+
+def force_field(table, N, L):
+    """F_d(x) = sum_n 2 Re(a_nd e^{i k_n . x}) at the cell centres
+    x_d = (i - 1/2) L_d / N_d, i = 1..N_d; arrays (3, N3, N2, N1)."""
+    N1, N2, N3 = N
+    x = (np.arange(N1) + 0.5) * L[0] / N1
+    y = (np.arange(N2) + 0.5) * L[1] / N2
+    z = (np.arange(N3) + 0.5) * L[2] / N3
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    F = np.zeros((3,) + X.shape)
+    for n, m in enumerate(forced_modes()):
+        ph = np.exp(1j * (2 * np.pi * m[0] / L[0] * X + 2 * np.pi * m[1] / L[1] * Y + 2 * np.pi * m[2] / L[2] * Z))
+        for d in range(3):
+            F[d] += 2.0 * np.real(table[n, d] * ph)
+    return F
 
 
 def dissipation(u, dx, nu):

@@ -103,6 +103,48 @@ def test_forcing_injects_eps_dt_in_one_step():
         assert abs(ke / (eps * d["dt"]) - 1.0) < 1e-2, ke / (eps * d["dt"])
 
 
+def _zero_network():
+    """A closure network whose stresses are exactly zero (all parameters and
+    the output offsets zero): attaching it switches a forced context to the
+    force-field path without changing the force."""
+    m = arkamd.MLP(seed=0)
+    m.set_params(np.zeros(m.nparam, dtype=np.float32))
+    m.set_normalization(np.zeros(10), np.ones(10), np.zeros(6), np.ones(6))
+    return m
+
+
+def test_forcing_field_matches_the_oracle():
+    """From rest, one step of white forcing leaves u = F dt in every cell,
+    with F the oracle's field of the step's mode table (counter: step 1, delta
+    = dt): this pins the mode list, amplitudes, phases and signs of the
+    forcing the fused step evaluates in place.  The force-field path (taken
+    when a closure adds to the field; here one that adds zero) gives the same
+    velocities to FP32 rounding.  Tolerances: 1e-2 against the oracle (within
+    the step the scheme's pressure coupling moves u by a few tenths of a
+    percent), 1e-5 between the two paths (FP32 rounding of F in two orders)."""
+    N, eps = (32, 24, 16), 0.1
+    L = (2 * np.pi,) * 3
+    out = {}
+    net = _zero_network()
+    for path in ("in place", "field"):
+        with rest(N) as s:
+            s.set_forcing(seed=3, eps=eps)
+            if path == "field":
+                s.set_closure(net, 1.0)
+            s.step(1)
+            out[path] = (np.stack(cells_of(s)), s.diagnostics()["dt"])
+    net.close()
+    u, dt = out["in place"]
+    F = T.force_field(T.forcing_table(3, 1, eps, dt, L), N, L)
+    err = np.sqrt(np.sum((u - F * dt) ** 2) / np.sum((F * dt) ** 2))
+    uf, dtf = out["field"]
+    assert dtf == dt
+    between = np.sqrt(np.sum((u - uf) ** 2) / np.sum(u ** 2))
+    print(f"oracle {err:.2e}, paths {between:.2e}")
+    assert err < 1e-2, err
+    assert between < 1e-5, between
+
+
 def test_forcing_with_correlation_time():
     """Amplitudes held over intervals of tau: from rest one step gives
     u = F dt with <|F|^2> = 2 eps / tau, so KE = eps dt^2 / tau; and the force

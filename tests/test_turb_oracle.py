@@ -121,3 +121,52 @@ def test_coarse_gradient_closed_form():
     want[3 * 2 + 0] = 0.5 * np.cos(3 * X) * np.sin(3 * h[0]) / h[0]    # d u_z / d x
     assert g.shape == want.shape
     assert np.abs(g - want).max() < 1e-13
+
+
+def test_forced_modes_closed_form():
+    """32 integer vectors have 1 <= |m|^2 <= 4 (6 + 12 + 8 + 6 for |m|^2 =
+    1, 2, 3, 4); one of each +-pair is forced."""
+    m = T.forced_modes()
+    assert len(m) == 16 and len(set(m)) == 16
+    for v in m:
+        assert 1 <= sum(c * c for c in v) <= 4
+        assert tuple(-c for c in v) not in m
+        assert next(c for c in v if c != 0) > 0
+    assert sorted(m) == m
+
+
+def test_forcing_table_and_field_closed_forms():
+    """Each amplitude is normal to its wavevector and of norm
+    sqrt(eps / (16 delta)); the field's grid mean square is then exactly
+    sum_n 2 |a_n|^2 = 2 eps / delta (distinct wavevectors are orthogonal on
+    the grid), its divergence is zero, and the table depends on seed and
+    counter only."""
+    L = (2 * np.pi, 2 * np.pi, 2 * np.pi)
+    eps, delta = 0.1, 3e-3
+    a = T.forcing_table(3, 1, eps, delta, L)
+    for n, m in enumerate(T.forced_modes()):
+        k = np.array(m, dtype=float)
+        assert abs(k @ a[n]) < 1e-12 * np.linalg.norm(k)
+        assert np.sqrt(np.sum(np.abs(a[n]) ** 2)) == pytest.approx(np.sqrt(eps / (16 * delta)), rel=1e-12)
+    F = T.force_field(a, (12, 10, 8), L)
+    assert np.sum(F ** 2, axis=0).mean() == pytest.approx(2 * eps / delta, rel=1e-12)
+    assert np.array_equal(T.forcing_table(3, 1, eps, delta, L), a)
+    assert not np.array_equal(T.forcing_table(3, 2, eps, delta, L), a)
+    assert not np.array_equal(T.forcing_table(4, 1, eps, delta, L), a)
+
+
+def test_force_field_single_mode():
+    """One amplitude (1 + 2i) on mode n, component d: F_d = 2 (cos(k.x) - 2 sin(k.x))."""
+    L = (2 * np.pi, 4 * np.pi, 2 * np.pi)
+    N = (8, 6, 10)
+    n = T.forced_modes().index((1, -1, 1))
+    a = np.zeros((16, 3), dtype=complex)
+    a[n, 1] = 1 + 2j
+    F = T.force_field(a, N, L)
+    x = (np.arange(N[0]) + 0.5) * L[0] / N[0]
+    y = (np.arange(N[1]) + 0.5) * L[1] / N[1]
+    z = (np.arange(N[2]) + 0.5) * L[2] / N[2]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    kx = 2 * np.pi / L[0] * X - 2 * np.pi / L[1] * Y + 2 * np.pi / L[2] * Z
+    assert np.abs(F[1] - 2 * (np.cos(kx) - 2 * np.sin(kx))).max() < 1e-12
+    assert np.abs(F[0]).max() == 0 and np.abs(F[2]).max() == 0
