@@ -36,9 +36,10 @@ namespace arkd {
 // 32x13 owned columns plus the ring: 34x15 = 510 threads, eight wavefronts
 // (the last one 62 lanes), two per SIMD (at most 256 VGPRs).  Eight wavefronts keep
 // the four SIMDs of a CU equally loaded.  32x13 covers a 256^2 plane with 160
-// tiles, 1.25 threads per owned column (30x14: 171 tiles, 1.34; 62x6: 1.68),
-// and the step at 256^3 in 2.21 ms (30x14: 2.27), at 320^3 in 4.18 ms (4.28);
-// at 384^3 the two are within 0.5%.
+// tiles, 1.25 threads per owned column (30x14: 171 tiles, 1.34; 62x6: 1.68).
+// Side by side on an earlier form of this kernel, the step at 256^3 took
+// 2.21 ms (30x14: 2.27), at 320^3 4.18 ms (4.28); at 384^3 the two were
+// within 0.5%.
 constexpr int TX = 32, TY = 13;
 constexpr int RX = TX + 2, RY = TY + 2;
 constexpr int NTHR = RX * RY;
