@@ -49,8 +49,9 @@ __device__ __forceinline__ double rcpd(double x) {
 
 // Tile shapes: TX x TY owned columns plus a one-cell ring, RX x RY = (TX+2)
 // x (TY+2) threads, a multiple of four wavefronts so the four SIMDs of a CU
-// are equally loaded.  The single-box step uses 30x14 (512 threads, two
-// waves per SIMD); the batched step lets its autotuner choose.
+// are equally loaded.  The single-box step times 30x14 (512 threads) and
+// 32x13 (510 threads), two waves per SIMD each, on its first step and keeps
+// the faster; the batched step lets its autotuner choose among all four.
 template <int TX_, int TY_>
 struct Shape {
     static constexpr int TX = TX_, TY = TY_, RX = TX_ + 2, RY = TY_ + 2, NTHR = (TX_ + 2) * (TY_ + 2);
