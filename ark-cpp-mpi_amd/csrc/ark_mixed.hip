@@ -1,8 +1,9 @@
 // Mixed-precision fused CABARET/ARK step.
 //
 // Storage: the level-n cell fields stay FP64 (rho, u1, u2, u3, t); the face
-// fields are FP32: u1, u2, u3, p and t' = t - t0, as one record per face:
-// (u1, u2, u3, p) a float4, t' a float of its own (faces_of below).  The face density is
+// fields are FP32: u1, u2, u3, p and t' = t - t0.  Each face has (u1, u2, u3,
+// p) as one float4 record; the t' of the x-, y- and z-faces at one index share
+// a three-float record (rec_of, trec_of below).  The face density is
 // rho0 + p/c^2 and is never stored; p itself is the density perturbation
 // c^2 (rho - rho0), so the weakly compressible equation of state
 // p = c^2 (rho - rho0) (source/main.cpp:1774-1775) never subtracts two
@@ -100,13 +101,21 @@ __device__ __forceinline__ void std_(char* grp, unsigned long long fb, int v, in
 // Face records of one orientation, in the allocation of its five FP64 fields
 // (group pointer grp = FP64 element (0,0,0), 120 bytes into the allocation):
 // face idx has (u1, u2, u3, p) as the float4 at grp + 8 + 16 idx (16-byte
-// aligned) and t' as the float at grp + 2 fb + 4 idx.  One record is two
-// memory instructions instead of five.
+// aligned).  The t' of the three faces at idx (x, y, z) are one T3 record at
+// xg + 2 fb + 12 idx, xg the x-faces' group pointer of the same buffer: that
+// area lies past the float4 records, inside the group's 5 fb bytes.  A lane
+// reads its x-, y- and z-faces' t' with one load and writes them with one
+// store, where one float per face took three of each.
+struct T3 { float x, y, z; };
 __device__ __forceinline__ const float4* rec_of(const char* grp, unsigned idx) {
     return (const float4*)(grp + 8 + idx * 16u);
 }
-__device__ __forceinline__ const float* tp_of(const char* grp, unsigned long long fb, unsigned idx) {
-    return (const float*)(grp + 2 * fb + idx * 4u);
+__device__ __forceinline__ const T3* trec_of(const char* xg, unsigned long long fb, unsigned idx) {
+    return (const T3*)(xg + 2 * fb + idx * 12u);
+}
+// t' of the orientation-d face at idx (a component of its T3 record)
+__device__ __forceinline__ const float* tp_of(const char* xg, unsigned long long fb, int d, unsigned idx) {
+    return (const float*)(xg + 2 * fb + idx * 12u + 4u * (unsigned)d);
 }
 
 __device__ __forceinline__ int wrapc(int q, int N) { return q < 1 ? q + N : (q > N ? q - N : q); }
@@ -132,17 +141,23 @@ __device__ __forceinline__ CellF to_f(const CellD& c, const Args& a) {
     f.tp = (float)(c.t - a.t0d);
     return f;
 }
-__device__ __forceinline__ FaceF load_face_g(const char* grp, unsigned long long fb, int idx) {
+// The float4 record of an orientation-d face; t' is set by the caller.
+__device__ __forceinline__ FaceF load_rec(const Args& a, int d, int idx) {
     FaceF f;
-    const float4 q = *rec_of(grp, (unsigned)idx);
+    const float4 q = *rec_of(a.F[d], (unsigned)idx);
     f.u[0] = q.x;
     f.u[1] = q.y;
     f.u[2] = q.z;
     f.p = q.w;
-    f.tp = *tp_of(grp, fb, (unsigned)idx);
+    f.tp = 0.f;
     return f;
 }
-__device__ __forceinline__ FaceF load_face(const Args& a, int d, int idx) { return load_face_g(a.F[d], a.fb, idx); }
+// One orientation-d face with its t'.
+__device__ __forceinline__ FaceF load_face(const Args& a, int d, int idx) {
+    FaceF f = load_rec(a, d, idx);
+    f.tp = *tp_of(a.F[0], a.fb, d, (unsigned)idx);
+    return f;
+}
 
 struct Flux { float m[5]; };
 
@@ -272,7 +287,11 @@ __device__ __forceinline__ void store_face(const Args& a, int idx, const FaceOut
     u[TA] = o.a;
     u[TB] = o.b;
     *(float4*)rec_of(a.Fo[D], (unsigned)idx) = make_float4(u[0], u[1], u[2], o.p);
-    *(float*)tp_of(a.Fo[D], a.fb, (unsigned)idx) = o.t;
+}
+// The t' of one orientation-d face alone (the periodic copies of face 1).
+template <int D>
+__device__ __forceinline__ void store_tp(const Args& a, int idx, float t) {
+    *(float*)tp_of(a.Fo[0], a.fb, D, (unsigned)idx) = t;
 }
 
 template <int D>
@@ -296,6 +315,7 @@ struct PlaneIn {
     FaceF xw, ys, zt, ex;
     double ou[3];
     float fo[3];    // body acceleration at the lane's cell (forcing)
+    float tn[2];    // t' of the x- and y-face one plane up (from zt's T3 record)
 };
 
 // One tile of one box: tile column (tbx, tby), owned planes k0 .. kend-1.
@@ -341,18 +361,44 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     const int tW = xfirst ? tid : tid - 1, tE = xlast ? tid : tid + 1;
     const int tS = yfirst ? tid : tid - RX, tN = ylast ? tid : tid + RX;
     const int upos = (ty + 1) * UX + tx + 1, fpos = ty * FX + tx;
-    const int opos = xfirst ? upos - 1 : xlast ? upos + 1 : yfirst ? upos - UX : upos + UX;
-    // cell and face offsets of this lane within a plane
+    // cell offset of this lane within a plane
     const int base = iw + jw * px;
-    const int ex_off = xlast ? wrapc(i + 1, g.N1) + jw * px : iw + wrapc(j + 1, g.N2) * px;
-    const bool has_ex = xlast || ylast;
-    int o_off = base;
-    if (xfirst) o_off = wrapc(i - 1, g.N1) + jw * px;
-    else if (xlast) o_off = wrapc(i + 1, g.N1) + jw * px;
-    else if (yfirst) o_off = iw + wrapc(j - 1, g.N2) * px;
-    else if (ylast) o_off = iw + wrapc(j + 1, g.N2) * px;
-    const bool has_o = xfirst || xlast || yfirst || ylast;
-    const bool put_o = in_region && has_o, put_ex = in_region && has_ex;
+    // What lies beyond the region, gathered onto dense lanes so that few
+    // wavefronts issue its loads.  The region is wx x wy lanes (wx, wy >= 3).
+    // (1) The velocity of each edge cell's outward stress neighbour, nh of
+    // them: lane h < nh takes the west column (LDS u-grid x = -1, y = h), the
+    // east column (x = wx), then the south (y = -1) and north (y = wy) rows
+    // without their corners (x = 1 .. wx-2).
+    // (2) The x-face beyond the last column (wy of them) and the y-face beyond
+    // the last row (wx-1 of them; the corner lane's is its x-face): lane
+    // EX0 + e, the last wavefront(s) of the workgroup.
+    const int wx = iend - i0 + 2, wy = jend - j0 + 2;
+    const int nh = 2 * wy + 2 * (wx - 2);
+    const bool do_o = tid < nh;
+    int o_off = 0, opos = 0;
+    if (do_o) {
+        int hx, hy;
+        if (tid < wy) { hx = -1; hy = tid; }
+        else if (tid < 2 * wy) { hx = wx; hy = tid - wy; }
+        else if (tid < 2 * wy + wx - 2) { hx = 1 + tid - 2 * wy; hy = -1; }
+        else { hx = 1 + tid - 2 * wy - (wx - 2); hy = wy; }
+        o_off = wrapc(i0 - 1 + hx, g.N1) + wrapc(j0 - 1 + hy, g.N2) * px;
+        opos = (hy + 1) * UX + hx + 1;
+    }
+    constexpr int EX0 = NTHR - (RX + RY - 1);
+    const int e = tid - EX0;
+    const bool do_ex = e >= 0 && e < wx + wy - 1;
+    const int ex_d = e < wy ? 0 : 1;       // orientation of the face beyond
+    int ex_off = 0, expos = 0;
+    if (do_ex) {
+        if (ex_d == 0) {
+            ex_off = wrapc(iend + 1, g.N1) + wrapc(j0 - 1 + e, g.N2) * px;
+            expos = e * FX + wx;
+        } else {
+            ex_off = wrapc(i0 - 1 + e - wy, g.N1) + wrapc(jend + 1, g.N2) * px;
+            expos = wy * FX + e - wy;
+        }
+    }
     Args A = a;
 
     // Forcing evaluated in place.  F_d(x) = sum_n 2 Re(a_nd e^{i m_n.theta}),
@@ -394,16 +440,25 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         }
     }
 
-    auto load_plane = [&](int p) {
+    // Plane p's inputs.  The x- and y-faces' t' come from the T3 record that
+    // plane p-1 read with its top z-face (prev.tn); this plane's top record
+    // gives zt's t' and hands the next plane its x and y t'.
+    auto load_plane = [&](int p, const PlaneIn& prev) {
         PlaneIn r;
-        const int zo = wrapc(p, g.N3) * sxy;
-        r.xw = load_face(A, 0, base + zo);
-        r.ys = load_face(A, 1, base + zo);
-        r.zt = load_face(A, 2, base + wrapc(p + 1, g.N3) * sxy);
+        const int zo = wrapc(p, g.N3) * sxy, zt_idx = base + wrapc(p + 1, g.N3) * sxy;
+        r.xw = load_rec(A, 0, base + zo);
+        r.ys = load_rec(A, 1, base + zo);
+        r.zt = load_rec(A, 2, zt_idx);
+        const T3 t3 = *trec_of(A.F[0], A.fb, (unsigned)zt_idx);
+        r.xw.tp = prev.tn[0];
+        r.ys.tp = prev.tn[1];
+        r.zt.tp = t3.z;
+        r.tn[0] = t3.x;
+        r.tn[1] = t3.y;
         r.ex = FaceF{{0.f, 0.f, 0.f}, 0.f, 0.f};
-        if (has_ex) r.ex = load_face_g(xlast ? A.F[0] : A.F[1], A.fb, ex_off + zo);
+        if (do_ex) r.ex = load_face(A, ex_d, ex_off + zo);
         r.ou[0] = r.ou[1] = r.ou[2] = 0.0;
-        if (has_o)
+        if (do_o)
             for (int m = 0; m < 3; ++m) r.ou[m] = ldd(A.A, A.fb, 1 + m, o_off + zo);
         r.fo[0] = r.fo[1] = r.fo[2] = 0.f;
         if (A.force) {
@@ -419,18 +474,16 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         f[0 * FN + q] = r.u[0]; f[1 * FN + q] = r.u[1]; f[2 * FN + q] = r.u[2];
         f[3 * FN + q] = r.p; f[4 * FN + q] = r.tp;
     };
-    // Plane p's faces and outward velocity into LDS (the lane's own velocity
-    // is written separately, from registers that are already there).
+    // Plane p's faces and outward velocities into LDS (the lane's own
+    // velocity is written separately, from registers that are already there).
     auto stage = [&](int p, const PlaneIn& r) {
-        if (!in_region) return;
         float* f = &s_f[0][0];
-        put_face(f, fpos, r.xw);
-        put_face(f + 5 * FN, fpos, r.ys);
-        if (put_ex) {
-            if (xlast) put_face(f, fpos + 1, r.ex);
-            else put_face(f + 5 * FN, fpos + FX, r.ex);
+        if (in_region) {
+            put_face(f, fpos, r.xw);
+            put_face(f + 5 * FN, fpos, r.ys);
         }
-        if (put_o)
+        if (do_ex) put_face(f + ex_d * 5 * FN, expos, r.ex);
+        if (do_o)
             for (int m = 0; m < 3; ++m) s_u[p & 1][m][opos] = (float)r.ou[m];
     };
     auto put_u = [&](int p, const CellD& c) {
@@ -453,8 +506,18 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         const CellD Ab = load_cell(A, base + wrapc(k0 - 2, g.N3) * sxy);
         for (int m = 0; m < 3; ++m) sz_lo[m] = A.VIS * ((float)Acur.u[m] - (float)Ab.u[m]) * A.inv_dx[2];
     }
-    PlaneIn cur = load_plane(k0 - 1);
-    FaceF zb = load_face(A, 2, base + wrapc(k0 - 1, g.N3) * sxy);
+    PlaneIn cur;
+    FaceF zb;
+    {
+        const int idx = base + wrapc(k0 - 1, g.N3) * sxy;
+        const T3 t3 = *trec_of(A.F[0], A.fb, (unsigned)idx);
+        PlaneIn below;
+        below.tn[0] = t3.x;
+        below.tn[1] = t3.y;
+        cur = load_plane(k0 - 1, below);
+        zb = load_rec(A, 2, idx);
+        zb.tp = t3.z;
+    }
     put_u(k0 - 1, Acur);
     stage(k0 - 1, cur);
     // Retire every prologue load here, so that the loop's waits only ever
@@ -479,7 +542,7 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         PlaneIn nxt = cur;
         CellD Anext2 = Anext;
         if (p < kend) {
-            nxt = load_plane(p + 1);
+            nxt = load_plane(p + 1, cur);
             Anext2 = load_cell(A, base + wrapc(p + 2, g.N3) * sxy);
         }
         const CellF Af = to_f(Acur, A);
@@ -559,16 +622,23 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
             bad += !isfinite(H.dr + H.u[0] + H.u[1] + H.u[2]);
         }
 
-        // z-face p, then the FP64 closing update of plane p-1
+        // z-face p, then the FP64 closing update of plane p-1.  The new t' of
+        // the lane's z-, x- and y-face (tz_s, tx_s, ty_s) go out together in
+        // one T3 record once the plane's x- and y-faces are done.
         Flux mz{};
+        float tx_s = 0.f, ty_s = 0.f, tz_s = 0.f;
         if (p >= k0) {
             const FaceOutF o = combine(fz_prev, bz, Hprev.u[2], H.u[2], Hprev.tp, H.tp, Hprev.u[0], H.u[0],
                                        Hprev.u[1], H.u[1], oz[0], oz[1], oz[2], A);
             mz = out_flux<2>(o, A);
+            tz_s = o.t;
             if (owned) {
                 if (p < kend) {
                     store_face<2>(A, at(g, i, j, p), o);
-                    if (p == 1) store_face<2>(A, at(g, i, j, g.N3 + 1), o);
+                    if (p == 1) {
+                        store_face<2>(A, at(g, i, j, g.N3 + 1), o);
+                        store_tp<2>(A, at(g, i, j, g.N3 + 1), o.t);
+                    }
                 }
                 if (p > k0) {
                     double Sc[5];
@@ -602,8 +672,12 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
                                                s_f[4][fpos], s_f[1][fpos], s_f[2][fpos], A);
                     if (own_x) {
                         store_face<0>(A, at(g, i, j, p), o);
-                        if (i == 1) store_face<0>(A, at(g, g.N1 + 1, j, p), o);
+                        if (i == 1) {
+                            store_face<0>(A, at(g, g.N1 + 1, j, p), o);
+                            store_tp<0>(A, at(g, g.N1 + 1, j, p), o.t);
+                        }
                     }
+                    tx_s = o.t;
                     mx = out_flux<0>(o, A);
                 }
                 if (yface) {
@@ -613,10 +687,16 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
                                                s_f[9][fpos], s_f[5][fpos], s_f[7][fpos], A);
                     if (own_y) {
                         store_face<1>(A, at(g, i, j, p), o);
-                        if (j == 1) store_face<1>(A, at(g, i, g.N2 + 1, p), o);
+                        if (j == 1) {
+                            store_face<1>(A, at(g, i, g.N2 + 1, p), o);
+                            store_tp<1>(A, at(g, i, g.N2 + 1, p), o.t);
+                        }
                     }
+                    ty_s = o.t;
                     my = out_flux<1>(o, A);
                 }
+                // an owned lane has computed all three of its faces here
+                if (owned) *(T3*)trec_of(A.Fo[0], A.fb, (unsigned)at(g, i, j, p)) = T3{tx_s, ty_s, tz_s};
                 for (int v = 0; v < 5; ++v) { s_m[v][tid] = mx.m[v]; s_m[5 + v][tid] = my.m[v]; }
             }
             stage(p + 1, nxt);
@@ -694,21 +774,23 @@ __global__ void k_batch_begin_step(Scalars* const* __restrict__ S, int nboxes) {
 
 // ---------------------------------------------------------------------------
 // Conversions between an FP64 field (element (0,0,0) at src/dst, elements
-// 0..n-1) and component v of the face records of a group (v < 4: u1, u2, u3,
-// p; v = 4: t').  To the records: t' = t - t0 for v = 4.  From them: t0 + t'
-// for v = 4, and with rho set, rho0 + p/c^2 from p.
-__global__ void k_face_from_double(const double* __restrict__ src, char* grp, unsigned long long fb, int v,
-                                   long long n, double t0) {
+// 0..n-1) and component v of the records of the orientation-d faces (group
+// pointer grp; xg the x-faces' group pointer of the same buffer, which holds
+// the T3 records): v < 4: u1, u2, u3, p; v = 4: t'.  To the records:
+// t' = t - t0 for v = 4.  From them: t0 + t' for v = 4, and with rho set,
+// rho0 + p/c^2 from p.
+__global__ void k_face_from_double(const double* __restrict__ src, char* grp, char* xg, unsigned long long fb, int d,
+                                   int v, long long n, double t0) {
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
-        if (v == 4) *(float*)tp_of(grp, fb, (unsigned)q) = (float)(src[q] - t0);
+        if (v == 4) *(float*)tp_of(xg, fb, d, (unsigned)q) = (float)(src[q] - t0);
         else ((float*)rec_of(grp, (unsigned)q))[v] = (float)src[q];
     }
 }
-__global__ void k_face_to_double(const char* grp, unsigned long long fb, int v, int rho, long long n, double t0,
-                                 double ro0, double c2, double* __restrict__ dst) {
+__global__ void k_face_to_double(const char* grp, const char* xg, unsigned long long fb, int d, int v, int rho,
+                                 long long n, double t0, double ro0, double c2, double* __restrict__ dst) {
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
         if (v == 4) {
-            dst[q] = t0 + (double)*tp_of(grp, fb, (unsigned)q);
+            dst[q] = t0 + (double)*tp_of(xg, fb, d, (unsigned)q);
         } else {
             const double x = (double)((const float*)rec_of(grp, (unsigned)q))[v];
             dst[q] = rho ? ro0 + x / c2 : x;
@@ -1035,8 +1117,8 @@ ark_status ark_mixed_face_to_host_layout(ark_ctx* c, int f, double* staging) {
     const int g = face_slot(fld, &v);
     if (g < 0) return c->fail(ARK_ERR_INVALID, "not a face field");
     hipLaunchKernelGGL(mixed::k_face_to_double, dim3(2048), dim3(256), 0, c->stream, c->gbase[g][c->gpar[g]],
-                       (unsigned long long)c->field_bytes, v, rho ? 1 : 0, face_elems(c), P.t0, P.ro0,
-                       P.sound * P.sound, staging);
+                       c->gbase[1][c->gpar[1]], (unsigned long long)c->field_bytes, g - 1, v, rho ? 1 : 0,
+                       face_elems(c), P.t0, P.ro0, P.sound * P.sound, staging);
     return c->hip(hipGetLastError(), "face to double");
 }
 
@@ -1049,8 +1131,8 @@ static ark_status face_from(ark_ctx* c, int f, const double* src, int which) {
     for (int b = 0; b < 2; ++b) {
         if (!(which & (1 << b))) continue;
         hipLaunchKernelGGL(mixed::k_face_from_double, dim3(2048), dim3(256), 0, c->stream, src,
-                           c->gbase[g][c->gpar[g] ^ b], (unsigned long long)c->field_bytes, v, face_elems(c),
-                           c->prm.t0);
+                           c->gbase[g][c->gpar[g] ^ b], c->gbase[1][c->gpar[1] ^ b], (unsigned long long)c->field_bytes,
+                           g - 1, v, face_elems(c), c->prm.t0);
         ark_status s = c->hip(hipGetLastError(), "face from double");
         if (s != ARK_OK) return s;
     }
@@ -1061,20 +1143,26 @@ ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* stag
     return face_from(c, f, staging, 3);
 }
 
-// After an initial state written as FP64 fields into both buffers of each
-// face group: the records of both buffers.  The records overlap the FP64
-// fields of their own allocation, so each group is converted from the other
-// buffer's FP64 copy, then copied over it.
+// After an initial state written as FP64 fields into the current buffer of
+// each face group: the records of both buffers.  The records overlap the FP64
+// fields of their own allocation, and the x group's also holds every
+// orientation's t' (T3 records), so all three groups are first copied to
+// their other buffer, then converted from that FP64 copy into the current
+// buffer, then copied over it.
 ark_status ark_mixed_faces_from_fp64(ark_ctx* c) {
     const size_t group_bytes = 5 * c->field_bytes;
     ark_status s = ARK_OK;
-    for (int g = 1; g < 4 && s == ARK_OK; ++g) {
-        char* cur = c->gbase[g][c->gpar[g]] - kOriginPad * sizeof(double);
-        char* oth = c->gbase[g][c->gpar[g] ^ 1] - kOriginPad * sizeof(double);
-        s = c->hip(hipMemcpyAsync(oth, cur, group_bytes, hipMemcpyDeviceToDevice, c->stream), "copy faces");
+    auto copy_groups = [&]() {
+        for (int g = 1; g < 4 && s == ARK_OK; ++g) {
+            char* cur = c->gbase[g][c->gpar[g]] - kOriginPad * sizeof(double);
+            char* oth = c->gbase[g][c->gpar[g] ^ 1] - kOriginPad * sizeof(double);
+            s = c->hip(hipMemcpyAsync(oth, cur, group_bytes, hipMemcpyDeviceToDevice, c->stream), "copy faces");
+        }
+    };
+    copy_groups();
+    for (int g = 1; g < 4 && s == ARK_OK; ++g)
         for (int v = 0; v < 5 && s == ARK_OK; ++v) s = face_from(c, kGroupFields[g][v], c->alt[kGroupFields[g][v]], 1);
-        if (s == ARK_OK) s = c->hip(hipMemcpyAsync(oth, cur, group_bytes, hipMemcpyDeviceToDevice, c->stream), "copy faces");
-    }
+    copy_groups();
     return s;
 }
 
