@@ -92,22 +92,69 @@ def dns(N: int, steps: int, snapshots: int, width: int, vis: float, eps: float, 
 
 
 def train(X: np.ndarray, T: np.ndarray, steps: int, batch: int, lr: float, seed: int = 0) -> Dict:
+    """A network trained on inputs X and targets T; "norm" holds the input and
+    output normalisation (mean, std of X, then of T) it was trained with."""
     m = MLP(seed=seed)
-    m.set_normalization(X.mean(0), X.std(0) + 1e-12, T.mean(0), T.std(0) + 1e-12)
+    norm = (X.mean(0), X.std(0) + 1e-12, T.mean(0), T.std(0) + 1e-12)
+    m.set_normalization(*norm)
     m.set_data(X, T)
     _, l0 = m.gradient(X[: min(len(X), 65536)], T[: min(len(T), 65536)])
     t0 = time.perf_counter()
     last = m.train(steps, batch=batch, lr=lr)
     wall = time.perf_counter() - t0
     _, l1 = m.gradient(X[: min(len(X), 65536)], T[: min(len(T), 65536)])
-    return {"mlp": m, "loss0": l0, "loss_end": l1, "last_batch_loss": last, "wall": wall}
+    return {"mlp": m, "norm": norm, "loss0": l0, "loss_end": l1, "last_batch_loss": last, "wall": wall}
 
 
-def les(ub0: np.ndarray, t_end: float, vis: float, eps: float, seed: int, mlp: Optional[MLP] = None,
+def set_amplitude(tr: Dict, alpha: float) -> None:
+    """Scale the trained network's stresses by alpha (through its output
+    normalisation: the stress is alpha (y std_T + mean_T) for network output y)."""
+    xm, xs, tm, ts = tr["norm"]
+    tr["mlp"].set_normalization(xm, xs, alpha * tm, alpha * ts)
+    tr["alpha"] = alpha
+
+
+def calibrate(d: Dict, tr: Dict, vis: float, eps: float, seed: int, delta: float, tau: float = 0.01,
+              alphas=(1.0, 1.25, 1.5, 1.75, 2.0, 2.5, 3.0)) -> Dict:
+    """A-posteriori amplitude of the closure, chosen on the training data.
+    A network fitted to the lag by least squares predicts its conditional mean
+    given the local gradients, which explain only part of it; the coarse run
+    then needs a stronger correction than that mean.  For each candidate
+    alpha, one coarse run from the filtered initial field passes through the
+    start times of the correction windows (d["windows"]), and its spectrum is
+    compared there with the filtered fine field it was trained on: the mean
+    error over shells 1..N/4 and over the windows that start in the last
+    quarter of the fine run (at least the last window).  Earlier windows see
+    a cascade still filling the high shells (at N/4, 1e-7 to 4e-5 of the peak
+    over the first half of the 512^3 run, 1e-4 to 1e-3 over its second half),
+    whose relative errors say little about the developed flow.  The
+    alpha with the least error among the runs that stay finite is set on the
+    network.  The fine run's final state, against which the closure is
+    judged, takes no part."""
+    w = d["windows"]
+    times = [x["t_n"] for x in w]
+    nc = d["ub0"].shape[-1]
+    later = [q for q in range(len(w)) if w[q]["t_n"] >= 0.75 * d["t_end"]] or [len(w) - 1]
+    table = []
+    for a in alphas:
+        set_amplitude(tr, a)
+        r = les(d["ub0"], times, vis, eps, seed, mlp=tr["mlp"], delta=delta, tau=tau)
+        errs = [spectrum_error(r["E_at"][q], np.asarray(w[q]["E_ref"]), nc // 4) for q in later]
+        table.append({"alpha": a, "nonfinite": r["nonfinite"], "err_windows": float(np.mean(errs))})
+    ok = [t for t in table if t["nonfinite"] == 0 and np.isfinite(t["err_windows"])]
+    best = min(ok, key=lambda t: t["err_windows"])["alpha"] if ok else 1.0
+    set_amplitude(tr, best)
+    return {"alpha": best, "candidates": table}
+
+
+def les(ub0: np.ndarray, t_end, vis: float, eps: float, seed: int, mlp: Optional[MLP] = None,
         delta: float = 0.0, tau: float = 0.01, kcut: int = 0) -> Dict:
     """A coarse run from the filtered field ub0 to t_end with the same
     forcing; with mlp, the learned closure, applied on the shells >= kcut only
-    when kcut > 0 (the scale-selective variant)."""
+    when kcut > 0 (the scale-selective variant).  t_end may be a list of
+    increasing times: the run then also records its spectrum on reaching each
+    ("E_at"), and "wall" excludes those spectra."""
+    times = list(t_end) if np.ndim(t_end) else [t_end]
     N3, N2, N1 = ub0.shape[1:]
     s = Solver(N1, N2, N3, VIS=vis, precision=PREC_MIXED)
     s.init_velocity(ub0)
@@ -117,13 +164,17 @@ def les(ub0: np.ndarray, t_end: float, vis: float, eps: float, seed: int, mlp: O
         if kcut > 0:
             s.set_closure_cutoff(kcut)
     s.synchronize()
-    t0 = time.perf_counter()
-    n = run_to(s, t_end)
-    s.synchronize()
-    wall = time.perf_counter() - t0
+    n, wall, E_at = 0, 0.0, []
+    for t in times:
+        t0 = time.perf_counter()
+        n += run_to(s, t)
+        s.synchronize()
+        wall += time.perf_counter() - t0
+        if len(times) > 1:
+            E_at.append(s.spectrum(N1 // 2 + 1))
     d = s.diagnostics(allow_nonfinite=True)
     out = {"steps": n, "time": d["time"], "wall": wall, "E": s.spectrum(N1 // 2 + 1), "ke": d["ke_physical"],
-           "eps_resolved": s.dissipation(), "nonfinite": d["nonfinite"]}
+           "eps_resolved": s.dissipation(), "nonfinite": d["nonfinite"], "E_at": E_at}
     s.close()
     return out
 
@@ -241,11 +292,12 @@ def correction_pairs(N: int, steps: int, pairs: int, width: int, vis: float, eps
         Xs.append(X)
         Ts.append(T.reshape(6, n).T.astype(np.float32))
         Ss.append(tsgs.reshape(6, n).T.astype(np.float32))
-        info.append({"step": state["done"], "t_n": t_n, "t_c": t_c, "F_rms": float(np.sqrt((F ** 2).mean()))})
+        info.append({"step": state["done"], "t_n": t_n, "t_c": t_c, "F_rms": float(np.sqrt((F ** 2).mean())),
+                     "E_ref": spectrum_of(ub, vis)})
     fine_to(steps)
     d = s.diagnostics(allow_nonfinite=True)
     e_end = s.dissipation()
-    out = {"X": np.concatenate(Xs), "T": np.concatenate(Ts), "T_sgs": np.concatenate(Ss), "pairs": info,
+    out = {"X": np.concatenate(Xs), "T": np.concatenate(Ts), "T_sgs": np.concatenate(Ss), "windows": info,
            "t_end": d["time"], "ub0": ub0, "ub_end": s.filter_box(width)[0], "dx": s.dx[0], "eps_end": e_end,
            "ke_end": d["ke_physical"], "Re_lambda_end": taylor_reynolds(d["ke_physical"], e_end, vis, volume),
            "nonfinite": d["nonfinite"], "wall": state["wall"], "steps": state["done"], "series": series}

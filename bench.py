@@ -223,7 +223,8 @@ def closure_experiment(N, steps, width, train_steps, batch, pairs=8, horizon=8):
     at eps = 0.1 (redrawn every 0.01 in time), nu = 1/360 so that
     Re_lambda is about 100.  `pairs` times the filtered fine state (width w)
     starts a coarse run of `horizon` steps, whose lag behind the filtered fine
-    run is the a-posteriori target of the closure network (the kept closure);
+    run is the a-posteriori target of the closure network (the kept closure,
+    its amplitude calibrated on the windows' start times: closure.calibrate);
     the exact subgrid stresses of the same states train a second network (the
     a-priori closure, for comparison).  (N/w)^3 runs from the filtered initial
     field, with and without each closure, are compared with the filtered fine
@@ -237,6 +238,10 @@ def closure_experiment(N, steps, width, train_steps, batch, pairs=8, horizon=8):
     nc = N // width
     delta = width * d["dx"]
     plain = C.les(d["ub0"], d["t_end"], vis, eps, seed)
+    raw = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=tr["mlp"], delta=delta)
+    t0 = time.perf_counter()
+    cal = C.calibrate(d, tr, vis, eps, seed, delta)
+    cal["wall_s"] = round(time.perf_counter() - t0, 3)
     model = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=tr["mlp"], delta=delta)
     prior = C.les(d["ub0"], d["t_end"], vis, eps, seed, mlp=ts["mlp"], delta=delta)
     tr["mlp"].close()
@@ -254,8 +259,15 @@ def closure_experiment(N, steps, width, train_steps, batch, pairs=8, horizon=8):
            "training": {"pairs": int(d["X"].shape[0]), "snapshots": pairs, "horizon_coarse_steps": horizon,
                         "steps": train_steps, "batch": batch, "a_posteriori_loss": [tr["loss0"], tr["loss_end"]],
                         "a_priori_loss": [ts["loss0"], ts["loss_end"]], "wall_s": round(tr["wall"] + ts["wall"], 3)},
+           "calibration": {"alpha": cal["alpha"], "wall_s": cal["wall_s"],
+                           "candidates": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in t.items()}
+                                          for t in cal["candidates"]],
+                           "note": "amplitude of the a-posteriori closure, chosen by the mean spectrum error "
+                                   "(shells 1..N/4) at the start times of the correction windows in the last "
+                                   "quarter of the fine run; the final state takes no part"},
            "coarse_grid": nc, "filter_width_cells": width, "Re_lambda_last_series": last.get("Re_lambda")}
-    for name, r in (("plain", plain), ("closure", model), ("closure_a_priori", prior)):
+    runs = (("plain", plain), ("closure", model), ("closure_alpha_1", raw), ("closure_a_priori", prior))
+    for name, r in runs:
         out[name] = {"steps": r["steps"], "wall_s": round(r["wall"], 4), "nonfinite": r["nonfinite"],
                      "spectrum_err_k<=%d" % (nc // 4): round(C.spectrum_error(r["E"], Eref, nc // 4), 4),
                      "spectrum_err_k<=%d" % (nc // 2): round(C.spectrum_error(r["E"], Eref, nc // 2), 4),
@@ -263,7 +275,7 @@ def closure_experiment(N, steps, width, train_steps, batch, pairs=8, horizon=8):
                      "dissipation_err_vs_fine": round(abs(r["eps_resolved"] - d["eps_end"]) / d["eps_end"], 4),
                      "wall_ratio_fine_over_coarse": round(d["wall"] / r["wall"], 1)}
     out["spectrum_filtered_fine"] = [float("%.5g" % v) for v in Eref[: nc // 2 + 1]]
-    for name, r in (("plain", plain), ("closure", model), ("closure_a_priori", prior)):
+    for name, r in runs:
         out["spectrum_" + name] = [float("%.5g" % v) for v in r["E"][: nc // 2 + 1]]
     return out
 
