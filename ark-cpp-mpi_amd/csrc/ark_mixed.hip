@@ -949,43 +949,45 @@ ark_status ensure_units(ark_ctx* c) {
     return s;
 }
 
-// Times both single-box shapes on this context (one launch, then kReps
-// launches each) and keeps the faster.  The timing launches write only the
-// buffers the step is about to overwrite and send their dt MIN to a scratch
-// slot, so the state is untouched; every cell's arithmetic is the same in
-// either tiling, so the choice changes only the speed.
+// Times both single-box shapes on this context and keeps the faster: one
+// untimed launch of each, then kRounds rounds that time one launch of each
+// shape in turn, each shape scored by its fastest launch (alternating and the
+// minimum keep a passing disturbance from deciding the choice).  The timing
+// launches write only the buffers the step is about to overwrite and send
+// their dt MIN to a scratch slot, so the state is untouched; every cell's
+// arithmetic is the same in either tiling, so the choice changes only the
+// speed.
 ark_status tune_single_shape(ark_ctx* c) {
-    constexpr int kReps = 3;
+    constexpr int kRounds = 4;
     unsigned long long* scratch = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     // three slots: the dt MIN, and the non-finite count and first step
     ark_status s = c->hip(hipMalloc(&scratch, 3 * sizeof(unsigned long long)), "hipMalloc tuning slot");
     if (s == ARK_OK) s = c->hip(hipEventCreate(&e0), "event");
     if (s == ARK_OK) s = c->hip(hipEventCreate(&e1), "event");
-    float best = 0.f;
-    int shape = 0;
     const mixed::Args a = make_args(c, 0, scratch, (long long*)(scratch + 1));
     for (int sh = 0; sh < 2 && s == ARK_OK; ++sh) {
         launch_single_shape(c, sh, a);
         s = c->hip(hipGetLastError(), "tuning launch");
-        if (s == ARK_OK) s = c->hip(hipEventRecord(e0, c->stream), "event");
-        for (int r = 0; r < kReps && s == ARK_OK; ++r) {
-            launch_single_shape(c, sh, a);
-            s = c->hip(hipGetLastError(), "tuning launch");
-        }
-        if (s == ARK_OK) s = c->hip(hipEventRecord(e1, c->stream), "event");
-        if (s == ARK_OK) s = c->hip(hipEventSynchronize(e1), "event sync");
-        float ms = 0.f;
-        if (s == ARK_OK) s = c->hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-        if (s == ARK_OK && (sh == 0 || ms < best)) {
-            best = ms;
-            shape = sh;
-        }
     }
+    float best[2] = {0.f, 0.f};
+    for (int r = 0; r < kRounds && s == ARK_OK; ++r)
+        for (int sh = 0; sh < 2 && s == ARK_OK; ++sh) {
+            s = c->hip(hipEventRecord(e0, c->stream), "event");
+            if (s == ARK_OK) {
+                launch_single_shape(c, sh, a);
+                s = c->hip(hipGetLastError(), "tuning launch");
+            }
+            if (s == ARK_OK) s = c->hip(hipEventRecord(e1, c->stream), "event");
+            if (s == ARK_OK) s = c->hip(hipEventSynchronize(e1), "event sync");
+            float ms = 0.f;
+            if (s == ARK_OK) s = c->hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            if (s == ARK_OK && (r == 0 || ms < best[sh])) best[sh] = ms;
+        }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (scratch) (void)hipFree(scratch);
-    if (s == ARK_OK) c->mixed_shape = shape;
+    if (s == ARK_OK) c->mixed_shape = best[1] < best[0] ? 1 : 0;
     return s;
 }
 
