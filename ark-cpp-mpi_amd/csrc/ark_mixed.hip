@@ -535,7 +535,6 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
     for (int p = k0 - 1; p <= kend; ++p) {
         asm volatile("" : "+s"(A.fb));
         const bool inner = p >= k0 && p < kend;
-        // prefetch plane p+1 (and the cell two planes up, for its z-stress)
         // plane p+1's velocities into LDS (it is read there from now on), then
         // prefetch plane p+1 (and the cell two planes up, for its z-stress)
         put_u(p + 1, Anext);
