@@ -31,7 +31,9 @@ Fixtures written:
       (recorded as "buffer_dependence").  Also the SHA-256 of the reference's
       own output files for the channel (WriteDataParaView after 10 and 20
       steps, WriteAverageValues of the averages from step 5 to 20).
-      ``python make_fixtures.py channel`` writes only this file.
+      ``python make_fixtures.py channel`` writes only this file;
+      ``python make_fixtures.py channel_long`` adds the same run's digests
+      after 2000 and 20000 steps ("long").
   tgv_digests.json
       per-array SHA-256 and norms of the 16^3 x 10-step and 64^3 x 200-step
       Taylor-Green runs (1 rank, full arrays), their per-step dt, KE and mass
@@ -238,6 +240,25 @@ def channel_fixture(path: str, work: str) -> None:
         json.dump(out, fh, indent=1)
 
 
+def channel_long_fixture(path: str, work: str, dumps=(2000, 20000)) -> None:
+    """The channel run of channel_fixture continued to `dumps` steps: per-array
+    SHA-256 of all 41 arrays and dt, TIME, KE and mass there, added to
+    channel_digests.json as "long" (the full per-step series would be
+    megabytes)."""
+    dig = json.load(open(path))
+    steps = max(dumps)
+    ref = channel_run(work, steps, list(dumps), CHANNEL_ZERO_ENV)
+    ser = read_series(ref)
+    long = {"steps": list(dumps), "states": {}, "series_at": {}}
+    for s in dumps:
+        st = OracleState.from_dir(os.path.join(ref, f"s{s}"))
+        long["states"][str(s)] = {n: digest(st[n]) for n in FIELDS}
+        long["series_at"][str(s)] = {k: ser[k][s] for k in ser if k != "step"}
+    dig["long"] = long
+    with open(path, "w") as fh:
+        json.dump(dig, fh, indent=1)
+
+
 def tgv256_fixture(path: str, work: str, steps: int = 5) -> None:
     """The 256^3 Taylor-Green case of the GPU workload, `steps` steps of the
     unmodified reference: one rank (all 41 arrays with their ghosts, so the
@@ -291,6 +312,16 @@ def main() -> None:
         finally:
             shutil.rmtree(work, ignore_errors=True)
         print("256^3 digests added to", os.path.join(HERE, "tgv_digests.json"))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "channel_long":
+        if not os.path.exists(HARNESS):
+            sys.exit("reference harness not built: make -C oracle ref")
+        work = tempfile.mkdtemp(prefix="ark_golden_")
+        try:
+            channel_long_fixture(os.path.join(HERE, "channel_digests.json"), work)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        print("long channel digests added to", os.path.join(HERE, "channel_digests.json"))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "channel":
         if not os.path.exists(HARNESS):

@@ -46,6 +46,34 @@ def test_channel_matches_reference_digests():
         assert s.dt == float.fromhex(dig["series"]["dt"][done])
 
 
+def test_channel_long_run_matches_reference_digests():
+    """The same channel run continued to 2000 and 20000 steps (TIME 3.6 and
+    35.3; the reference harness took 13 minutes for it on one host core, the
+    GPU takes about a second): all 41 arrays bitwise against the unmodified
+    reference's digests there, dt and TIME bitwise, and its kinetic-energy
+    and mass sums to 1e-13 (the library sums in a fixed tree order, the
+    reference in loop order).  The reference's kinetic energy (WriteEnergy,
+    source/main.cpp:2851-2876) is the sum of |u|^2/2 times the domain volume,
+    20 x 6 x 0.1 here; the library reports the sum alone."""
+    dig = json.load(open(os.path.join(GOLDEN, "channel_digests.json")))["long"]
+    st = OracleState.channel()
+    volume = float(np.prod([n * h for n, h in zip(st.N, st.dx)]))
+    with channel_solver(st) as s:
+        done = 0
+        for k in dig["steps"]:
+            s.step(k - done)
+            done = k
+            got = s.download_all()
+            bad = [n for n in FIELDS if sha(got[n]) != dig["states"][str(k)][n]["sha256"]]
+            assert not bad, (k, bad)
+            ref = dig["series_at"][str(k)]
+            d = s.diagnostics()
+            assert d["nstep"] == k and d["nonfinite"] == 0
+            assert s.dt == float.fromhex(ref["dt"]) and d["time"] == float.fromhex(ref["TIME"]), k
+            assert d["ke_reference"] * volume == pytest.approx(float.fromhex(ref["ke_reference"]), rel=1e-13)
+            assert d["mass"] == pytest.approx(float.fromhex(ref["mass"]), rel=1e-13)
+
+
 def test_channel_phase_by_phase_vs_oracle():
     """Each reference phase on its own, first step and step 20, against the
     oracle, every array bitwise (localises a mismatch to one phase)."""
