@@ -39,6 +39,9 @@ Fixtures written:
       Taylor-Green runs (1 rank, full arrays), their per-step dt, KE and mass
       series, and the result of the decomposition-invariance check (the same
       32^3 run on 1, 2, 4 and 8 ranks must give bitwise-equal fields).
+      ``python make_fixtures.py tgv256`` adds the 256^3 x 5 run (one rank, all
+      arrays; eight ranks, interior cells); ``python make_fixtures.py
+      tgv256_long`` the 256^3 x 50 run (eight ranks, interior cells).
 """
 from __future__ import annotations
 
@@ -302,7 +305,37 @@ def tgv256_fixture(path: str, work: str, steps: int = 5) -> None:
         json.dump(dig, fh, indent=1)
 
 
+def tgv256_long_fixture(path: str, work: str, steps: int = 50) -> None:
+    """The 256^3 Taylor-Green case over `steps` steps of the unmodified
+    reference on eight ranks (dims 4x2x1, z unsplit): SHA-256 of the gathered
+    interior of the level-n cell fields (after adding +0.0, as in
+    tgv256_fixture) and the dt series, added to tgv_digests.json as
+    "256^3x<steps>_interior"."""
+    dig = json.load(open(path))
+    N = 256
+    d8 = tgv_run(N, steps, work, [steps], ranks=8)
+    h8 = {}
+    for n in ("ronCon", "u1nCon", "u2nCon", "u3nCon", "tnCon"):
+        a = np.fromfile(os.path.join(d8, f"s{steps}", n + ".interior.f64"), dtype="<f8").reshape(N, N, N) + 0.0
+        h8[n] = hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+    dig["runs"][f"{N}^3x{steps}_interior"] = {"N": N, "steps": steps, "ranks": 8, "series": read_series(d8),
+                                              "interior_sha256_plus_zero": h8}
+    shutil.rmtree(d8, ignore_errors=True)
+    with open(path, "w") as fh:
+        json.dump(dig, fh, indent=1)
+
+
 def main() -> None:
+    if len(sys.argv) > 1 and sys.argv[1] == "tgv256_long":
+        if not os.path.exists(HARNESS):
+            sys.exit("reference harness not built: make -C oracle ref")
+        work = tempfile.mkdtemp(prefix="ark_golden_")
+        try:
+            tgv256_long_fixture(os.path.join(HERE, "tgv_digests.json"), work)
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+        print("256^3 x 50 interior digests added to", os.path.join(HERE, "tgv_digests.json"))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "tgv256":
         if not os.path.exists(HARNESS):
             sys.exit("reference harness not built: make -C oracle ref")

@@ -180,6 +180,32 @@ def test_tgv_digest_256(how):
         assert s.diagnostics()["nstep"] == 5
 
 
+@pytest.mark.parametrize("how", ["step", "advance"])
+def test_tgv_digest_256_long(how):
+    """Fifty steps of the 256^3 workload against the unmodified reference on
+    eight ranks (dims 4x2x1): the dt of every tenth step and, at the end, the
+    interior of the level-n density and velocities (after adding +0.0, as the
+    reference's gathered digest).  The passive t is not compared: the
+    reference never exchanges the t ghost faces (source/main.cpp:1047-1698),
+    so on eight ranks its t differs from one rank's, which this GPU run is
+    (the 5-step run above records both).  `advance`: fused steps only;
+    `step`: each call of ten runs nine fused steps and one through the phase
+    kernels."""
+    run = json.load(open(os.path.join(GOLDEN, "tgv_digests.json")))["runs"]["256^3x50_interior"]
+    want_dt = [float.fromhex(v) for v in run["series"]["dt"][1:]]
+    assert len(want_dt) == 50
+    N = 256
+    with arkamd.Solver(N, N, N) as s:
+        s.init_tgv()
+        for done in range(10, 51, 10):
+            s.step(10) if how == "step" else s.advance(10)
+            assert s.dt == want_dt[done - 1], done
+        for n in ("ronCon", "u1nCon", "u2nCon", "u3nCon"):
+            got = s.download(n)[1:-1, 1:-1, 1:-1] + 0.0
+            assert sha(got) == run["interior_sha256_plus_zero"][n], n
+        assert s.diagnostics()["nstep"] == 50
+
+
 @pytest.mark.parametrize("N", [(32, 32, 32), (20, 13, 7), (70, 40, 90)])
 def test_fused_equals_phases(N):
     with arkamd.Solver(*N) as a, arkamd.Solver(*N) as b:
