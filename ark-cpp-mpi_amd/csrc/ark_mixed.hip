@@ -321,7 +321,8 @@ struct PlaneIn {
 // One tile of one box: tile column (tbx, tby), owned planes k0 .. kend-1.
 // A lane keeps in registers only what is not also in LDS: its own x- and
 // y-faces and the FP64 velocities of its cells are read back from the
-// staged planes, which keeps the tile within 168 VGPRs.
+// staged planes, which keeps the tile within two waves per SIMD (about 225
+// VGPRs, none spilled).
 template <bool Quirk, class S>
 __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int k0, int kend) {
     constexpr int TX = S::TX, TY = S::TY, RX = S::RX, RY = S::RY, NTHR = S::NTHR;
