@@ -337,6 +337,8 @@ static ark_status drop_pending(ark_ctx* c);
 
 ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2, int e3) {
     if (!c || !host) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     ark_status s = check_extents(c, field, e1, e2, e3);
     if (s != ARK_OK) return s;
     // TimeStepSize reads these: a dt left pending by fused steps no longer holds
@@ -377,6 +379,8 @@ ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2,
 
 ark_status ark_download(ark_ctx* c, int field, double* host, int e1, int e2, int e3) {
     if (!c || !host) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     ark_status s = check_extents(c, field, e1, e2, e3);
     if (s != ARK_OK) return s;
     if (c->prm.precision == ARK_PREC_MIXED && is_face_state(field)) {
@@ -620,11 +624,15 @@ ark_status ark_cfl_finalize(ark_ctx* c, int advance) {
 }
 
 ark_status ark_cfl_scan(ark_ctx* c) {
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     CflArgs a{c};
     return c->launch("cfl", run_cfl, &a);
 }
 
 ark_status ark_cfl_step(ark_ctx* c, int advance) {
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     CflArgs a{c};
     ark_status s = c->launch("cfl", run_cfl, &a);
     return s == ARK_OK ? ark_cfl_finalize(c, advance) : s;
@@ -807,6 +815,8 @@ ark_status ark_step(ark_ctx* c, int nsteps) {
 
 ark_status ark_diagnostics(ark_ctx* c, ark_diag* out) {
     if (!c || !out) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     double* part = c->diag_part;
     double* res = c->diag_part + 5 * c->diag_blocks;
     ark_status s = c->hip(launch_diag(c->stream, c->geo, c->fields(), c->K, part, c->diag_blocks, res), "diag");
@@ -897,6 +907,8 @@ ark_status ark_init_channel(ark_ctx* c, double u10) {
 
 ark_status ark_init_tgv(ark_ctx* c, double U0) {
     if (!c) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     const int N[3] = {c->geo.N1, c->geo.N2, c->geo.N3};
     const double dx[3] = {c->prm.dx1, c->prm.dx2, c->prm.dx3};
     // 18 tables of up to max(N)+3 entries
@@ -1088,6 +1100,8 @@ static ark_status plane_args(ark_ctx* c, int field, int axis, int q, int na, int
 
 ark_status ark_get_plane(ark_ctx* c, int field, int axis, int q, double* host, int na, int nb) {
     if (!c || !host) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     ark_status s = plane_args(c, field, axis, q, na, nb);
     if (s == ARK_OK && (field == F_ro1 || field == F_ro2 || field == F_ro3)) s = ark_refresh_rho(c);
     if (s == ARK_OK) s = c->hip(launch_plane(c->stream, c->geo, c->cur[field], axis, q, c->plane_buf, 1), "plane");
@@ -1099,6 +1113,8 @@ ark_status ark_get_plane(ark_ctx* c, int field, int axis, int q, double* host, i
 
 ark_status ark_set_plane(ark_ctx* c, int field, int axis, int q, const double* host, int na, int nb) {
     if (!c || !host) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     ark_status s = plane_args(c, field, axis, q, na, nb);
     if (s != ARK_OK) return s;
     if (field >= F_sigm11) c->sigma_dirty = true;
@@ -1119,6 +1135,8 @@ ark_status ark_set_plane(ark_ctx* c, int field, int axis, int q, const double* h
 
 ark_status ark_boundary_conditions(ark_ctx* c, int axis, const int fields[5], int periodic, double t0) {
     if (!c || !fields || axis < 0 || axis > 2) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     BcFields F;
     for (int q = 0; q < 5; ++q) {
         if (fields[q] < 0 || !is_cell_field(fields[q]))

@@ -78,6 +78,7 @@ struct ark_ctx {
     bool rho_consistent = false;       // face densities 1..N+1 equal ro0 + p/c^2 bit for bit
     bool rho_stored = true;            // the stored face-density arrays are current
     bool dt_pending = false;           // S->dt_bits holds the next step's dt from a fused step
+    bool cells_rec = false;            // mixed: the level-n cells are in the stepping layout (ark_cells_layout)
     bool sigma_dirty = false;          // an upload left non-zero values in a stress array
     double** sigma_ptrs = nullptr;     // device array: allocation bases of the nine stress arrays
     // Non-finite detection: the stepping calls copy Scalars::nf into nf_host
@@ -110,7 +111,8 @@ ark_status ark_turb_force_step(ark_ctx* c, hipStream_t st);   // this step's for
 ark_status ark_closure_step(ark_ctx* c, hipStream_t st);      // the closure's body force (after the forcing)
 // the learned closure's stresses on the solver's grid (ark_mlp.hip)
 ark_status ark_mlp_closure_device(struct ark_mlp* m, const arkd::Geo& g, const double* u1, const double* u2,
-                                  const double* u3, const double inv[3], float delta, float* Y, hipStream_t st);
+                                  const double* u3, int stride, const double inv[3], float delta, float* Y,
+                                  hipStream_t st);
 ark_status ark_cfl_step(ark_ctx* c, int advance);       // TimeStepSize on Con, then dt := MIN
 ark_status ark_cfl_finalize(ark_ctx* c, int mode);      // see k_cfl_finalize
 ark_status ark_begin_step(ark_ctx* c);                  // dt of a new step (pending MIN or Con), TIME, nStep
@@ -139,6 +141,11 @@ void ark_swap_group(ark_ctx* c, int g);
 // face field between its FP32 record storage and an FP64 field in the common
 // layout.
 ark_status ark_mixed_steps(ark_ctx* c, int nsteps);
+// Mixed precision steps the level-n cells as 32-byte records of (rho, u1, u2,
+// u3) with t in its own field; everything else reads and writes them per
+// field.  Converts the current buffer of the cell group into the other one
+// and swaps the two when the layout differs; a no-op for FP64 contexts.
+ark_status ark_cells_layout(ark_ctx* c, bool records);
 ark_status ark_mixed_face_to_host_layout(ark_ctx* c, int f, double* staging);
 ark_status ark_mixed_face_from_host_layout(ark_ctx* c, int f, const double* staging);
 // After an initial state in FP64 face fields (both buffers): the FP32 records.

@@ -27,6 +27,7 @@
 // kernel (ark_fused.hip): owned columns plus a one-cell ring, z-march.
 #include <algorithm>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "ark_internal.hpp"
@@ -70,7 +71,7 @@ struct Args {
     float cfl_dx[3];
     // FP64 constants of the closing update
     double ro0d, t0d, dvc;
-    const char* A;      // FP64 level-n cells: rho, u1, u2, u3, t
+    const char* A;      // FP64 level-n cells: (rho, u1, u2, u3) records (cell_rec), t in field 4
     char* Ao;
     const char* F[3];   // FP32 faces normal to d: u1, u2, u3, p, t'
     char* Fo[3];
@@ -121,16 +122,24 @@ __device__ __forceinline__ const float* tp_of(const char* xg, unsigned long long
 __device__ __forceinline__ int wrapc(int q, int N) { return q < 1 ? q + N : (q > N ? q - N : q); }
 __device__ __forceinline__ int mapf(int f, int N) { return f == 0 ? N : (f == N + 2 ? 2 : f); }
 
+// Level-n cells while a context steps in mixed precision: (rho, u1, u2, u3)
+// of cell idx as one 32-byte record at grp + 8 + 32 idx (two 16-byte loads
+// instead of four 8-byte ones; 32-byte aligned, since grp + 8 starts a
+// 128-byte line), t in its own field (field 4 of the group, which the
+// records stop short of).  The offset is 64-bit: 32 idx passes 2^32 above
+// about 500^3 cells.
+__device__ __forceinline__ const double2* cell_rec(const char* grp, int idx) {
+    return (const double2*)(grp + 8 + (unsigned long long)(unsigned)idx * 32ull);
+}
 struct CellD { double ro, u[3], t; };                 // FP64 level-n cell
 struct CellF { float dr, u[3], tp; };                 // FP32 view: rho - rho0, u, t - t0
 struct FaceF { float u[3], p, tp; };
 
 __device__ __forceinline__ CellD load_cell(const Args& a, int idx) {
     CellD c;
-    c.ro = ldd(a.A, a.fb, 0, idx);
-    c.u[0] = ldd(a.A, a.fb, 1, idx);
-    c.u[1] = ldd(a.A, a.fb, 2, idx);
-    c.u[2] = ldd(a.A, a.fb, 3, idx);
+    const double2* r = cell_rec(a.A, idx);
+    const double2 a01 = r[0], a23 = r[1];
+    c.ro = a01.x; c.u[0] = a01.y; c.u[1] = a23.x; c.u[2] = a23.y;
     c.t = ldd(a.A, a.fb, 4, idx);
     return c;
 }
@@ -460,7 +469,13 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
         if (do_ex) r.ex = load_face(A, ex_d, ex_off + zo);
         r.ou[0] = r.ou[1] = r.ou[2] = 0.0;
         if (do_o)
-            for (int m = 0; m < 3; ++m) r.ou[m] = ldd(A.A, A.fb, 1 + m, o_off + zo);
+        {
+            const double2* rc = cell_rec(A.A, o_off + zo);
+            r.ou[0] = ((const double*)rc)[1];
+            const double2 u23 = rc[1];
+            r.ou[1] = u23.x;
+            r.ou[2] = u23.y;
+        }
         r.fo[0] = r.fo[1] = r.fo[2] = 0.f;
         if (A.force) {
             for (int m = 0; m < 3; ++m) r.fo[m] = A.force[m * A.fstride + base + zo];
@@ -646,8 +661,9 @@ __device__ __forceinline__ void mixed_tile(const Args& a, int tbx, int tby, int 
                     const double ro1 = Sc[0];
                     const double inv = rcpd(ro1);
                     const int co = at(g, i, j, p - 1);
-                    std_(A.Ao, A.fb, 0, co, ro1);
-                    for (int m = 0; m < 3; ++m) std_(A.Ao, A.fb, 1 + m, co, Sc[1 + m] * inv);
+                    double2* w = (double2*)cell_rec(A.Ao, co);
+                    w[0] = make_double2(ro1, Sc[1] * inv);
+                    w[1] = make_double2(Sc[2] * inv, Sc[3] * inv);
                     std_(A.Ao, A.fb, 4, co, A.t0d + Sc[4] * inv);
                 }
             }
@@ -760,6 +776,36 @@ __global__ void __launch_bounds__(S::NTHR) k_fused_mixed_batch(const Args* __res
 }
 
 // dt of a new step for every box of a batch (k_cfl_finalize, mode 1, per box).
+// Level-n cells between the per-field layout (fields 0-4 of the group at
+// stride fb) and the stepping layout (cell_rec, t in field 4), from one
+// buffer of the group into the other; n indices of the common index map.
+// Packing converts every index.  Unpacking writes every index too: interior
+// cells from their records, ghost cells (and row padding) from the interior
+// cell they are a periodic image of, since the mixed step reads no ghost.
+__global__ void k_cells_convert(Geo g, const char* __restrict__ src, char* __restrict__ dst, unsigned long long fb,
+                                long long n, int pack) {
+    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double v[5];
+    if (pack) {
+        for (int f = 0; f < 5; ++f) v[f] = *(const double*)(src + f * fb + q * 8);
+        double2* w = (double2*)(dst + 8 + q * 32);
+        w[0] = make_double2(v[0], v[1]);
+        w[1] = make_double2(v[2], v[3]);
+        *(double*)(dst + 4 * fb + q * 8) = v[4];
+        return;
+    }
+    const int k = (int)(q / g.sxy), rem = (int)(q - (long long)k * g.sxy);
+    const int j = rem / g.px, i = rem - j * g.px;
+    auto img = [](int a, int N) { return (((a - 1) % N) + N) % N + 1; };
+    const long long qs = img(i, g.N1) + (long long)img(j, g.N2) * g.px + (long long)img(k, g.N3) * g.sxy;
+    const double2* r = (const double2*)(src + 8 + qs * 32);
+    const double2 a = r[0], b = r[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    v[4] = *(const double*)(src + 4 * fb + qs * 8);
+    for (int f = 0; f < 5; ++f) *(double*)(dst + f * fb + q * 8) = v[f];
+}
+
 __global__ void k_batch_begin_step(Scalars* const* __restrict__ S, int nboxes) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nboxes) return;
@@ -1090,6 +1136,19 @@ ark_status ark_get_tile_shape(const ark_ctx* c, int* shape) {
     return ARK_OK;
 }
 
+ark_status ark_cells_layout(ark_ctx* c, bool records) {
+    if (c->prm.precision != ARK_PREC_MIXED || c->cells_rec == records) return ARK_OK;
+    const long long n = (long long)c->geo.sxy * (c->geo.N3 + 3);
+    hipLaunchKernelGGL(mixed::k_cells_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->geo,
+                       (const char*)c->gbase[0][c->gpar[0]], c->gbase[0][c->gpar[0] ^ 1], c->field_bytes, n,
+                       records ? 1 : 0);
+    ark_status s = c->hip(hipGetLastError(), records ? "cells to records" : "cells to fields");
+    if (s != ARK_OK) return s;
+    ark_swap_group(c, 0);
+    c->cells_rec = records;
+    return ARK_OK;
+}
+
 ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     if (c->geo.N1 < 2 || c->geo.N2 < 2 || c->geo.N3 < 2)
         return c->fail(ARK_ERR_UNSUPPORTED, "mixed precision needs N1, N2, N3 >= 2");
@@ -1097,6 +1156,7 @@ ark_status ark_mixed_steps(ark_ctx* c, int nsteps) {
     s = ensure_units(c);
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
+        if (s == ARK_OK) s = ark_cells_layout(c, true);
         // the first step of a context picks its tile shape, with this step's dt
         if (s == ARK_OK && c->mixed_shape < 0) s = tune_single_shape(c);
         if (s == ARK_OK) s = ark_turb_force_step(c, c->stream);
@@ -1196,7 +1256,7 @@ struct ark_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     ark_force_batch* forcing = nullptr;           // the forced boxes' mode tables, one launch per step
     bool built = false;
-    std::vector<int> gpar_at_build;               // box buffer parities the tables were built for
+    std::vector<std::array<int, 4>> gpar_at_build;   // box group parities the tables were built for
     int parity = 0;                               // which table the next step uses
     std::vector<double> tune_ms;                  // ms per step of each candidate (shape-major)
     std::string err;
@@ -1281,7 +1341,7 @@ ark_status build_graphs(ark_batch* b) {
     }
     if (s != ARK_OK) return s;
     b->gpar_at_build.clear();
-    for (ark_ctx* c : b->box) b->gpar_at_build.push_back(c->gpar[0]);
+    for (ark_ctx* c : b->box) b->gpar_at_build.push_back({c->gpar[0], c->gpar[1], c->gpar[2], c->gpar[3]});
     b->parity = 0;
     b->built = true;
     return ARK_OK;
@@ -1296,7 +1356,9 @@ ark_status prepare_boxes(ark_batch* b) {
             if (s != ARK_OK) return b->fail(s, std::string("batch: CFL scan of a box: ") + c->err);
             c->dt_pending = true;
         }
-        ark_status s = b->hip(hipStreamSynchronize(c->stream), "box stream");
+        ark_status s = ark_cells_layout(c, true);
+        if (s != ARK_OK) return b->fail(s, std::string("batch: cell layout of a box: ") + c->err);
+        s = b->hip(hipStreamSynchronize(c->stream), "box stream");
         if (s != ARK_OK) return s;
     }
     return ARK_OK;
@@ -1449,13 +1511,12 @@ ark_status ark_batch_step(ark_batch* b, int nsteps) {
     ark_status s = b->hip(hipSetDevice(b->device), "hipSetDevice");
     if (s == ARK_OK) s = prepare_boxes(b);
     if (s != ARK_OK) return s;
-    // The tables hold buffer addresses: rebuild them when a box has stepped
-    // on its own since (its buffers swapped), or when the batch is new.
+    // The tables hold buffer addresses: rebuild them when a box's buffers
+    // have swapped since other than by the batch's own steps (a step on its
+    // own, a change of cell layout), or when the batch is new.
     bool same = b->built;
-    for (size_t i = 0; same && i < b->box.size(); ++i) {
-        const int expect = b->gpar_at_build[i] ^ b->parity;
-        same = b->box[i]->gpar[0] == expect;
-    }
+    for (size_t i = 0; same && i < b->box.size(); ++i)
+        for (int g = 0; g < 4; ++g) same = same && b->box[i]->gpar[g] == (b->gpar_at_build[i][g] ^ b->parity);
     if (!same) s = build_graphs(b);
     for (int n = 0; n < nsteps && s == ARK_OK; ++n) {
         s = b->hip(hipGraphLaunch(b->exec[b->parity], b->stream), "graph launch");

@@ -343,6 +343,7 @@ __global__ void k_reduce_adam(const float* partial, int nslots, double* G, float
 struct GridIn {
     Geo g;
     const double* u[3];
+    int cs;             // element stride of u: 1 per field, 4 in the 32-byte cell records
     double inv[3];      // 1 / (2 dx_d)
     float delta;
 };
@@ -404,9 +405,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                         const int jp = j == G.N2 ? 1 : j + 1, jm = j == 1 ? G.N2 : j - 1;
                         const int kp = k == G.N3 ? 1 : k + 1, km = k == 1 ? G.N3 : k - 1;
                         const double* u = gi.u[g];
-                        x[0] = (float)((u[at(G, ip, j, k)] - u[at(G, im_, j, k)]) * gi.inv[0]);
-                        x[1] = (float)((u[at(G, i, jp, k)] - u[at(G, i, jm, k)]) * gi.inv[1]);
-                        x[2] = (float)((u[at(G, i, j, kp)] - u[at(G, i, j, km)]) * gi.inv[2]);
+                        const long long cs = gi.cs;
+                        x[0] = (float)((u[cs * at(G, ip, j, k)] - u[cs * at(G, im_, j, k)]) * gi.inv[0]);
+                        x[1] = (float)((u[cs * at(G, i, jp, k)] - u[cs * at(G, i, jm, k)]) * gi.inv[1]);
+                        x[2] = (float)((u[cs * at(G, i, j, kp)] - u[cs * at(G, i, j, km)]) * gi.inv[2]);
                     } else {
                         x[0] = gi.delta;
                     }
@@ -697,7 +699,7 @@ ark_status ark_mlp_predict(ark_mlp* m, const float* X, float* Y, long long n) {
 // 6 per cell into Y), on the solver's stream, ordered after everything
 // already enqueued on the network's own stream.
 ark_status ark_mlp_closure_device(ark_mlp* m, const arkd::Geo& g, const double* u1, const double* u2, const double* u3,
-                                  const double inv[3], float delta, float* Y, hipStream_t st) {
+                                  int stride, const double inv[3], float delta, float* Y, hipStream_t st) {
     ark_status s = m->hip(hipEventRecord(m->done, m->stream), "event");
     if (s == ARK_OK) s = m->hip(hipStreamWaitEvent(st, m->done, 0), "wait");
     if (s != ARK_OK) return s;
@@ -706,6 +708,7 @@ ark_status ark_mlp_closure_device(ark_mlp* m, const arkd::Geo& g, const double* 
     gi.u[0] = u1;
     gi.u[1] = u2;
     gi.u[2] = u3;
+    gi.cs = stride;
     for (int d = 0; d < 3; ++d) gi.inv[d] = inv[d];
     gi.delta = delta;
     const long long n = (long long)g.N1 * g.N2 * g.N3;

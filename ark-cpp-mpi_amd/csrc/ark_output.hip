@@ -90,6 +90,8 @@ extern "C" {
 
 ark_status ark_average_values(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     if (!c->average) {
         double* p = nullptr;
         ark_status s = c->hip(hipMalloc(&p, 3 * c->elems * sizeof(double)), "hipMalloc averages");
@@ -107,6 +109,8 @@ ark_status ark_average_values(ark_ctx* c) {
 
 ark_status ark_write_paraview(ark_ctx* c, const char* dir_path, int need_swap) {
     if (!c || !dir_path) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     Scalars sc;
     ark_status s = scalars_of(c, sc);
     if (s != ARK_OK) return s;

@@ -518,6 +518,8 @@ dim3 interior_grid(const Geo& g, int ext) {
 // real field) are work.
 ark_status install_velocity(ark_ctx* c, Fft& f, const hipfftDoubleComplex* U, hipfftDoubleComplex* W, double* R,
                             double scale, const hipfftDoubleComplex* P = nullptr) {
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     const long long nc = (long long)f.s.H1 * g.N2 * g.N3;
     ark_status s = ARK_OK;
@@ -731,8 +733,17 @@ ark_status ark_closure_step(ark_ctx* c, hipStream_t st) {
     if (!c->closure) return ARK_OK;
     const Geo& g = c->geo;
     const double inv[3] = {0.5 / c->prm.dx1, 0.5 / c->prm.dx2, 0.5 / c->prm.dx3};
-    ark_status s = ark_mlp_closure_device(c->closure, g, c->cur[F_u1nCon], c->cur[F_u2nCon], c->cur[F_u3nCon], inv,
-                                          (float)c->closure_delta, c->les_Y, st);
+    // velocities per field, or (while stepping) doubles 1-3 of each cell's
+    // 32-byte record at group + 8 + 32 idx
+    const double* u[3] = {c->cur[F_u1nCon], c->cur[F_u2nCon], c->cur[F_u3nCon]};
+    int stride = 1;
+    if (c->cells_rec) {
+        const double* rec = (const double*)(c->gbase[0][c->gpar[0]] + 8);
+        for (int m = 0; m < 3; ++m) u[m] = rec + 1 + m;
+        stride = 4;
+    }
+    ark_status s = ark_mlp_closure_device(c->closure, g, u[0], u[1], u[2], stride, inv, (float)c->closure_delta,
+                                          c->les_Y, st);
     if (s != ARK_OK) return s;
     const int add = c->force_eps > 0.0 ? 1 : 0;
     const float inv0 = (float)(0.5 / c->prm.dx1), inv1 = (float)(0.5 / c->prm.dx2), inv2 = (float)(0.5 / c->prm.dx3);
@@ -821,6 +832,8 @@ ark_status ark_set_closure_cutoff(ark_ctx* c, int kcut) {
 
 ark_status ark_init_state(ark_ctx* c, const double* u, const double* rho) {
     if (!c || !u) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "init_velocity: N must be even");
     Fft f;
@@ -855,6 +868,8 @@ ark_status ark_init_velocity(ark_ctx* c, const double* u) { return ark_init_stat
 
 ark_status ark_filter_density(ark_ctx* c, int width, double* rhobar) {
     if (!c || width < 1 || !rhobar) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if (g.N1 % width || g.N2 % width || g.N3 % width)
         return c->fail(ARK_ERR_INVALID, "filter: the width must divide N1, N2, N3");
@@ -873,6 +888,8 @@ ark_status ark_filter_density(ark_ctx* c, int width, double* rhobar) {
 
 ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, double urms) {
     if (!c || !(k0 > 0.0) || !(urms >= 0.0)) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "turbulence: N must be even");
     Fft f;
@@ -914,6 +931,8 @@ ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, d
 
 ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
     if (!c || nbins < 1 || !E) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     Fft f;
     ark_status s = make_fft(c, f, false);
@@ -943,6 +962,8 @@ ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
 
 ark_status ark_dissipation(ark_ctx* c, double* eps) {
     if (!c || !eps) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     DevBuf acc;
     ark_status s = c->hip(hipMalloc(&acc.p, (kSumBlocks + 1) * sizeof(double)), "hipMalloc");
     double* part = (double*)acc.p;
@@ -992,6 +1013,8 @@ ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps, doub
 
 ark_status ark_filter_box(ark_ctx* c, int width, double* ubar, double* tau, double* grad) {
     if (!c || width < 1 || !ubar || !tau || !grad) return ARK_ERR_INVALID;
+    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if (g.N1 % width || g.N2 % width || g.N3 % width)
         return c->fail(ARK_ERR_INVALID, "filter: the width must divide N1, N2, N3");

@@ -171,6 +171,84 @@ def test_mixed_tile_shapes_bitwise_equal(kind, scheme):
 
 # ---------------------------------------------------------------------------
 # The 256^3 Taylor-Green workload itself, every persistent field.
+def _mixed_box(kind, N=(24, 20, 16)):
+    s = arkamd.Solver(*N, precision=arkamd.PREC_MIXED, scheme=arkamd.SCHEME_CORRECTED)
+    if kind == "tgv":
+        s.init_tgv(1.0)
+    else:
+        s.init_turbulence(5, 4.0, 1.0)
+        s.set_forcing(6, 0.1, 0.01)
+    return s
+
+
+def _periodic_images(a, N):
+    """A cell array (extents N + 2) rebuilt from its interior: ghost 0 is
+    cell N and ghost N + 1 is cell 1 along every axis."""
+    N1, N2, N3 = N
+    return np.take(np.take(np.take(a, [N3, *range(1, N3 + 1), 1], 0), [N2, *range(1, N2 + 1), 1], 1),
+                   [N1, *range(1, N1 + 1), 1], 2)
+
+
+@pytest.mark.parametrize("kind", ["tgv", "forced"])
+def test_mixed_cell_layout_changes_keep_steps_bitwise(kind):
+    """Mixed steps hold the level-n cells as 32-byte records; every per-field
+    reader (download, diagnostics, spectrum) converts them back first, and the
+    next step converts again.  Five steps taken with such reads between them
+    equal five steps in one call bit for bit, every persistent array whole,
+    and the cell ghosts are the periodic images of the interior (the step
+    itself reads no ghost)."""
+    N = (24, 20, 16)
+    with _mixed_box(kind, N) as a:
+        a.step(5)
+        ref = a.download_all(arkamd.PERSISTENT)
+    with _mixed_box(kind, N) as b:
+        b.step(2)
+        b.download("u2nCon")
+        b.diagnostics()
+        b.step(1)
+        b.spectrum(8)
+        b.step(1)
+        b.download("tnCon")
+        b.step(1)
+        got = b.download_all(arkamd.PERSISTENT)
+    for n in arkamd.PERSISTENT:
+        assert bits_equal(got[n], ref[n]), n
+    for n in CELL:
+        assert bits_equal(ref[n], _periodic_images(ref[n], N)), n
+
+
+def test_batch_after_a_box_is_read_between_batch_steps():
+    """A box read on its own between batch steps (its cells converted to the
+    per-field layout and back, one buffer swap each way) still steps in the
+    batch bit for bit like a standalone run: the batch rebuilds its argument
+    tables when any buffer group of a box has swapped outside its steps."""
+    kinds = ["tgv", "forced", "tgv"]
+    alone = []
+    for q, k in enumerate(kinds):
+        with _mixed_box(k) as s:
+            s.step(6 if q == 0 else 5)      # box 0 also takes one step on its own
+            alone.append(s.download_all(arkamd.PERSISTENT))
+    boxes = [_mixed_box(k) for k in kinds]
+    try:
+        with arkamd.Batch(boxes) as b:
+            b.step(2)
+            b.synchronize()
+            boxes[1].download("u1nCon")
+            b.step(2)
+            b.synchronize()
+            boxes[0].step(1)
+            b.step(1)
+            b.synchronize()
+            boxes[0].download("ronCon")
+        for q, s in enumerate(boxes):
+            got = s.download_all(arkamd.PERSISTENT)
+            for n in arkamd.PERSISTENT:
+                assert bits_equal(got[n], alone[q][n]), (q, n)
+    finally:
+        for s in boxes:
+            s.close()
+
+
 VEL = {"u1nCon", "u2nCon", "u3nCon", "u11", "u21", "u31", "u12", "u22", "u32", "u13", "u23", "u33"}
 PRES = {"p1", "p2", "p3"}
 TEMP = {"tnCon", "t1", "t2", "t3"}
