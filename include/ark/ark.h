@@ -114,7 +114,9 @@ const char* ark_last_error(const ark_ctx* ctx);
 
 /* Copy one Arr3d (host pointer = Arr3d::ref(), extents as constructed) into
  * or out of the context.  Extents must be the reference's for that field.
- * ark_download synchronises. */
+ * ark_download synchronises.  In a mixed-precision context that has stepped,
+ * the ghost cells of the level-n cell fields read as the periodic images of
+ * the interior (the mixed step itself reads no ghost). */
 ark_status ark_upload(ark_ctx* ctx, int field, const double* host, int e1, int e2, int e3);
 ark_status ark_download(ark_ctx* ctx, int field, double* host, int e1, int e2, int e3);
 
