@@ -252,7 +252,8 @@ def closure_experiment(N, steps, width, train_steps, batch, pairs=8, horizon=8):
                     "nu": vis, "dissipation_end": d["eps_end"], "Re_lambda_end": round(d["Re_lambda_end"], 1),
                     "eps_injected": eps,
                     "eps_balance": "not reached: dissipation %.3g against injection %.3g at t=%.3g (about half an "
-                                   "eddy turnover)" % (d["eps_end"], eps, d["t_end"])
+                                   "eddy turnover; the longer runs of forced_turbulence_stationary reach "
+                                   "it)" % (d["eps_end"], eps, d["t_end"])
                     if abs(d["eps_end"] - eps) > 0.1 * eps else "reached",
                     "series": [{k: (float("%.6g" % v) if isinstance(v, float) else v) for k, v in r.items()}
                                for r in d["series"]]},
@@ -277,6 +278,57 @@ def closure_experiment(N, steps, width, train_steps, batch, pairs=8, horizon=8):
     out["spectrum_filtered_fine"] = [float("%.5g" % v) for v in Eref[: nc // 2 + 1]]
     for name, r in runs:
         out["spectrum_" + name] = [float("%.5g" % v) for v in r["E"][: nc // 2 + 1]]
+    return out
+
+
+def stationary_turbulence(t_end, grids=(64, 128)):
+    """Forced homogeneous turbulence run to a statistically stationary state:
+    the closure experiment's initial field, forcing and viscosity (eps = 0.1
+    injected, random phases redrawn every 0.01, nu = 1/360), mixed precision,
+    corrected scheme (the reference scheme's runs become non-finite near
+    t = 14 at these grids).  Over the second half of the run: the kinetic
+    energy per unit volume, its drift between the last two quarters, and the
+    resolved viscous dissipation against the injected power.  With the energy
+    stationary the total dissipation equals the injection; the resolved share
+    rises with the grid, the rest being the scheme's own grid-scale
+    dissipation."""
+    import numpy as np
+    import arkamd
+    vis, eps, seed = 1.0 / 360.0, 0.1, 31
+    out = {"eps_injected": eps, "nu": vis, "scheme": "corrected", "t_end": t_end}
+    for N in grids:
+        V = (2 * np.pi) ** 3
+        with arkamd.Solver(N, N, N, VIS=vis, precision=arkamd.PREC_MIXED, scheme=arkamd.SCHEME_CORRECTED) as s:
+            s.init_turbulence(seed, 4.0, 1.0)
+            s.set_forcing(seed + 1, eps, 0.01)
+            series, steps, nxt = [], 0, 0.0
+            t0 = time.perf_counter()
+            while True:
+                d = s.diagnostics(allow_nonfinite=True)
+                finite = bool(np.isfinite(d["ke_physical"]))
+                if finite and d["time"] >= nxt:
+                    series.append((d["time"], d["ke_physical"] / V, s.dissipation()))
+                    nxt += 0.5
+                if d["time"] >= t_end or not finite:
+                    break
+                s.step(100)
+                steps += 100
+            wall = time.perf_counter() - t0
+        t = np.array([r[0] for r in series])
+        ke = np.array([r[1] for r in series])
+        er = np.array([r[2] for r in series])
+        h, q3 = t >= t_end / 2, t >= 0.75 * t_end
+        ke_mean, er_mean = float(ke[h].mean()), float(er[h].mean())
+        up2 = 2.0 * ke_mean / 3.0
+        out["%d^3" % N] = {
+            "steps": steps, "wall_s": round(wall, 2), "finite": finite,
+            "ke_per_volume_mean": round(ke_mean, 4), "ke_per_volume_range": [round(float(ke[h].min()), 4),
+                                                                            round(float(ke[h].max()), 4)],
+            "ke_drift_last_quarters": round(float((ke[q3].mean() - ke[h & ~q3].mean()) / ke_mean), 4),
+            "eps_resolved_mean": round(er_mean, 5), "resolved_share_of_injection": round(er_mean / eps, 3),
+            "Re_lambda_total_eps": round(float(up2 * np.sqrt(15.0 / (vis * eps))), 1),
+            "Re_lambda_resolved_eps": round(float(up2 * np.sqrt(15.0 / (vis * er_mean))), 1),
+            "series_every_1": [[round(a, 3), round(b, 5), round(c, 5)] for a, b, c in series[::2]]}
     return out
 
 
@@ -517,6 +569,8 @@ def main():
     ap.add_argument("--closure-steps", type=int, default=2000)
     ap.add_argument("--closure-train-steps", type=int, default=20000)
     ap.add_argument("--batch-boxes", type=int, default=64, help="boxes of the batched run (0: skip)")
+    ap.add_argument("--stationary-time", type=float, default=30.0,
+                    help="time of the forced-turbulence runs to a stationary state, 64^3 and 128^3 (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--phase-steps", type=int, default=3,
                     help="steps of the phase-by-phase FP64 path timed per kernel (0: skip)")
@@ -526,7 +580,7 @@ def main():
                     help="steps of the CPU baseline's 256^3 sample (about 45 s on 16 host cores)")
     ap.add_argument("--cpu-ranks", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["fp64", "mixed", "channel", "batched", "closure", "les"], default=None,
+    ap.add_argument("--only", choices=["fp64", "mixed", "channel", "batched", "closure", "les", "stationary"], default=None,
                     help="time one precision only, or only the reference's channel (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
@@ -548,6 +602,9 @@ def main():
         return
     if args.only == "les":
         print(json.dumps({"les_timing": les_timing(args.closure_size // 4, args.steps)}))
+        return
+    if args.only == "stationary":
+        print(json.dumps({"forced_turbulence_stationary": stationary_turbulence(args.stationary_time)}))
         return
     if args.only == "closure":
         print(json.dumps({"forced_turbulence_closure": closure_experiment(
@@ -601,6 +658,8 @@ def main():
     if args.closure_size > 0 and not args.only:
         line["forced_turbulence_closure"] = closure_experiment(args.closure_size, args.closure_steps, 4,
                                                                args.closure_train_steps, 65536)
+    if args.stationary_time > 0 and not args.only:
+        line["forced_turbulence_stationary"] = stationary_turbulence(args.stationary_time)
     if args.accuracy_steps > 0 and not args.only:
         line["accuracy"] = accuracy(N, args.accuracy_steps)
     if args.pcie_steps > 0 and not args.only:

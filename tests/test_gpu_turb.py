@@ -187,6 +187,38 @@ def test_forced_turbulence_runs():
         assert s.dissipation() > 0
 
 
+def test_forced_turbulence_reaches_a_stationary_state():
+    """Long-run regression of the forcing and the corrected scheme at 64^3
+    (seed 31, eps = 0.1 injected, nu = 1/360, 24 time units, about 10 000
+    steps): after the initial field's transient the kinetic energy per unit
+    volume stays bounded and stops drifting (the means of the last two
+    quarters differ by less than 20%; it fluctuates by about +-20% within a
+    quarter), so the total dissipation balances the injection on average,
+    and the resolved viscous part of it is a fraction of the injection (about
+    half at this grid; the rest is the scheme's grid-scale dissipation)."""
+    vis, eps, V = 1.0 / 360.0, 0.1, (2 * np.pi) ** 3
+    with arkamd.Solver(64, 64, 64, VIS=vis, precision=arkamd.PREC_MIXED, scheme=arkamd.SCHEME_CORRECTED) as s:
+        s.init_turbulence(31, 4.0, 1.0)
+        s.set_forcing(32, eps, 0.01)
+        rec = []
+        while True:
+            d = s.diagnostics()
+            rec.append((d["time"], d["ke_physical"] / V, s.dissipation()))
+            if d["time"] >= 24.0:
+                break
+            s.step(200)
+    t = np.array([r[0] for r in rec])
+    ke = np.array([r[1] for r in rec])
+    er = np.array([r[2] for r in rec])
+    a, b = (t >= 12) & (t < 18), t >= 18
+    assert np.isfinite(ke).all() and np.isfinite(er).all()
+    assert ke[0] == pytest.approx(1.5, rel=1e-6)        # u_rms 1 per component
+    assert 0.3 < ke[a | b].min() and ke[a | b].max() < 1.0
+    assert abs(ke[b].mean() - ke[a].mean()) < 0.2 * ke[a | b].mean()
+    share = er[a | b].mean() / eps
+    assert 0.3 < share < 1.0, share
+
+
 @pytest.mark.parametrize("prec", [arkamd.PREC_FP64, arkamd.PREC_MIXED])
 def test_init_state_and_filter_density(prec):
     """ark_init_state puts a given density on the cells (and, shifted half a
