@@ -44,6 +44,20 @@ const int kGroupFields[4][5] = {
     {F_u13, F_u23, F_u33, F_p3, F_t3},
 };
 
+void ark_set_group_buffers(ark_ctx* c, int g, char* cur, char* alt) {
+    c->gbase[g][c->gpar[g]] = cur;
+    c->gbase[g][c->gpar[g] ^ 1] = alt;
+    for (int v = 0; v < 5; ++v) {
+        c->cur[kGroupFields[g][v]] = (double*)(cur + v * c->field_bytes);
+        c->alt[kGroupFields[g][v]] = (double*)(alt + v * c->field_bytes);
+    }
+}
+
+ark_status ark_per_field(ark_ctx* c) {
+    ark_status s = ark_cells_layout(c, false);
+    return s == ARK_OK ? ark_fused_layout(c, false) : s;
+}
+
 void ark_swap_group(ark_ctx* c, int g) {
     for (int v = 0; v < 5; ++v) std::swap(c->cur[kGroupFields[g][v]], c->alt[kGroupFields[g][v]]);
     c->gpar[g] ^= 1;
@@ -337,7 +351,7 @@ static ark_status drop_pending(ark_ctx* c);
 
 ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2, int e3) {
     if (!c || !host) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     ark_status s = check_extents(c, field, e1, e2, e3);
     if (s != ARK_OK) return s;
@@ -379,7 +393,7 @@ ark_status ark_upload(ark_ctx* c, int field, const double* host, int e1, int e2,
 
 ark_status ark_download(ark_ctx* c, int field, double* host, int e1, int e2, int e3) {
     if (!c || !host) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     ark_status s = check_extents(c, field, e1, e2, e3);
     if (s != ARK_OK) return s;
@@ -624,14 +638,14 @@ ark_status ark_cfl_finalize(ark_ctx* c, int advance) {
 }
 
 ark_status ark_cfl_scan(ark_ctx* c) {
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     CflArgs a{c};
     return c->launch("cfl", run_cfl, &a);
 }
 
 ark_status ark_cfl_step(ark_ctx* c, int advance) {
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     CflArgs a{c};
     ark_status s = c->launch("cfl", run_cfl, &a);
@@ -660,7 +674,8 @@ hipError_t run_rho(void* p) {
 
 ark_status ark_refresh_rho(ark_ctx* c) {
     if (c->rho_stored) return ARK_OK;
-    ark_status s = c->launch("rho_faces", run_rho, c);
+    ark_status s = ark_per_field(c);
+    if (s == ARK_OK) s = c->launch("rho_faces", run_rho, c);
     if (s == ARK_OK) c->rho_stored = true;
     return s;
 }
@@ -673,7 +688,8 @@ ark_status ark_refresh_rho(ark_ctx* c) {
 static ark_status stress_tensor(ark_ctx* c, bool use_forces);
 static ark_status phase1(ark_ctx* c, bool use_forces);
 ark_status ark_phase_body(ark_ctx* c) {
-    ark_status s = phase1(c, false);
+    ark_status s = ark_per_field(c);
+    if (s == ARK_OK) s = phase1(c, false);
     if (s == ARK_OK) s = stress_tensor(c, true);
     if (s == ARK_OK) s = ark_phase2(c);
     if (s == ARK_OK) s = phase1(c, true);
@@ -709,6 +725,8 @@ static ark_status phase1(ark_ctx* c, bool use_forces) {
 
 ark_status ark_phase1(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
+    if (lay != ARK_OK) return lay;
     return phase1(c, false);
 }
 
@@ -730,11 +748,15 @@ static ark_status stress_tensor(ark_ctx* c, bool use_forces) {
 
 ark_status ark_stress_tensor(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
+    if (lay != ARK_OK) return lay;
     return stress_tensor(c, false);
 }
 
 ark_status ark_use_forces(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
+    if (lay != ARK_OK) return lay;
     ark_status s = require_fluid(c);
     if (s == ARK_OK) s = drop_pending(c);
     return s == ARK_OK ? c->launch("use_forces", run_use_forces, c) : s;
@@ -742,6 +764,8 @@ ark_status ark_use_forces(ark_ctx* c) {
 
 ark_status ark_phase2(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
+    if (lay != ARK_OK) return lay;
     ark_status s = require_fluid(c);
     if (s == ARK_OK) s = drop_pending(c);
     if (s == ARK_OK) s = c->launch("ghost_phase2", run_phase2_ghost, c);
@@ -764,6 +788,8 @@ ark_status ark_phase2(ark_ctx* c) {
 // after the capture (which ran that host side once) is the state after any
 // number of pairs.  The graph lives for this call only.
 ark_status ark_phase_pairs(ark_ctx* c, int npairs) {
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
+    if (lay != ARK_OK) return lay;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ark_status s = c->hip(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -815,7 +841,7 @@ ark_status ark_step(ark_ctx* c, int nsteps) {
 
 ark_status ark_diagnostics(ark_ctx* c, ark_diag* out) {
     if (!c || !out) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     double* part = c->diag_part;
     double* res = c->diag_part + 5 * c->diag_blocks;
@@ -907,7 +933,7 @@ ark_status ark_init_channel(ark_ctx* c, double u10) {
 
 ark_status ark_init_tgv(ark_ctx* c, double U0) {
     if (!c) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     const int N[3] = {c->geo.N1, c->geo.N2, c->geo.N3};
     const double dx[3] = {c->prm.dx1, c->prm.dx2, c->prm.dx3};
@@ -1100,7 +1126,7 @@ static ark_status plane_args(ark_ctx* c, int field, int axis, int q, int na, int
 
 ark_status ark_get_plane(ark_ctx* c, int field, int axis, int q, double* host, int na, int nb) {
     if (!c || !host) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     ark_status s = plane_args(c, field, axis, q, na, nb);
     if (s == ARK_OK && (field == F_ro1 || field == F_ro2 || field == F_ro3)) s = ark_refresh_rho(c);
@@ -1113,7 +1139,7 @@ ark_status ark_get_plane(ark_ctx* c, int field, int axis, int q, double* host, i
 
 ark_status ark_set_plane(ark_ctx* c, int field, int axis, int q, const double* host, int na, int nb) {
     if (!c || !host) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     ark_status s = plane_args(c, field, axis, q, na, nb);
     if (s != ARK_OK) return s;
@@ -1135,7 +1161,7 @@ ark_status ark_set_plane(ark_ctx* c, int field, int axis, int q, const double* h
 
 ark_status ark_boundary_conditions(ark_ctx* c, int axis, const int fields[5], int periodic, double t0) {
     if (!c || !fields || axis < 0 || axis > 2) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     BcFields F;
     for (int q = 0; q < 5; ++q) {

@@ -73,21 +73,58 @@ __device__ __forceinline__ void gst(char* grp, unsigned long long fb, int v, int
 __device__ __forceinline__ int wrapc(int q, int N) { return q < 1 ? q + N : (q > N ? q - N : q); }
 __device__ __forceinline__ int mapf(int f, int N) { return f == 0 ? N : (f == N + 2 ? 2 : f); }
 
+// Record layout of a group (ark_fused_layout; a long run of fused steps):
+// fields 0-3 of index idx as one 32-byte record at grp + 8 + 32 idx (two
+// 16-byte accesses instead of four 8-byte ones; 32-byte aligned, since
+// grp + 8 starts a 128-byte line), field 4 where the per-field layout has it
+// (the records stop short of it).  64-bit offset: 32 idx passes 2^32 above
+// about 500^3 cells.
+__device__ __forceinline__ const double2* grec(const char* grp, int idx) {
+    return (const double2*)(grp + 8 + (unsigned long long)(unsigned)idx * 32ull);
+}
 struct Cell5 { double ro, u1, u2, u3, t; };
 struct Face5 { double u1, u2, u3, p, t; };
 struct Flux5 { double m[5]; };
 
+template <bool Rec>
 __device__ __forceinline__ Cell5 load_cell(const FusedArgs& a, int idx) {
     Cell5 c;
-    c.ro = gld(a.A, a.fb, 0, idx); c.u1 = gld(a.A, a.fb, 1, idx); c.u2 = gld(a.A, a.fb, 2, idx);
-    c.u3 = gld(a.A, a.fb, 3, idx); c.t = gld(a.A, a.fb, 4, idx);
+    if constexpr (Rec) {
+        const double2* r = grec(a.A, idx);
+        const double2 a01 = r[0], a23 = r[1];
+        c.ro = a01.x; c.u1 = a01.y; c.u2 = a23.x; c.u3 = a23.y;
+    } else {
+        c.ro = gld(a.A, a.fb, 0, idx); c.u1 = gld(a.A, a.fb, 1, idx); c.u2 = gld(a.A, a.fb, 2, idx);
+        c.u3 = gld(a.A, a.fb, 3, idx);
+    }
+    c.t = gld(a.A, a.fb, 4, idx);
     return c;
 }
+// The velocity (fields 1-3) of a cell, for the stress.
+template <bool Rec>
+__device__ __forceinline__ void load_velocity(const FusedArgs& a, int idx, double u[3]) {
+    if constexpr (Rec) {
+        const double2* r = grec(a.A, idx);
+        u[0] = ((const double*)r)[1];
+        const double2 b = r[1];
+        u[1] = b.x; u[2] = b.y;
+    } else {
+        for (int m = 0; m < 3; ++m) u[m] = gld(a.A, a.fb, 1 + m, idx);
+    }
+}
 
+template <bool Rec>
 __device__ __forceinline__ Face5 load_face(const FusedArgs& a, int d, int idx) {
     Face5 f;
-    f.u1 = gld(a.F[d], a.fb, 0, idx); f.u2 = gld(a.F[d], a.fb, 1, idx); f.u3 = gld(a.F[d], a.fb, 2, idx);
-    f.p = gld(a.F[d], a.fb, 3, idx); f.t = gld(a.F[d], a.fb, 4, idx);
+    if constexpr (Rec) {
+        const double2* r = grec(a.F[d], idx);
+        const double2 a01 = r[0], a23 = r[1];
+        f.u1 = a01.x; f.u2 = a01.y; f.u3 = a23.x; f.p = a23.y;
+    } else {
+        f.u1 = gld(a.F[d], a.fb, 0, idx); f.u2 = gld(a.F[d], a.fb, 1, idx); f.u3 = gld(a.F[d], a.fb, 2, idx);
+        f.p = gld(a.F[d], a.fb, 3, idx);
+    }
+    f.t = gld(a.F[d], a.fb, 4, idx);
     return f;
 }
 
@@ -161,13 +198,21 @@ __device__ __forceinline__ FaceOut new_face(const Fwd& fw, const Cell5& hb, cons
                               ucomp(hb, TB), ucomp(hf, TB), old.t, fu(old, TA), fu(old, TB), K);
 }
 
-template <int D>
+template <int D, bool Rec>
 __device__ __forceinline__ void store_face(const FusedArgs& a, int idx, const FaceOut& o) {
     constexpr int TA = D == 0 ? 1 : 0, TB = D == 2 ? 1 : 2;
-    gst(a.Fo[D], a.fb, D, idx, o.un);
-    gst(a.Fo[D], a.fb, TA, idx, o.a);
-    gst(a.Fo[D], a.fb, TB, idx, o.b);
-    gst(a.Fo[D], a.fb, 3, idx, o.p);
+    if constexpr (Rec) {
+        double u[3];
+        u[D] = o.un; u[TA] = o.a; u[TB] = o.b;
+        double2* w = (double2*)grec(a.Fo[D], idx);
+        w[0] = make_double2(u[0], u[1]);
+        w[1] = make_double2(u[2], o.p);
+    } else {
+        gst(a.Fo[D], a.fb, D, idx, o.un);
+        gst(a.Fo[D], a.fb, TA, idx, o.a);
+        gst(a.Fo[D], a.fb, TB, idx, o.b);
+        gst(a.Fo[D], a.fb, 3, idx, o.p);
+    }
     gst(a.Fo[D], a.fb, 4, idx, o.t);
 }
 
@@ -199,7 +244,7 @@ union LdsSlot {
 // the same height and share their rings through L2; die (XCD) b % 8 takes a
 // contiguous run of chunks, so the columns in flight on one die are
 // neighbours.
-template <bool Quirk>
+template <bool Quirk, bool Rec>
 __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __restrict__ units, int nbx) {
     __shared__ LdsSlot lds[NTHR];
     const Geo& g = a.g;
@@ -227,11 +272,11 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
     FusedArgs A = a;
 
     // rolling state along z; plane p+1's cells are loaded during plane p-1
-    Cell5 A_cur = load_cell(A, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
-    Cell5 A_nx = load_cell(A, at(g, iw, jw, wrapc(k0, g.N3)));
+    Cell5 A_cur = load_cell<Rec>(A, at(g, iw, jw, wrapc(k0 - 1, g.N3)));
+    Cell5 A_nx = load_cell<Rec>(A, at(g, iw, jw, wrapc(k0, g.N3)));
     double sz_lo[3];
     {
-        const Cell5 A_below = load_cell(A, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
+        const Cell5 A_below = load_cell<Rec>(A, at(g, iw, jw, wrapc(k0 - 2, g.N3)));
         sz_lo[0] = div_rn(VIS * (A_cur.u1 - A_below.u1), K.dx[2], K.rdx[2]);
         sz_lo[1] = div_rn(VIS * (A_cur.u2 - A_below.u2), K.dx[2], K.rdx[2]);
         sz_lo[2] = div_rn(VIS * (A_cur.u3 - A_below.u3), K.dx[2], K.rdx[2]);
@@ -259,17 +304,22 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         const int pw = wrapc(p, g.N3);
         const int c = at(g, iw, jw, pw);
         const Cell5 A_next = A_nx;
-        if (p < kend) A_nx = load_cell(A, at(g, iw, jw, wrapc(p + 2, g.N3)));
+        if (p < kend) A_nx = load_cell<Rec>(A, at(g, iw, jw, wrapc(p + 2, g.N3)));
 
         // ---- stress force f (StressTensor, source/main.cpp:856-1023) -------
         double f[3];
         {
             const int cw = at(g, iwm, jw, pw), ce = at(g, iwp, jw, pw);
             const int cs = at(g, iw, jwm, pw), cn = at(g, iw, jwp, pw);
+            double vW[3], vE[3], vS[3], vN[3];
+            load_velocity<Rec>(A, cw, vW);
+            load_velocity<Rec>(A, ce, vE);
+            load_velocity<Rec>(A, cs, vS);
+            load_velocity<Rec>(A, cn, vN);
             for (int m = 0; m < 3; ++m) {
                 const double uc = ucomp(A_cur, m);
-                const double uw = gld(A.A, A.fb, 1 + m, cw), ue = gld(A.A, A.fb, 1 + m, ce);
-                const double us = gld(A.A, A.fb, 1 + m, cs), un = gld(A.A, A.fb, 1 + m, cn);
+                const double uw = vW[m], ue = vE[m];
+                const double us = vS[m], un = vN[m];
                 double sx_lo = div_rn(VIS * (uc - uw), K.dx[0], K.rdx[0]);
                 double sx_hi = div_rn(VIS * (ue - uc), K.dx[0], K.rdx[0]);
                 double sy_lo = div_rn(VIS * (uc - us), K.dx[1], K.rdx[1]);
@@ -288,10 +338,10 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
         Bwd bz;
         double oz[3];                 // old t, tangential a, tangential b of the low z-face
         {
-            const Face5 Xw = load_face(A, 0, c), Xe = load_face(A, 0, c + 1);
-            const Face5 Ys = load_face(A, 1, c), Yn = load_face(A, 1, c + px);
-            const Face5 Zb = zt_prev_at == pw ? zt_prev : load_face(A, 2, c);
-            const Face5 Zt = load_face(A, 2, c + sxy);
+            const Face5 Xw = load_face<Rec>(A, 0, c), Xe = load_face<Rec>(A, 0, c + 1);
+            const Face5 Ys = load_face<Rec>(A, 1, c), Yn = load_face<Rec>(A, 1, c + px);
+            const Face5 Zb = zt_prev_at == pw ? zt_prev : load_face<Rec>(A, 2, c);
+            const Face5 Zt = load_face<Rec>(A, 2, c + sxy);
             zt_prev = Zt;
             zt_prev_at = pw + 1;
             {
@@ -312,22 +362,22 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
             Face5 Pzb = Zb, Pzf = Zt;
             if (i != iw) {
                 const int cl = at(g, 0, jw, pw);
-                Pxb = load_face(A, 0, cl + mapf(i, g.N1));
-                Pxf = load_face(A, 0, cl + mapf(i + 1, g.N1));
+                Pxb = load_face<Rec>(A, 0, cl + mapf(i, g.N1));
+                Pxf = load_face<Rec>(A, 0, cl + mapf(i + 1, g.N1));
                 Pxb.t = gld(A.F[0], A.fb, 4, cl + i);
                 Pxf.t = gld(A.F[0], A.fb, 4, cl + i + 1);
             }
             if (j != jw) {
                 const int cl = at(g, iw, 0, pw);
-                Pyb = load_face(A, 1, cl + mapf(j, g.N2) * px);
-                Pyf = load_face(A, 1, cl + mapf(j + 1, g.N2) * px);
+                Pyb = load_face<Rec>(A, 1, cl + mapf(j, g.N2) * px);
+                Pyf = load_face<Rec>(A, 1, cl + mapf(j + 1, g.N2) * px);
                 Pyb.t = gld(A.F[1], A.fb, 4, cl + j * px);
                 Pyf.t = gld(A.F[1], A.fb, 4, cl + (j + 1) * px);
             }
             if (p != pw) {
                 const int cl = at(g, iw, jw, 0);
-                Pzb = load_face(A, 2, cl + mapf(p, g.N3) * sxy);
-                Pzf = load_face(A, 2, cl + mapf(p + 1, g.N3) * sxy);
+                Pzb = load_face<Rec>(A, 2, cl + mapf(p, g.N3) * sxy);
+                Pzf = load_face<Rec>(A, 2, cl + mapf(p + 1, g.N3) * sxy);
                 Pzb.t = gld(A.F[2], A.fb, 4, cl + p * sxy);
                 Pzf.t = gld(A.F[2], A.fb, 4, cl + (p + 1) * sxy);
             }
@@ -360,17 +410,23 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
                                                  H_prev.u2, H.u2, oz[0], oz[1], oz[2], K);
             mz = new_flux<2>(o);
             if (owned) {
-                if (p < kend || kend == g.N3 + 1) store_face<2>(A, at(g, i, j, p), o);
+                if (p < kend || kend == g.N3 + 1) store_face<2, Rec>(A, at(g, i, j, p), o);
                 if (p > k0) {
                     double S[5], f_prev[3];
                     for (int v = 0; v < 5; ++v) S[v] = cy_get(12 + v) + (mz.m[v] - cy_get(17 + v)) * K.ds3;
                     for (int m = 0; m < 3; ++m) f_prev[m] = cy_get(9 + m);
                     const Cell5 An = conservative_update(H_prev, S, f_prev, dt, K);
                     const int co = at(g, i, j, p - 1);
-                    gst(A.Ao, A.fb, 0, co, An.ro);
-                    gst(A.Ao, A.fb, 1, co, An.u1);
-                    gst(A.Ao, A.fb, 2, co, An.u2);
-                    gst(A.Ao, A.fb, 3, co, An.u3);
+                    if constexpr (Rec) {
+                        double2* w = (double2*)grec(A.Ao, co);
+                        w[0] = make_double2(An.ro, An.u1);
+                        w[1] = make_double2(An.u2, An.u3);
+                    } else {
+                        gst(A.Ao, A.fb, 0, co, An.ro);
+                        gst(A.Ao, A.fb, 1, co, An.u1);
+                        gst(A.Ao, A.fb, 2, co, An.u2);
+                        gst(A.Ao, A.fb, 3, co, An.u3);
+                    }
                     gst(A.Ao, A.fb, 4, co, An.t);
                 }
             }
@@ -407,7 +463,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
                 const Fwd fwb{nb.fx[0], nb.fx[1], nb.fx[2], nb.fx[3]};
                 const FaceOut o = face_combine<true>(fwb, bx, nb.h[1], H.u1, nb.h[4], H.t, nb.h[2], H.u2, nb.h[3],
                                                      H.u3, ox[0], ox[1], ox[2], K);
-                if (xface_store) store_face<0>(A, at(g, i, j, p), o);
+                if (xface_store) store_face<0, Rec>(A, at(g, i, j, p), o);
                 mx = new_flux<0>(o);
             }
             if (yface) {
@@ -415,7 +471,7 @@ __global__ void __launch_bounds__(NTHR) k_fused_step(FusedArgs a, const int4* __
                 const Fwd fwb{nb.fy[0], nb.fy[1], nb.fy[2], nb.fy[3]};
                 const FaceOut o = face_combine<true>(fwb, by, nb.h[2], H.u2, nb.h[4], H.t, nb.h[1], H.u1, nb.h[3],
                                                      H.u3, oy[0], oy[1], oy[2], K);
-                if (yface_store) store_face<1>(A, at(g, i, j, p), o);
+                if (yface_store) store_face<1, Rec>(A, at(g, i, j, p), o);
                 my = new_flux<1>(o);
             }
             __syncthreads();
@@ -467,11 +523,34 @@ hipError_t run_fused(void* p) {
     const dim3 grid(c->fused_nunits);
     const int4* units = (const int4*)c->fused_units;
     const int nbx = (c->geo.N1 + TX - 1) / TX;
-    if (c->prm.scheme == ARK_SCHEME_REFERENCE)
-        hipLaunchKernelGGL(k_fused_step<true>, grid, dim3(NTHR), 0, c->stream, a, units, nbx);
-    else
-        hipLaunchKernelGGL(k_fused_step<false>, grid, dim3(NTHR), 0, c->stream, a, units, nbx);
+    const bool q = c->prm.scheme == ARK_SCHEME_REFERENCE;
+    auto kern = c->fused_rec ? (q ? k_fused_step<true, true> : k_fused_step<false, true>)
+                             : (q ? k_fused_step<true, false> : k_fused_step<false, false>);
+    hipLaunchKernelGGL(kern, grid, dim3(NTHR), 0, c->stream, a, units, nbx);
     return hipGetLastError();
+}
+
+// One buffer of a group between the per-field layout and the record layout
+// (grec), index for index over the n indices of the common index map.
+__global__ void k_group_layout(const char* __restrict__ src, char* __restrict__ dst, unsigned long long fb, long long n,
+                               int to_records) {
+    const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (to_records) {
+        double v[4];
+        for (int f = 0; f < 4; ++f) v[f] = *(const double*)(src + f * fb + q * 8);
+        double2* w = (double2*)(dst + 8 + q * 32);
+        w[0] = make_double2(v[0], v[1]);
+        w[1] = make_double2(v[2], v[3]);
+    } else {
+        const double2* r = (const double2*)(src + 8 + q * 32);
+        const double2 a = r[0], b = r[1];
+        *(double*)(dst + 0 * fb + q * 8) = a.x;
+        *(double*)(dst + 1 * fb + q * 8) = a.y;
+        *(double*)(dst + 2 * fb + q * 8) = b.x;
+        *(double*)(dst + 3 * fb + q * 8) = b.y;
+    }
+    *(double*)(dst + 4 * fb + q * 8) = *(const double*)(src + 4 * fb + q * 8);
 }
 
 // The chunk table of the FP64 fused step, built once per context for the
@@ -481,8 +560,8 @@ ark_status ensure_fused_units(ark_ctx* c) {
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
-    const void* fn = c->prm.scheme == ARK_SCHEME_REFERENCE ? (const void*)k_fused_step<true>
-                                                          : (const void*)k_fused_step<false>;
+    const void* fn = c->prm.scheme == ARK_SCHEME_REFERENCE ? (const void*)k_fused_step<true, true>
+                                                          : (const void*)k_fused_step<false, true>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NTHR, 0) != hipSuccess || per_cu < 1) per_cu = 1;
     return ark_upload_chunks(c, ark_chunk_table(c->geo, TX, TY, cus * per_cu), &c->fused_units, &c->fused_nunits);
 }
@@ -494,6 +573,48 @@ bool ark_fused_eligible(const ark_ctx* c) {
            c->prm.precision == ARK_PREC_FP64 && c->prm.periodic[0] && c->prm.periodic[1] && c->prm.periodic[2];
 }
 
+// Every index of both buffers of every group is converted: current -> spare,
+// other -> current, and the three buffers rotate.  Each buffer then holds in
+// the new layout exactly what it held in the old one, ghosts and entries a
+// step does not write included, so the fused step's results do not depend on
+// the layout.  Cost at 256^3: two conversions of four groups, about 2.5 ms.
+ark_status ark_fused_layout(ark_ctx* c, bool records) {
+    if (c->prm.precision != ARK_PREC_FP64 || c->fused_rec == records) return ARK_OK;
+    if (!c->group_spare) {
+        char* mem = nullptr;
+        const size_t nb = 5 * c->field_bytes;
+        ark_status s = c->hip(hipMalloc(&mem, nb), "hipMalloc spare group");
+        if (s != ARK_OK) return s;
+        c->allocs.push_back(mem);
+        c->bytes += nb;
+        s = c->hip(hipMemsetAsync(mem, 0, nb, c->stream), "hipMemset spare group");
+        if (s != ARK_OK) return s;
+        c->group_spare = mem + kOriginPad * sizeof(double);
+    }
+    const long long n = (long long)c->geo.sxy * (c->geo.N3 + 3);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    for (int g = 0; g < 4; ++g) {
+        char* cur = c->gbase[g][c->gpar[g]];
+        char* oth = c->gbase[g][c->gpar[g] ^ 1];
+        char* spare = c->group_spare;
+        hipLaunchKernelGGL(k_group_layout, grid, dim3(256), 0, c->stream, (const char*)cur, spare, c->field_bytes, n,
+                           records ? 1 : 0);
+        hipLaunchKernelGGL(k_group_layout, grid, dim3(256), 0, c->stream, (const char*)oth, cur, c->field_bytes, n,
+                           records ? 1 : 0);
+        ark_status s = c->hip(hipGetLastError(), records ? "groups to records" : "groups to fields");
+        if (s != ARK_OK) return s;
+        ark_set_group_buffers(c, g, spare, cur);
+        c->group_spare = oth;
+    }
+    c->fused_rec = records;
+    return ARK_OK;
+}
+
+// A run of at least this many fused steps converts the groups to the record
+// layout first (two conversions cost about as much as 9 steps save); shorter
+// runs use it only if a previous run left it.
+constexpr int kRecordSteps = 32;
+
 // One fused step per iteration: begin the step (dt), run the kernel, swap
 // the double-buffered persistent fields.  The kernel leaves the next step's
 // dt as the running MIN in S->dt_bits (dt_pending).
@@ -501,6 +622,7 @@ static ark_status fused_steps(ark_ctx* c, int nsteps) {
     ark_status s = ensure_fused_units(c);
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
+        if (s == ARK_OK && n == 0 && nsteps >= kRecordSteps) s = ark_fused_layout(c, true);
         if (s == ARK_OK) s = c->launch("fused_step", run_fused, c);
         if (s != ARK_OK) break;
         for (int g = 0; g < 4; ++g) ark_swap_group(c, g);

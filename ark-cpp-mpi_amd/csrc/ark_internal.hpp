@@ -79,6 +79,8 @@ struct ark_ctx {
     bool rho_stored = true;            // the stored face-density arrays are current
     bool dt_pending = false;           // S->dt_bits holds the next step's dt from a fused step
     bool cells_rec = false;            // mixed: the level-n cells are in the stepping layout (ark_cells_layout)
+    bool fused_rec = false;            // FP64: the four groups are in the record layout (ark_fused_layout)
+    char* group_spare = nullptr;       // FP64: a spare group buffer for the layout conversions (lazily allocated)
     bool sigma_dirty = false;          // an upload left non-zero values in a stress array
     double** sigma_ptrs = nullptr;     // device array: allocation bases of the nine stress arrays
     // Non-finite detection: the stepping calls copy Scalars::nf into nf_host
@@ -136,6 +138,11 @@ void ark_force_batch_free(ark_force_batch* f);
 extern const int kGroupFields[4][5];
 // Exchange the current and the other buffer of group g.
 void ark_swap_group(ark_ctx* c, int g);
+// Make cur the current and alt the other buffer of group g (its field pointers too).
+void ark_set_group_buffers(ark_ctx* c, int g, char* cur, char* alt);
+// Every field in the per-field layout, whatever a stepping call left them in:
+// call before reading or writing any field per field.
+ark_status ark_per_field(ark_ctx* c);
 
 // Mixed precision (ark_mixed.hip): fused steps, and the conversion of one
 // face field between its FP32 record storage and an FP64 field in the common
@@ -162,4 +169,8 @@ ark_status ark_upload_chunks(ark_ctx* c, const std::vector<int4>& u, void** dst,
 
 // The fused step (ark_fused.hip).
 ark_status ark_step_fused(ark_ctx* c, int nsteps);
+// The FP64 fused step's record layout of all four groups (fields 0-3 of an
+// index as one 32-byte record, field 4 per field), or back; both buffers of
+// every group are converted index for index, through group_spare.
+ark_status ark_fused_layout(ark_ctx* c, bool records);
 bool ark_fused_eligible(const ark_ctx* c);

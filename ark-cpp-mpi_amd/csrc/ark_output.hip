@@ -90,7 +90,7 @@ extern "C" {
 
 ark_status ark_average_values(ark_ctx* c) {
     if (!c) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     if (!c->average) {
         double* p = nullptr;
@@ -109,7 +109,7 @@ ark_status ark_average_values(ark_ctx* c) {
 
 ark_status ark_write_paraview(ark_ctx* c, const char* dir_path, int need_swap) {
     if (!c || !dir_path) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     Scalars sc;
     ark_status s = scalars_of(c, sc);

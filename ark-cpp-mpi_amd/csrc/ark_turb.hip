@@ -518,7 +518,7 @@ dim3 interior_grid(const Geo& g, int ext) {
 // real field) are work.
 ark_status install_velocity(ark_ctx* c, Fft& f, const hipfftDoubleComplex* U, hipfftDoubleComplex* W, double* R,
                             double scale, const hipfftDoubleComplex* P = nullptr) {
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     const long long nc = (long long)f.s.H1 * g.N2 * g.N3;
@@ -832,7 +832,7 @@ ark_status ark_set_closure_cutoff(ark_ctx* c, int kcut) {
 
 ark_status ark_init_state(ark_ctx* c, const double* u, const double* rho) {
     if (!c || !u) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "init_velocity: N must be even");
@@ -868,7 +868,7 @@ ark_status ark_init_velocity(ark_ctx* c, const double* u) { return ark_init_stat
 
 ark_status ark_filter_density(ark_ctx* c, int width, double* rhobar) {
     if (!c || width < 1 || !rhobar) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if (g.N1 % width || g.N2 % width || g.N3 % width)
@@ -888,7 +888,7 @@ ark_status ark_filter_density(ark_ctx* c, int width, double* rhobar) {
 
 ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, double urms) {
     if (!c || !(k0 > 0.0) || !(urms >= 0.0)) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if ((g.N1 & 1) || (g.N2 & 1) || (g.N3 & 1)) return c->fail(ARK_ERR_UNSUPPORTED, "turbulence: N must be even");
@@ -931,7 +931,7 @@ ark_status ark_init_turbulence(ark_ctx* c, unsigned long long seed, double k0, d
 
 ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
     if (!c || nbins < 1 || !E) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     Fft f;
@@ -962,7 +962,7 @@ ark_status ark_spectrum(ark_ctx* c, int nbins, double* E) {
 
 ark_status ark_dissipation(ark_ctx* c, double* eps) {
     if (!c || !eps) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     DevBuf acc;
     ark_status s = c->hip(hipMalloc(&acc.p, (kSumBlocks + 1) * sizeof(double)), "hipMalloc");
@@ -1013,7 +1013,7 @@ ark_status ark_set_forcing(ark_ctx* c, unsigned long long seed, double eps, doub
 
 ark_status ark_filter_box(ark_ctx* c, int width, double* ubar, double* tau, double* grad) {
     if (!c || width < 1 || !ubar || !tau || !grad) return ARK_ERR_INVALID;
-    ark_status lay = ark_cells_layout(c, false);   // cells in the per-field layout
+    ark_status lay = ark_per_field(c);   // every field in the per-field layout
     if (lay != ARK_OK) return lay;
     const Geo& g = c->geo;
     if (g.N1 % width || g.N2 % width || g.N3 % width)

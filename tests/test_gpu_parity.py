@@ -220,6 +220,51 @@ def test_fused_equals_phases(N):
         assert da["dt"] == db["dt"] and da["time"] == db["time"] and da["nstep"] == db["nstep"] == 7
 
 
+@pytest.mark.parametrize("N", [(32, 32, 32), (20, 13, 7)])
+def test_fused_equals_phases_record_layout(N):
+    """A call with 32 or more fused steps first converts the four field groups
+    to 32-byte records (fields 0-3 of an index; field 4 per field), both
+    buffers index for index.  40 steps that way equal 40 phase-by-phase steps
+    on every one of the 41 arrays."""
+    with arkamd.Solver(*N) as a, arkamd.Solver(*N) as b:
+        a.init_tgv()
+        b.init_tgv()
+        a.step(40)
+        b.step_phases(40)
+        ga, gb = a.download_all(), b.download_all()
+        bad = [describe_mismatch(n, ga[n], gb[n]) for n in FIELDS if not bits_equal(ga[n], gb[n])]
+        assert not bad, "\n".join(bad)
+        da, db = a.diagnostics(), b.diagnostics()
+        assert da["dt"] == db["dt"] and da["time"] == db["time"] and da["nstep"] == db["nstep"] == 40
+
+
+def test_record_layout_is_exact_everywhere():
+    """The record layout changes no bit anywhere: ark_advance in one run of 45
+    (records) and in runs of 9 (per field) leave all 41 arrays equal, ghosts
+    and the entries a fused step does not write included.  So do a following
+    ark_step(36) (35 fused steps in records, after a download between the two
+    record runs, then one step phase by phase) and the same 35 fused steps in
+    runs below the threshold followed by ark_step(1)."""
+    N = (20, 13, 7)
+    with arkamd.Solver(*N) as a, arkamd.Solver(*N) as b:
+        a.init_tgv()
+        b.init_tgv()
+        a.advance(45)
+        for _ in range(5):
+            b.advance(9)
+        for what in ("advance", "step"):
+            ga, gb = a.download_all(), b.download_all()
+            bad = [describe_mismatch(n, ga[n], gb[n]) for n in FIELDS if not bits_equal(ga[n], gb[n])]
+            assert not bad, what + "\n" + "\n".join(bad)
+            if what == "advance":
+                a.step(36)
+                for k in (9, 9, 9, 8):
+                    b.advance(k)
+                b.step(1)
+        da, db = a.diagnostics(), b.diagnostics()
+        assert da["time"] == db["time"] and da["nstep"] == db["nstep"] == 81
+
+
 def test_fused_equals_phases_random_state():
     st = random_fluid_state((19, 11, 6), 99, dt=0.004)
     with solver_for(st) as a, solver_for(st) as b:
