@@ -562,7 +562,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=40,
+                    help="fused steps timed per kernel with HIP events (32 or more: the FP64 record layout)")
     ap.add_argument("--accuracy-steps", type=int, default=500)
     ap.add_argument("--channel-steps", type=int, default=2000, help="steps of the reference's own channel (0: skip)")
     ap.add_argument("--closure-size", type=int, default=512, help="fine grid of the closure experiment (0: skip)")
