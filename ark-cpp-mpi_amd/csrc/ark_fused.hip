@@ -577,7 +577,8 @@ bool ark_fused_eligible(const ark_ctx* c) {
 // other -> current, and the three buffers rotate.  Each buffer then holds in
 // the new layout exactly what it held in the old one, ghosts and entries a
 // step does not write included, so the fused step's results do not depend on
-// the layout.  Cost at 256^3: two conversions of four groups, about 2.5 ms.
+// the layout.  Cost at 256^3: two conversions of each of the four groups,
+// about 2.1 ms.
 ark_status ark_fused_layout(ark_ctx* c, bool records) {
     if (c->prm.precision != ARK_PREC_FP64 || c->fused_rec == records) return ARK_OK;
     if (!c->group_spare) {
@@ -611,8 +612,8 @@ ark_status ark_fused_layout(ark_ctx* c, bool records) {
 }
 
 // A run of at least this many fused steps converts the groups to the record
-// layout first (two conversions cost about as much as 9 steps save); shorter
-// runs use it only if a previous run left it.
+// layout first (a switch there and back, about 4.2 ms at 256^3, costs what
+// 15 record steps save); shorter runs use it only if a previous run left it.
 constexpr int kRecordSteps = 32;
 
 // One fused step per iteration: begin the step (dt), run the kernel, swap
