@@ -581,14 +581,16 @@ bool ark_fused_eligible(const ark_ctx* c) {
 // about 2.1 ms.
 ark_status ark_fused_layout(ark_ctx* c, bool records) {
     if (c->prm.precision != ARK_PREC_FP64 || c->fused_rec == records) return ARK_OK;
-    if (!c->group_spare) {
+    if (!c->group_spare) {   // only ever missing on the way to records
         char* mem = nullptr;
         const size_t nb = 5 * c->field_bytes;
-        ark_status s = c->hip(hipMalloc(&mem, nb), "hipMalloc spare group");
-        if (s != ARK_OK) return s;
+        if (hipMalloc(&mem, nb) != hipSuccess) {
+            (void)hipGetLastError();
+            return ARK_ERR_UNSUPPORTED;   // no room for the spare: the caller stays per field
+        }
         c->allocs.push_back(mem);
         c->bytes += nb;
-        s = c->hip(hipMemsetAsync(mem, 0, nb, c->stream), "hipMemset spare group");
+        ark_status s = c->hip(hipMemsetAsync(mem, 0, nb, c->stream), "hipMemset spare group");
         if (s != ARK_OK) return s;
         c->group_spare = mem + kOriginPad * sizeof(double);
     }
@@ -623,7 +625,12 @@ static ark_status fused_steps(ark_ctx* c, int nsteps) {
     ark_status s = ensure_fused_units(c);
     for (int n = 0; s == ARK_OK && n < nsteps; ++n) {
         s = ark_begin_step(c);
-        if (s == ARK_OK && n == 0 && nsteps >= kRecordSteps) s = ark_fused_layout(c, true);
+        if (s == ARK_OK && n == 0 && nsteps >= kRecordSteps) {
+            // the record layout is only faster: without memory for its spare
+            // buffer (ARK_ERR_UNSUPPORTED) the run stays on the per-field one
+            const ark_status r = ark_fused_layout(c, true);
+            if (r != ARK_ERR_UNSUPPORTED) s = r;
+        }
         if (s == ARK_OK) s = c->launch("fused_step", run_fused, c);
         if (s != ARK_OK) break;
         for (int g = 0; g < 4; ++g) ark_swap_group(c, g);

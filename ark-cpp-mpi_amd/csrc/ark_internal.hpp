@@ -172,5 +172,7 @@ ark_status ark_step_fused(ark_ctx* c, int nsteps);
 // The FP64 fused step's record layout of all four groups (fields 0-3 of an
 // index as one 32-byte record, field 4 per field), or back; both buffers of
 // every group are converted index for index, through group_spare.
+// ARK_ERR_UNSUPPORTED, with nothing changed, when group_spare cannot be
+// allocated on the way to records.
 ark_status ark_fused_layout(ark_ctx* c, bool records);
 bool ark_fused_eligible(const ark_ctx* c);
