@@ -180,7 +180,7 @@ def test_tgv_digest_256(how):
         assert s.diagnostics()["nstep"] == 5
 
 
-@pytest.mark.parametrize("how", ["step", "advance"])
+@pytest.mark.parametrize("how", ["step", "advance", "records"])
 def test_tgv_digest_256_long(how):
     """Fifty steps of the 256^3 workload against the unmodified reference on
     eight ranks (dims 4x2x1): the dt of every tenth step and, at the end, the
@@ -190,14 +190,19 @@ def test_tgv_digest_256_long(how):
     so on eight ranks its t differs from one rank's, which this GPU run is
     (the 5-step run above records both).  `advance`: fused steps only;
     `step`: each call of ten runs nine fused steps and one through the phase
-    kernels."""
+    kernels; `records`: ark_step(50) in one call, 49 fused steps in the
+    record layout, then one through the phase kernels (only its final dt is
+    compared)."""
     run = json.load(open(os.path.join(GOLDEN, "tgv_digests.json")))["runs"]["256^3x50_interior"]
     want_dt = [float.fromhex(v) for v in run["series"]["dt"][1:]]
     assert len(want_dt) == 50
     N = 256
     with arkamd.Solver(N, N, N) as s:
         s.init_tgv()
-        for done in range(10, 51, 10):
+        if how == "records":
+            s.step(50)
+            assert s.dt == want_dt[49]
+        for done in range(10, 51, 10) if how != "records" else ():
             s.step(10) if how == "step" else s.advance(10)
             assert s.dt == want_dt[done - 1], done
         for n in ("ronCon", "u1nCon", "u2nCon", "u3nCon"):
